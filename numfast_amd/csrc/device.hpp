@@ -1,0 +1,126 @@
+// device.hpp -- wave64 / workgroup primitives shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nfh {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+__device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (uint64_t(1) << lane_id()) - 1;
+}
+
+// Lanes of this wave whose 8-bit digit equals mine (restricted to `valid`
+// lanes).  Eight ballots, one per digit bit; every lane must execute it.
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return peers;
+}
+
+// Inclusive scan of a 32-bit value across the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    uint32_t t = __shfl_up(x, off);
+    if (lane >= off) x += t;
+  }
+  return x;
+}
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    uint64_t t = __shfl_up(x, off);
+    if (lane >= off) x += t;
+  }
+  return x;
+}
+
+// Exclusive scan across a workgroup of NW waves; `s_tmp` holds NW words.
+// Every thread must call it.  Returns the exclusive prefix, sets *total.
+template <int NW>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
+                                                    uint32_t* total) {
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane_id() == WAVE - 1) s_tmp[wave_id()] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t c = s_tmp[w];
+    if (w < wave_id()) off += c;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+template <int NW>
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tmp,
+                                                      uint64_t* total) {
+  const uint64_t inc = wave_incl_scan64(v);
+  if (lane_id() == WAVE - 1) s_tmp[wave_id()] = inc;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint64_t c = s_tmp[w];
+    if (w < wave_id()) off += c;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// ---------------------------------------------------------------- look-back
+// A look-back word: [63:34] epoch (30 bits), [33:32] status, [31:0] value.
+// The epoch tags each pass/call so stale words from earlier passes read as
+// "not ready" and the buffer never needs clearing.
+constexpr uint64_t LB_AGG = 1;
+constexpr uint64_t LB_PREFIX = 2;
+
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t status, uint32_t v) {
+  return (uint64_t(epoch) << 34) | (status << 32) | uint64_t(v);
+}
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t w) {
+  __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Walk back from tile `tile-1` summing aggregates until a prefix is found.
+// `stride` is the distance between consecutive tiles' words.
+__device__ __forceinline__ uint32_t lb_lookback(const uint64_t* lb, uint32_t tile,
+                                                uint32_t stride, uint32_t epoch) {
+  uint32_t excl = 0;
+  int64_t j = int64_t(tile) - 1;
+  while (j >= 0) {
+    const uint64_t w = lb_load(lb + size_t(j) * stride);
+    const uint32_t ep = uint32_t(w >> 34);
+    const uint32_t st = uint32_t(w >> 32) & 3u;
+    if (ep != epoch || st == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += uint32_t(w);
+    if (st == LB_PREFIX) break;
+    --j;
+  }
+  return excl;
+}
+
+}  // namespace nfh

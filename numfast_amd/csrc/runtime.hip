@@ -1,0 +1,214 @@
+// runtime.hip -- per-thread device context, scratch pool and staging.
+#include <mutex>
+
+#include "runtime.hpp"
+
+namespace nfh {
+
+// Context memory is intentionally not released at thread exit: thread-local
+// destructors of the main thread can run after the HIP runtime has been torn
+// down, and a hipFree at that point is unsafe.  Worker threads of a pool live
+// for the process, so the pool is bounded by (threads x high-water mark).
+Ctx::~Ctx() {}
+
+Ctx& ctx() {
+  static thread_local Ctx c;
+  return c;
+}
+
+namespace {
+std::once_flag g_init_once;
+hipError_t g_init_err = hipSuccess;
+int g_cus = 0;
+}  // namespace
+
+void ensure_device() {
+  std::call_once(g_init_once, [] {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+      g_init_err = e;
+      return;
+    }
+    if (n <= 0) {
+      g_init_err = hipErrorNoDevice;
+      return;
+    }
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+      g_init_err = e;
+      return;
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) {
+      g_init_err = e;
+      return;
+    }
+    g_cus = prop.multiProcessorCount;
+  });
+  if (g_init_err != hipSuccess) throw DeviceError{g_init_err, "device init"};
+}
+
+int num_cus() {
+  ensure_device();
+  return g_cus > 0 ? g_cus : 256;
+}
+
+hipStream_t stream() {
+  Ctx& c = ctx();
+  if (c.stream == nullptr && !c.stream_set_by_user) {
+    // Blocking stream at default priority: orders after work on the legacy
+    // null stream (where torch's default-stream kernels run).
+    NFH_CHECK(hipStreamCreate(&c.stream));
+  }
+  return c.stream;
+}
+
+uint32_t next_epoch() {
+  Ctx& c = ctx();
+  c.epoch = (c.epoch + 1) & 0x3FFFFFFFu;
+  if (c.epoch == 0) c.epoch = 1;
+  return c.epoch;
+}
+
+void* scratch(int id, size_t bytes) {
+  Ctx& c = ctx();
+  if (bytes == 0) bytes = 256;
+  if (c.cap[id] < bytes) {
+    if (c.slot[id] != nullptr) {
+      // The stream may still use the old buffer.
+      NFH_CHECK(hipStreamSynchronize(stream()));
+      NFH_CHECK(hipFree(c.slot[id]));
+      c.slot[id] = nullptr;
+      c.cap[id] = 0;
+    }
+    size_t want = bytes + bytes / 8;  // headroom for slowly growing sizes
+    want = (want + 4095) & ~size_t(4095);
+    void* p = nullptr;
+    NFH_CHECK(hipMalloc(&p, want));
+    c.slot[id] = p;
+    c.cap[id] = want;
+  }
+  return c.slot[id];
+}
+
+void* pinned(size_t bytes) {
+  Ctx& c = ctx();
+  if (c.pinned_cap < bytes) {
+    if (c.pinned != nullptr) {
+      NFH_CHECK(hipStreamSynchronize(stream()));
+      NFH_CHECK(hipHostFree(c.pinned));
+      c.pinned = nullptr;
+      c.pinned_cap = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    NFH_CHECK(hipHostMalloc(&c.pinned, want, hipHostMallocDefault));
+    c.pinned_cap = want;
+  }
+  return c.pinned;
+}
+
+bool is_device_ptr(const void* p) {
+  if (p == nullptr) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory: clear the status
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+         a.type == hipMemoryTypeUnified;
+}
+
+Stage::Stage() {
+  ensure_device();
+  s = stream();
+}
+
+const void* Stage::in_raw(const void* p, size_t bytes, int slot) {
+  if (p == nullptr || bytes == 0) return p;
+  if (is_device_ptr(p)) return p;
+  void* d = scratch(slot, bytes);
+  NFH_CHECK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, s));
+  return d;
+}
+
+void* Stage::out_raw(void* p, size_t bytes, int slot, bool copy_in) {
+  if (p == nullptr || bytes == 0) return p;
+  if (is_device_ptr(p)) return p;
+  void* d = scratch(slot, bytes);
+  if (copy_in) NFH_CHECK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, s));
+  pend_.push_back({p, d, bytes});
+  return d;
+}
+
+void Stage::set_out_bytes(const void* host_p, size_t bytes) {
+  for (auto& q : pend_)
+    if (q.host == host_p && bytes < q.bytes) q.bytes = bytes;
+}
+
+void Stage::finish() {
+  for (auto& q : pend_)
+    if (q.bytes > 0)
+      NFH_CHECK(hipMemcpyAsync(q.host, q.dev, q.bytes, hipMemcpyDeviceToHost, s));
+  pend_.clear();
+  NFH_CHECK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count) {
+  size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t count) {
+  size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+void fill_u32(void* p, uint32_t v, size_t count, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s,
+                     static_cast<uint32_t*>(p), v, count);
+  NFH_LAUNCH_CHECK();
+}
+void fill_u64(void* p, uint64_t v, size_t count, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s,
+                     static_cast<uint64_t*>(p), v, count);
+  NFH_LAUNCH_CHECK();
+}
+
+}  // namespace nfh
+
+// ------------------------------------------------------------ additive ABI
+// Run this thread's calls on a caller-owned stream (nullptr restores the
+// library's own per-thread stream).  No reference counterpart: a
+// device-resident caller uses it to order library work after its own.
+NF_EXPORT int32_t nfh_set_stream(void* stream) {
+  nfh::Ctx& c = nfh::ctx();
+  if (stream == nullptr) {
+    c.stream_set_by_user = false;
+    c.stream = nullptr;
+  } else {
+    c.stream = static_cast<hipStream_t>(stream);
+    c.stream_set_by_user = true;
+  }
+  return 0;
+}
+
+// 1 when a GPU is usable by this library, 0 otherwise (never throws).
+NF_EXPORT int32_t nfh_device_ok(void) {
+  try {
+    nfh::ensure_device();
+    return 1;
+  } catch (...) {
+    return 0;
+  }
+}
+
+// ABI version of this library's additive surface.
+NF_EXPORT int32_t nfh_abi_version(void) { return 1; }

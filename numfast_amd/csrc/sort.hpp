@@ -1,0 +1,30 @@
+// sort.hpp -- device-pointer radix argsort shared by sort, unique and the
+// sort-based pipelines.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include <type_traits>
+
+namespace nfh {
+
+template <class K>
+using word_t = typename std::conditional<sizeof(K) == 4, uint32_t, uint64_t>::type;
+
+// Stable argsort of device keys into device `perm` (int32 positions).
+// W = uint32_t for int32 keys, uint64_t for int64 keys; t0/t1/tp are device
+// scratch of n elements each.  When `sorted_out` is not null the keys are
+// also written in sorted order (it may alias t0).  Asynchronous on `s`
+// except for one small readback that decides which digit passes to skip.
+template <class K>
+void radix_argsort(const K* keys, size_t n, bool desc, int32_t* perm, word_t<K>* t0,
+                   word_t<K>* t1, int32_t* tp, K* sorted_out, hipStream_t s);
+
+// Look-back scratch of `words` 64-bit words (zeroed whenever it grows) and
+// a fresh epoch for one decoupled look-back pass.
+uint64_t* lookback_words(size_t words, hipStream_t s);
+uint32_t lookback_epoch(hipStream_t s);
+
+}  // namespace nfh

@@ -1,0 +1,524 @@
+"""ctypes binding of libnumfast_hip.so -- the host mirror of NumFast's
+native boundary (src/Drivers/CPU/_lib/native_cpu.py).
+
+Same loader contract as the reference:
+
+* the library path comes from ``NUMFAST_NATIVE_DLL`` (default: the in-tree
+  ``numfast_amd/libnumfast_hip.so``), ``NUMFAST_NATIVE_DISABLE=1`` disables
+  it (native_cpu.py:93-143);
+* the 8 eager symbols get their argtypes at load time and all must exist
+  (native_cpu.py:125-140); each optional group (variant, mixed, owner,
+  fused, unique, sort) is bound on first use and fails independently
+  (native_cpu.py:324-352, :503-532, :548-574, :621-651, :1399-1431,
+  :1546-1576);
+* return codes map to the same exceptions (-4 OverflowError, -2
+  RuntimeError "dense key outside", anything else ``assert rc == 0``).
+
+One deliberate difference: there is NO NumPy fallback.  When the library
+is missing or disabled the wrappers raise :class:`NativeUnavailable`; the
+HIP path is the only compute path of this package.
+
+The wrappers accept NumPy arrays (host memory: the library stages them
+through device memory) and, for the ``*_device`` helpers, torch tensors
+already resident on the GPU (pointers are passed through untouched).
+"""
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+DEFAULT_LIB = str(_HERE / "libnumfast_hip.so")
+
+_vp, _sz, _i32, _i64, _u8, _u32 = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32,
+                                   ctypes.c_int64, ctypes.c_uint8, ctypes.c_uint32)
+
+# (name, argtypes, restype) -- native_cpu.py:125-140 (the eager 8).
+EAGER = [
+    ("nf_group_sum_count", [_vp] * 2 + [_sz] + [_vp] * 2 + [_sz], _i32),
+    ("nf_group_multi_sum_count", [_vp] * 2 + [_sz] * 2 + [_vp] * 2 + [_sz], _i32),
+    ("nf_pack_i32_direct", [_vp] * 2 + [_i32, _sz, _vp], _i32),
+    ("nf_pattern_encode", [_vp, _sz] * 3 + [_vp] * 4, _i32),
+    ("nf_sorted_run_i64", [_vp] * 2 + [_sz] + [_vp] * 3, _i64),
+    ("nf_sorted_run_f64", [_vp] * 2 + [_sz] + [_vp] * 3, _i64),
+    ("nf_carry_build_i64", [_vp] * 2 + [_sz] + [_vp] * 3, _i64),
+    ("nf_carry_build_f64", [_vp] * 2 + [_sz] + [_vp] * 3, _i64),
+]
+
+# Optional groups, each all-or-nothing (native_cpu.py, lines in the doc).
+GROUPS = {
+    "variant": [
+        ("nf_group_variant_i64", [_vp, _vp, _sz, _u32, _vp, _vp, _sz], _i32),
+        ("nf_group_variant_i32", [_vp, _vp, _sz, _u32, _vp, _vp, _sz], _i32),
+        # lib.rs:539 -- exported by the reference, not bound by its Python
+        ("nf_group_variant_f64", [_vp, _vp, _sz, _u32, _vp, _vp, _sz], _i32),
+    ],
+    "mixed": [
+        ("nf_group_mixed_sum_only", [_vp, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _sz], _i32),
+        ("nf_group_count_only", [_vp, _sz, _vp, _sz], _i32),
+    ],
+    "owner": [
+        ("nf_group_owner_2i32_1f64", [_vp] * 4 + [_sz] * 3 + [_vp] * 4 + [_sz], _i32),
+    ],
+    "fused": [
+        ("nf_pack_sum_count_i32", [_vp, _vp, _i32, _vp, _sz, _vp, _vp, _sz], _i32),
+        ("nf_pack_sum_count_f64", [_vp, _vp, _i32, _vp, _sz, _vp, _vp, _sz], _i32),
+    ],
+    "unique": [
+        ("nf_unique_inverse_i32", [_vp, _sz] + [_vp] * 6, _i64),
+        ("nf_unique_inverse_i64", [_vp, _sz] + [_vp] * 6, _i64),
+    ],
+    "sort": [
+        ("nf_sort_perm_i32", [_vp, _sz, _u8] + [_vp] * 4, _i32),
+        ("nf_sort_perm_i64", [_vp, _sz, _u8] + [_vp] * 4, _i32),
+    ],
+    # Additive surface of this library (no reference counterpart).
+    "hip": [
+        ("nfh_set_stream", [_vp], _i32),
+        ("nfh_device_ok", [], _i32),
+        ("nfh_abi_version", [], _i32),
+    ],
+}
+
+
+class NativeUnavailable(RuntimeError):
+    """The HIP library is missing, disabled or lacks a symbol group."""
+
+
+def bind(lib, groups=None):
+    """Set argtypes/restype on a loaded CDLL; returns the set of bound groups.
+
+    Raises AttributeError when an eager symbol is missing (the reference's
+    all-or-nothing rule).  Optional groups that are incomplete are skipped.
+    """
+    for name, args, res in EAGER:
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    ok = set()
+    for g, syms in GROUPS.items():
+        if groups is not None and g not in groups:
+            continue
+        try:
+            for name, args, res in syms:
+                f = getattr(lib, name)
+                f.argtypes = args
+                f.restype = res
+            ok.add(g)
+        except AttributeError:
+            pass
+    return ok
+
+
+_lib = None
+_groups = set()
+_why = "unprobed"
+_probe_env = None
+
+
+def _probe():
+    global _lib, _groups, _why, _probe_env
+    key = (os.environ.get("NUMFAST_NATIVE_DISABLE"),
+           os.environ.get("NUMFAST_NATIVE_DLL", DEFAULT_LIB))
+    if key == _probe_env and _why != "unprobed":
+        return
+    _probe_env = key
+    if key[0] == "1":
+        _lib, _groups, _why = None, set(), "disabled-by-env"
+        return
+    try:
+        lib = ctypes.CDLL(key[1])
+        _groups = bind(lib)
+        _lib, _why = lib, "loaded:" + key[1]
+    except (OSError, AttributeError) as e:
+        _lib, _groups, _why = None, set(), "unavailable:%s" % e
+
+
+def available():
+    """True when the library is loaded (env-aware, like native_cpu.available)."""
+    _probe()
+    return _lib is not None
+
+
+def why():
+    _probe()
+    return _why
+
+
+def lib(group=None):
+    """The loaded CDLL, or NativeUnavailable (no silent fallback)."""
+    _probe()
+    if _lib is None:
+        raise NativeUnavailable("libnumfast_hip %s" % _why)
+    if group is not None and group not in _groups:
+        raise NativeUnavailable("libnumfast_hip lacks the %r symbol group" % group)
+    return _lib
+
+
+def loaded_path():
+    _probe()
+    return _why.split(":", 1)[1] if _why.startswith("loaded:") else None
+
+
+def _p(a):
+    """Data pointer of a NumPy array or a torch tensor (None for None)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ------------------------------------------------------------ group-by lanes
+def i32_gate_ok(vals, n=None, _b=None):
+    """int32-direct gate (native_cpu.py:360-379): n*maxabs < 2**31-1."""
+    vals = np.asarray(vals)
+    if _b is None:
+        if vals.size == 0:
+            return True
+        mn, mx = int(vals.min()), int(vals.max())
+    else:
+        mn, mx = _b
+    nn = vals.size if n is None else int(n)
+    return nn * max(abs(mn), abs(mx)) < 2 ** 31 - 1
+
+
+def fused_sum_count(keys, vals, g):
+    """native_cpu.py:791-802 -> nf_group_sum_count."""
+    keys, vals = _c(keys, np.int32), _c(vals, np.float64)
+    L = lib()
+    sums = np.zeros(g)
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_group_sum_count(_p(keys), _p(vals), keys.size, _p(sums), _p(counts), g)
+    assert rc == 0, rc
+    return sums, counts
+
+
+def multi_sum_count(keys, cols, g):
+    """native_cpu.py:805-818 -> nf_group_multi_sum_count (SoA columns)."""
+    keys = _c(keys, np.int32)
+    cols = [_c(c, np.float64) for c in cols]
+    ncols = len(cols)
+    L = lib()
+    vflat = np.ascontiguousarray(np.stack(cols).reshape(-1)) if ncols else np.zeros(0)
+    sums = np.zeros(ncols * g)
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_group_multi_sum_count(_p(keys), _p(vflat), keys.size, ncols, _p(sums),
+                                    _p(counts), g)
+    assert rc == 0, rc
+    return [sums[i * g:(i + 1) * g] for i in range(ncols)], counts
+
+
+def sum_count_i32(keys, vals, g):
+    """native_cpu.py:394-413 -> nf_group_variant_i32(variant 0)."""
+    keys, vals = _c(keys, np.int32), _c(vals, np.int32)
+    L = lib("variant")
+    sums = np.zeros(g, dtype=np.int32)
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_group_variant_i32(_p(keys), _p(vals), keys.size, 0, _p(sums), _p(counts), g)
+    if rc == -4:
+        raise OverflowError("sum_count_i32: int32 accumulator overflow")
+    assert rc == 0, rc
+    return sums, counts
+
+
+def sum_count_i64(keys, ticks, g):
+    """native_cpu.py:416-445 -> nf_group_variant_i64(variant 1, V_CHECKED)."""
+    keys, ticks = _c(keys, np.int32), _c(ticks, np.int64)
+    L = lib("variant")
+    sums = np.zeros(g, dtype=np.int64)
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_group_variant_i64(_p(keys), _p(ticks), keys.size, 1, _p(sums), _p(counts), g)
+    if rc == -4:
+        raise OverflowError("sum_count_i64: int64 accumulator overflow")
+    if rc == -2:
+        raise RuntimeError("sum_count_i64: dense key outside [0, %d)" % g)
+    assert rc == 0, rc
+    return sums, counts
+
+
+def mixed_sum_count(keys, cols, g, _bounds=None):
+    """native_cpu.py:448-492: per-column dtype routing; None on any lane failure."""
+    try:
+        keys = _c(keys, np.int32)
+        raw = [np.ascontiguousarray(c) for c in cols]
+        n = int(keys.size)
+        outs, counts = [], None
+        for i, c in enumerate(raw):
+            b = None if _bounds is None else _bounds[i]
+            if c.dtype == np.dtype(np.int32):
+                if i32_gate_ok(c, n, _b=b):
+                    s, cc = sum_count_i32(keys, c, g)
+                else:
+                    s, cc = sum_count_i64(keys, _c(c, np.int64), g)
+            elif c.dtype.kind in "iu":
+                s, cc = sum_count_i64(keys, _c(c, np.int64), g)
+            elif c.dtype.kind == "f":
+                s, cc = fused_sum_count(keys, _c(c, np.float64), g)
+            else:
+                return None
+            outs.append(s)
+            if counts is None:
+                counts = cc
+        return outs, counts
+    except (OverflowError, RuntimeError, AssertionError, MemoryError, ValueError):
+        return None
+
+
+def count_only(keys, g):
+    """native_cpu.py:719-729 -> nf_group_count_only."""
+    keys = _c(keys, np.int32)
+    L = lib("mixed")
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_group_count_only(_p(keys), keys.size, _p(counts), g)
+    assert rc == 0, rc
+    return counts
+
+
+def mixed_sum_only_count(keys, cols, g, _bounds=None):
+    """native_cpu.py:732-788: one sum-only pass over i32+f64 lanes + counts."""
+    try:
+        keys = _c(keys, np.int32)
+        raw = [np.ascontiguousarray(c) for c in cols]
+        L = lib("mixed")
+        n = int(keys.size)
+        i32c, f64c, order = [], [], []
+        for i, c in enumerate(raw):
+            b = None if _bounds is None else _bounds[i]
+            if c.dtype == np.dtype(np.int32):
+                if not i32_gate_ok(c, n, _b=b):
+                    return None
+                order.append(("i", len(i32c)))
+                i32c.append(c)
+            elif c.dtype.kind == "f":
+                order.append(("f", len(f64c)))
+                f64c.append(_c(c, np.float64))
+            else:
+                return None
+        ni, nf = len(i32c), len(f64c)
+        i32a = np.ascontiguousarray(np.stack(i32c).reshape(-1)) if ni else None
+        f64a = np.ascontiguousarray(np.stack(f64c).reshape(-1)) if nf else None
+        s32 = np.zeros(ni * g, dtype=np.int32)
+        s64 = np.zeros(nf * g, dtype=np.float64)
+        rc = L.nf_group_mixed_sum_only(_p(keys), _p(i32a), _p(f64a), n, ni, nf,
+                                       _p(s32) if ni else None, _p(s64) if nf else None, g)
+        if rc != 0:
+            return None
+        counts = count_only(keys, g)
+        outs = [None] * len(raw)
+        for j, (k, ci) in enumerate(order):
+            outs[j] = s32[ci * g:(ci + 1) * g] if k == "i" else s64[ci * g:(ci + 1) * g]
+        return outs, counts
+    except (OverflowError, RuntimeError, AssertionError, MemoryError, ValueError):
+        return None
+
+
+def owner_shard_into(keys, a, b, c, lo, hi, m, s1, s2, s3, cc):
+    """native_cpu.py:582-615 -> nf_group_owner_2i32_1f64; True on rc == 0."""
+    try:
+        L = lib("owner")
+        keys, av, bv, cv = _c(keys, np.int32), _c(a, np.int32), _c(b, np.int32), _c(c, np.float64)
+        rc = L.nf_group_owner_2i32_1f64(_p(keys), _p(av), _p(bv), _p(cv), keys.size, int(lo),
+                                        int(hi), _p(s1), _p(s2), _p(s3), _p(cc), int(m))
+        return rc == 0
+    except (OverflowError, RuntimeError, AssertionError, MemoryError, ValueError):
+        return False
+
+
+def pack_sum_count_i32(k1, k2, m2, vals, g):
+    """native_cpu.py:674-695 -> nf_pack_sum_count_i32."""
+    k1, k2, vals = _c(k1, np.int32), _c(k2, np.int32), _c(vals, np.int32)
+    L = lib("fused")
+    sums = np.zeros(g, dtype=np.int32)
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_pack_sum_count_i32(_p(k1), _p(k2), int(m2), _p(vals), k1.size, _p(sums),
+                                 _p(counts), g)
+    if rc == -4:
+        raise OverflowError("pack_sum_count_i32: int32 accumulator overflow")
+    assert rc == 0, rc
+    return sums, counts
+
+
+def pack_sum_count_f64(k1, k2, m2, vals, g):
+    """native_cpu.py:698-716 -> nf_pack_sum_count_f64."""
+    k1, k2, vals = _c(k1, np.int32), _c(k2, np.int32), _c(vals, np.float64)
+    L = lib("fused")
+    sums = np.zeros(g, dtype=np.float64)
+    counts = np.zeros(g, dtype=np.int64)
+    rc = L.nf_pack_sum_count_f64(_p(k1), _p(k2), int(m2), _p(vals), k1.size, _p(sums),
+                                 _p(counts), g)
+    assert rc == 0, rc
+    return sums, counts
+
+
+def pack_i32_direct(k1, k2, m2):
+    """native_cpu.py:1288-1297 -> nf_pack_i32_direct."""
+    k1, k2 = _c(k1, np.int32), _c(k2, np.int32)
+    L = lib()
+    out = np.zeros(k1.size, dtype=np.int32)
+    rc = L.nf_pack_i32_direct(_p(k1), _p(k2), int(m2), k1.size, _p(out))
+    assert rc == 0, rc
+    return out
+
+
+def build_pattern_buffers(strs):
+    """strings -> (data bytes, offs int32) (native_cpu.py:1300-1305)."""
+    enc = [s.encode("utf-8") if isinstance(s, str) else bytes(s) for s in strs]
+    offs = np.zeros(len(enc) + 1, dtype=np.int32)
+    offs[1:] = np.cumsum([len(b) for b in enc]).astype(np.int64)
+    return b"".join(enc), offs
+
+
+def pattern_encode_buffers(data, offs, prefix):
+    """native_cpu.py:1308-1325 -> nf_pattern_encode."""
+    n = offs.size - 1
+    L = lib()
+    p = prefix.encode() if isinstance(prefix, str) else bytes(prefix)
+    dp = np.frombuffer(data + b"\0", dtype=np.uint8)
+    pp = np.frombuffer(p + b"\0", dtype=np.uint8)
+    offs = _c(offs, np.int32)
+    codes = np.zeros(n, dtype=np.int32)
+    valid = np.zeros(n, dtype=np.uint8)
+    w = np.zeros(1, dtype=np.int32)
+    e = np.zeros(1, dtype=np.int32)
+    rc = L.nf_pattern_encode(_p(dp), len(data), _p(offs), n, _p(pp), len(p), _p(codes),
+                             _p(valid), _p(w), _p(e))
+    if rc == -2:
+        raise OverflowError("pattern_encode: int32 overflow at row %d" % int(e[0]))
+    assert rc == 0, (rc, int(e[0]))
+    return codes, valid, int(w[0])
+
+
+def sorted_run(keys, vals):
+    """native_cpu.py:1328-1353 -> nf_sorted_run_i64/_f64."""
+    keys = _c(keys, np.int32)
+    n = keys.size
+    L = lib()
+    if np.asarray(vals).dtype.kind in "iu":
+        vals = _c(vals, np.int64)
+        uk, s, c = (np.zeros(n, dtype=np.int64) for _ in range(3))
+        ng = int(L.nf_sorted_run_i64(_p(keys), _p(vals), n, _p(uk), _p(s), _p(c)))
+    else:
+        vals = _c(vals, np.float64)
+        uk, c = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+        s = np.zeros(n, dtype=np.float64)
+        ng = int(L.nf_sorted_run_f64(_p(keys), _p(vals), n, _p(uk), _p(s), _p(c)))
+    assert ng >= 0, ng
+    return uk[:ng], s[:ng], c[:ng]
+
+
+def carry_build(counts_m, sums_m):
+    """native_cpu.py:1356-1381 -> nf_carry_build_i64/_f64."""
+    counts_m = _c(counts_m, np.int64)
+    m = counts_m.size
+    L = lib()
+    uk, c = np.zeros(m, dtype=np.int64), np.zeros(m, dtype=np.int64)
+    if np.asarray(sums_m).dtype.kind in "iu":
+        sums_m = _c(sums_m, np.int64)
+        s = np.zeros(m, dtype=np.int64)
+        ng = int(L.nf_carry_build_i64(_p(counts_m), _p(sums_m), m, _p(uk), _p(c), _p(s)))
+    else:
+        sums_m = _c(sums_m, np.float64)
+        s = np.zeros(m, dtype=np.float64)
+        ng = int(L.nf_carry_build_f64(_p(counts_m), _p(sums_m), m, _p(uk), _p(c), _p(s)))
+    assert ng >= 0, ng
+    return uk[:ng], c[:ng], s[:ng]
+
+
+# ------------------------------------------------------------ unique / sort
+def unique_inverse(keys):
+    """native_cpu.py:1465-1496: (uniq, inv int64, backend)."""
+    keys = np.ascontiguousarray(np.asarray(keys))
+    dt = keys.dtype
+    if dt not in (np.dtype(np.int32), np.dtype(np.int64)):
+        raise TypeError("unique_inverse: int32/int64 keys only, got %s" % dt)
+    n = keys.size
+    if n == 0:
+        return keys[:0], np.zeros(0, dtype=np.int64), "native"
+    L = lib("unique")
+    tu = np.uint32 if dt == np.dtype(np.int32) else np.uint64
+    uniq = np.empty(n, dtype=dt)
+    inv = np.empty(n, dtype=np.int32)
+    perm = np.empty(n, dtype=np.int32)
+    t0, t1 = np.empty(n, dtype=tu), np.empty(n, dtype=tu)
+    tp = np.empty(n, dtype=np.int32)
+    fn = L.nf_unique_inverse_i32 if dt == np.dtype(np.int32) else L.nf_unique_inverse_i64
+    ng = int(fn(_p(keys), n, _p(uniq), _p(inv), _p(perm), _p(t0), _p(t1), _p(tp)))
+    if ng < 0:
+        raise RuntimeError("nf_unique_inverse rc=%d" % ng)
+    return uniq[:ng].copy(), inv.astype(np.int64), "native"
+
+
+def _g_sort_asc(keys):
+    """Monotone probe (native_cpu.py:1512-1522): (nondecreasing, strict)."""
+    if keys.size < 2:
+        return True, True
+    if not bool(np.all(keys[1:] >= keys[:-1])):
+        return False, False
+    return True, bool(np.all(keys[1:] != keys[:-1]))
+
+
+def _g_sort_desc(keys):
+    """native_cpu.py:1525-1533."""
+    if keys.size < 2:
+        return True, True
+    if not bool(np.all(keys[1:] <= keys[:-1])):
+        return False, False
+    return True, bool(np.all(keys[1:] != keys[:-1]))
+
+
+def sort_perm_raw(keys, descending):
+    """One nf_sort_perm_* call (no monotone shortcuts): int32 permutation."""
+    k = np.ascontiguousarray(np.asarray(keys))
+    dt = k.dtype
+    n = k.size
+    if dt not in (np.dtype(np.int32), np.dtype(np.int64)):
+        raise TypeError("sort_perm: int32/int64 keys only, got %s" % dt)
+    if n == 0:
+        return np.zeros(0, dtype=np.int32)
+    L = lib("sort")
+    tu = np.uint32 if dt == np.dtype(np.int32) else np.uint64
+    perm = np.empty(n, dtype=np.int32)
+    t0, t1 = np.empty(n, dtype=tu), np.empty(n, dtype=tu)
+    tp = np.empty(n, dtype=np.int32)
+    fn = L.nf_sort_perm_i32 if dt == np.dtype(np.int32) else L.nf_sort_perm_i64
+    rc = int(fn(_p(k), n, 1 if descending else 0, _p(perm), _p(t0), _p(t1), _p(tp)))
+    if rc != 0:
+        raise RuntimeError("nf_sort_perm rc=%d" % rc)
+    return perm
+
+
+def sort_perm_step(keys, descending):
+    """native_cpu.py:1602-1652: (step, backend) with the monotone guards.
+
+    "ident"/"rev" answer without a library call exactly where the reference
+    does; a non-strict cross-monotone column goes to the radix kernel (the
+    reference hands it to NumPy, which produces the same permutation).
+    """
+    k = np.ascontiguousarray(np.asarray(keys))
+    n = k.size
+    if k.dtype not in (np.dtype(np.int32), np.dtype(np.int64)) or n > 2 ** 31 - 1:
+        raise TypeError("sort_perm_step: int32/int64 keys, n <= 2**31-1")
+    if n == 0:
+        return np.zeros(0, dtype=np.int32), "ident"
+    if descending:
+        ok_d, _ = _g_sort_desc(k)
+        if ok_d:
+            return np.arange(n, dtype=np.int32), "ident"
+        ok_a, st_a = _g_sort_asc(k)
+        if ok_a and st_a:
+            return np.arange(n - 1, -1, -1, dtype=np.int32), "rev"
+    else:
+        ok_a, _ = _g_sort_asc(k)
+        if ok_a:
+            return np.arange(n, dtype=np.int32), "ident"
+        ok_d, st_d = _g_sort_desc(k)
+        if ok_d and st_d:
+            return np.arange(n - 1, -1, -1, dtype=np.int32), "rev"
+    return sort_perm_raw(k, descending), "native"

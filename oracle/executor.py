@@ -1,0 +1,174 @@
+"""NumPy restatement of the reference executor semantics (TEST ONLY).
+
+Checker for numfast_amd; see oracle/__init__.py for the import rule.
+Each function cites the reference file:line it restates.
+"""
+
+import ctypes
+import os
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+ORACLE_LIB = ROOT / "libnf_oracle.so"
+
+
+def build_if_missing():
+    """Compile nf_oracle.c when the shared object is absent or stale."""
+    src = ROOT / "nf_oracle.c"
+    if ORACLE_LIB.exists() and ORACLE_LIB.stat().st_mtime >= src.stat().st_mtime:
+        return ORACLE_LIB
+    import subprocess
+    tmp = str(ORACLE_LIB) + ".tmp"
+    subprocess.run(["gcc", "-O3", "-march=x86-64-v2", "-fPIC", "-shared",
+                    "-fvisibility=hidden", "-o", tmp, str(src)], check=True)
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def load_c_oracle():
+    """ctypes handle on the C restatement, argtypes bound like the product."""
+    from numfast_amd import native
+    lib = ctypes.CDLL(str(build_if_missing()))
+    groups = native.bind(lib)
+    return lib, groups
+
+
+# ----------------------------------------------------------------- sort
+def stable_desc_idx(k):
+    """src/Drivers/CPU/_lib/cpu.py:1154-1171: ascending stable argsort, then
+    reverse the order of equal-key groups keeping input order inside."""
+    asc = np.argsort(k, kind="stable")
+    if asc.size == 0:
+        return asc
+    sk = k[asc]
+    ch = np.empty(sk.shape, dtype=bool)
+    ch[0] = True
+    if sk.size > 1:
+        ch[1:] = sk[1:] != sk[:-1]
+    grp = np.cumsum(ch)
+    rev = grp.max(initial=0) - grp
+    return asc[np.argsort(rev, kind="stable")]
+
+
+def sort_step(k, desc):
+    """One stable key step (np.argsort / _stable_desc_idx), int32 positions."""
+    k = np.asarray(k)
+    return (stable_desc_idx(k) if desc else np.argsort(k, kind="stable")).astype(np.int32)
+
+
+def sort_perm(cols, descending, valid=None):
+    """src/Drivers/CPU/_lib/cpu.py:1174-1225 (_sort_perm): lexicographic via
+    successive stable passes from the last key; invalid rows last, in input
+    order."""
+    cols = [np.asarray(c) for c in cols]
+    n = cols[0].size
+    if valid is not None:
+        m = np.asarray(valid, dtype=bool).reshape(-1)
+        vpos, ipos = np.flatnonzero(m), np.flatnonzero(~m)
+    else:
+        vpos, ipos = np.arange(n), np.zeros(0, dtype=np.int64)
+    idx = np.arange(vpos.size)
+    for c, desc in zip(reversed(cols), reversed(list(descending))):
+        k = c[vpos][idx] if vpos.size else c[:0]
+        idx = idx[sort_step(k, desc)]
+    perm = np.empty(n, dtype=np.int32)
+    perm[:vpos.size] = vpos[idx]
+    perm[vpos.size:] = ipos
+    return perm
+
+
+# ----------------------------------------------------------------- unique
+def unique_inverse(keys, valid=None):
+    """np.unique(return_inverse=True) with int32 codes; with validity the
+    unique set is taken over valid rows and inv = -1 on NULL rows
+    (src/Drivers/CPU/_lib/cpu.py:4006-4040)."""
+    keys = np.asarray(keys)
+    if valid is None:
+        u, inv = np.unique(keys, return_inverse=True)
+        return u, inv.astype(np.int32)
+    m = np.asarray(valid, dtype=bool)
+    u, iv = np.unique(keys[m], return_inverse=True)
+    inv = np.full(keys.shape, -1, dtype=np.int32)
+    inv[m] = iv.astype(np.int32)
+    return u, inv
+
+
+# ----------------------------------------------------------------- group-by
+def group_agg(keys, vals, valid=None, key_valid=None):
+    """Group-by with sum/count/min/max/mean and explicit NULL masks.
+
+    Restates the groupby_multi min/max branch of
+    src/Drivers/CPU/_lib/cpu.py:3507-3563: rows whose key or value is NULL
+    are compacted away first (_valid_mask AND, cpu.py:1374-1382,
+    :3508-3511), so a key whose every value is NULL has no group
+    (KNOWN_LIMITATIONS.md:147-165).  Groups come from the sorted unique of
+    the surviving keys (ukeys ascending, groupindex.py:3-8); counts are
+    occurrences; integer sums are exact int64 (cpu.py:3381-3383 casts the
+    integer lane to int64 -- the reference's float64 bincount is exact under
+    its 2**53 gate _int_exact_ok, cpu.py:424-446); min/max keep the value
+    dtype (cpu.py:2470-2601); mean = sums.astype(f64) / counts
+    (carry.py:55-62).
+    """
+    keys = np.asarray(keys)
+    vals = np.asarray(vals)
+    m = np.ones(keys.shape, dtype=bool)
+    if valid is not None:
+        m &= np.asarray(valid, dtype=bool)
+    if key_valid is not None:
+        m &= np.asarray(key_valid, dtype=bool)
+    kk, vv = keys[m], vals[m]
+    u, inv = np.unique(kk, return_inverse=True)
+    ng = u.size
+    counts = np.bincount(inv, minlength=ng).astype(np.int64)
+    if vv.dtype.kind in "iu":
+        sums = np.zeros(ng, dtype=np.int64)
+        np.add.at(sums, inv, vv.astype(np.int64))
+        info = np.iinfo(vv.dtype)
+        mins = np.full(ng, info.max, dtype=vv.dtype)
+        maxs = np.full(ng, info.min, dtype=vv.dtype)
+    else:
+        sums = np.bincount(inv, weights=vv.astype(np.float64), minlength=ng)
+        mins = np.full(ng, np.inf, dtype=vv.dtype)
+        maxs = np.full(ng, -np.inf, dtype=vv.dtype)
+    np.minimum.at(mins, inv, vv)
+    np.maximum.at(maxs, inv, vv)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        means = sums.astype(np.float64) / counts
+    return {"ukeys": u.astype(np.int64), "counts": counts, "sums": sums,
+            "mins": mins, "maxs": maxs, "means": means}
+
+
+def exact_sums_f64(keys, vals, g):
+    """Exact-rational per-group sums (Fraction), for error reporting only.
+    Pure Python: small inputs."""
+    acc = [Fraction(0)] * g
+    for k, v in zip(np.asarray(keys).tolist(), np.asarray(vals, dtype=np.float64).tolist()):
+        acc[k] += Fraction(v)
+    return acc
+
+
+def rel_err_vs_exact(got, exact):
+    """max |got - exact| / max(|exact|, tiny) over groups (float)."""
+    worst = 0.0
+    for gv, ev in zip(np.asarray(got, dtype=np.float64).tolist(), exact):
+        d = abs(Fraction(gv) - ev)
+        den = abs(ev) if ev != 0 else Fraction(1)
+        worst = max(worst, float(d / den))
+    return worst
+
+
+# ----------------------------------------------------------------- pipeline
+def filter_derive_sort_limit(v, limit=1000, mul=3):
+    """filter(v>0) -> derive(w = v*mul with int32 wrap) -> sort(w) stable
+    ascending -> limit(limit).  The derive multiplies in NumPy's int32
+    arithmetic and keeps the int32 result (cpu.py:3034-3035, :3074); sort
+    is _sort_perm with one ascending key (cpu.py:1174-1225); limit is the
+    row slice (cpu.py:3188-3202).  Returns (w_sorted_limited, source rows)."""
+    v = np.asarray(v, dtype=np.int32)
+    rows = np.flatnonzero(v > 0)
+    w = (v[rows].astype(np.int64) * mul).astype(np.int32)
+    perm = np.argsort(w, kind="stable")[:limit]
+    return w[perm], rows[perm].astype(np.int64)

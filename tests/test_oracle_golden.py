@@ -1,0 +1,304 @@
+"""Pin the oracle before trusting it (CPU only).
+
+The C restatement (oracle/nf_oracle.c) and the NumPy executor restatement
+(oracle/executor.py) are checked against
+  * the reference test-suite's literal vectors (tests/golden/*.json),
+  * NumPy library routines the reference itself treats as ground truth
+    (np.argsort(kind="stable"), np.unique, np.bincount),
+  * closed forms and hand-derived rc precedence cases where a plausible
+    mistake (final-sum-only overflow check, unstable sort, wrong error
+    order) gives a different answer.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import executor as ox
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+I32MIN, I32MAX = -(2 ** 31), 2 ** 31 - 1
+
+
+def c_sort(lib, k, desc):
+    k = np.ascontiguousarray(k)
+    n = k.size
+    tu = np.uint32 if k.dtype == np.int32 else np.uint64
+    perm = np.empty(n, np.int32)
+    t0, t1, tp = np.empty(n, tu), np.empty(n, tu), np.empty(n, np.int32)
+    fn = lib.nf_sort_perm_i32 if k.dtype == np.int32 else lib.nf_sort_perm_i64
+    rc = fn(k.ctypes.data, n, 1 if desc else 0, perm.ctypes.data, t0.ctypes.data,
+            t1.ctypes.data, tp.ctypes.data)
+    assert rc == 0
+    return perm
+
+
+def c_unique(lib, k):
+    k = np.ascontiguousarray(k)
+    n = k.size
+    tu = np.uint32 if k.dtype == np.int32 else np.uint64
+    u, inv, perm = np.empty(n, k.dtype), np.empty(n, np.int32), np.empty(n, np.int32)
+    t0, t1, tp = np.empty(n, tu), np.empty(n, tu), np.empty(n, np.int32)
+    fn = lib.nf_unique_inverse_i32 if k.dtype == np.int32 else lib.nf_unique_inverse_i64
+    ng = fn(k.ctypes.data, n, u.ctypes.data, inv.ctypes.data, perm.ctypes.data,
+            t0.ctypes.data, t1.ctypes.data, tp.ctypes.data)
+    return ng, u[:max(ng, 0)], inv
+
+
+def c_variant(lib, sym, keys, vals, variant, g):
+    keys = np.ascontiguousarray(keys, np.int32)
+    dt = {"nf_group_variant_i64": np.int64, "nf_group_variant_i32": np.int32,
+          "nf_group_variant_f64": np.float64}[sym]
+    vals = np.ascontiguousarray(vals, dt)
+    sums, counts = np.zeros(g, dt), np.zeros(g, np.int64)
+    rc = getattr(lib, sym)(keys.ctypes.data, vals.ctypes.data, keys.size, variant,
+                           sums.ctypes.data, counts.ctypes.data, g)
+    return rc, sums, counts
+
+
+# ------------------------------------------------------------------ sort
+@pytest.mark.parametrize("case", GOLD["sort"], ids=lambda c: c["src"])
+def test_sort_golden_vectors(c_oracle, case):
+    dt = np.int64 if case.get("dtype") == "int64" else np.int32
+    cols = [np.array(c, dtype=dt) for c in case["keys"]]
+    # executor restatement (multi-key, validity)
+    perm = ox.sort_perm(cols, case["desc"], case.get("valid"))
+    assert perm.tolist() == case["perm"]
+    # the C radix restatement, single key without validity
+    if len(cols) == 1 and "valid" not in case:
+        assert c_sort(c_oracle, cols[0], case["desc"][0]).tolist() == case["perm"]
+
+
+def test_topk_golden(c_oracle):
+    v = np.array(GOLD["topk"]["vals"], np.int32)
+    p = c_sort(c_oracle, v, True)[:GOLD["topk"]["limit"]]
+    assert v[p].tolist() == GOLD["topk"]["top"]
+
+
+@pytest.mark.parametrize("dist", ["uniform", "small", "dup_runs", "extremes"])
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+def test_c_radix_equals_numpy_stable(c_oracle, dist, dt):
+    rng = np.random.default_rng(42)
+    n = 20011
+    info = np.iinfo(dt)
+    if dist == "uniform":
+        k = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+    elif dist == "small":
+        k = rng.integers(-3, 4, n).astype(dt)
+    elif dist == "dup_runs":
+        k = np.repeat(rng.integers(0, 50, n // 10 + 1), 10)[:n].astype(dt)
+    else:
+        k = rng.choice(np.array([info.min, info.max, 0, -1, 1], dtype=dt), n)
+    asc = c_sort(c_oracle, k, False)
+    assert np.array_equal(asc, np.argsort(k, kind="stable"))
+    desc = c_sort(c_oracle, k, True)
+    assert np.array_equal(desc, ox.stable_desc_idx(k))
+    # descending == reversed groups, input order inside a group (closed form)
+    sk = k[desc]
+    assert np.all(sk[:-1] >= sk[1:])
+    same = sk[:-1] == sk[1:]
+    assert np.all(desc[:-1][same] < desc[1:][same])
+
+
+def test_lexsort_reference_case():
+    """tests/fast/test_ops_sort.py:73-87: n=10k two keys vs np.lexsort."""
+    rng = np.random.default_rng(42)
+    k1 = rng.integers(0, 50, size=10_000).astype(np.int32)
+    k2 = rng.integers(0, 50, size=10_000).astype(np.int32)
+    perm = ox.sort_perm([k1, k2], [False, False])
+    assert np.array_equal(perm, np.lexsort((k2, k1)))
+
+
+# ------------------------------------------------------------------ unique
+def test_unique_reference_lane_cases(c_oracle):
+    """tests/fast/test_ops_m7b_native_lanes.py:209-293 shapes vs np.unique."""
+    spec = GOLD["unique"]
+    rng = np.random.default_rng(42)
+    n = spec["gate"]
+    cases = [rng.integers(0, max(1, n // 100), size=n).astype(np.int32)]
+    cases += [(np.arange(n, dtype=np.int64) * spec["hash_mult"] % c).astype(np.int32)
+              for c in spec["cards"]]
+    cases += [rng.integers(-2 ** 40, 2 ** 40, size=n).astype(np.int64),
+              np.zeros(n, np.int32), rng.permutation(n).astype(np.int32)]
+    for k in cases:
+        ng, u, inv = c_unique(c_oracle, k)
+        ru, ri = np.unique(k, return_inverse=True)
+        assert ng == ru.size
+        assert np.array_equal(u, ru)
+        assert np.array_equal(inv, ri.astype(np.int32))
+        assert np.array_equal(u[inv], k)
+
+
+def test_unique_validity_restatement():
+    keys = np.arange(40, dtype=np.int32) % 7
+    vl = (np.arange(40) % 2 == 0)
+    u, inv = ox.unique_inverse(keys, vl)
+    ru, ri = np.unique(keys[vl], return_inverse=True)
+    assert np.array_equal(u, ru)
+    assert np.all(inv[~vl] == -1) and np.array_equal(inv[vl], ri)
+
+
+# ------------------------------------------------------------------ group-by
+@pytest.mark.parametrize("case", GOLD["group"], ids=lambda c: c["src"])
+def test_group_agg_golden(case):
+    vals = np.array(case["vals"])
+    vals = vals.astype(np.float64) if vals.dtype.kind == "f" else vals.astype(np.int32)
+    r = ox.group_agg(np.array(case["keys"], np.int32), vals, case.get("valid"))
+    assert r["ukeys"].tolist() == case["ukeys"]
+    if "sums" in case:
+        assert r["sums"].tolist() == case["sums"]
+        assert r["sums"].dtype == (np.int64 if vals.dtype.kind == "i" else np.float64)
+    if "counts" in case:
+        assert r["counts"].tolist() == case["counts"]
+    if "means" in case:
+        assert np.allclose(r["means"], case["means"], rtol=1e-12, atol=0)
+
+
+def test_group_agg_minmax_vs_python_dict():
+    """tests/fast/test_ops_groupby_quality.py:32-41 oracle shape: a plain
+    dict grouping over small N, with min/max added."""
+    rng = np.random.default_rng(42)
+    keys = rng.integers(0, 2000, size=3000).astype(np.int32)
+    vals = rng.integers(-50, 50, size=3000).astype(np.int32)
+    valid = rng.random(3000) > 0.1
+    acc = {}
+    for k, v, ok in zip(keys.tolist(), vals.tolist(), valid.tolist()):
+        if not ok:
+            continue
+        e = acc.setdefault(k, [0, 0, v, v])
+        e[0] += v
+        e[1] += 1
+        e[2] = min(e[2], v)
+        e[3] = max(e[3], v)
+    r = ox.group_agg(keys, vals, valid)
+    assert r["ukeys"].tolist() == sorted(acc)
+    assert r["sums"].tolist() == [acc[k][0] for k in sorted(acc)]
+    assert r["counts"].tolist() == [acc[k][1] for k in sorted(acc)]
+    assert r["mins"].tolist() == [acc[k][2] for k in sorted(acc)]
+    assert r["maxs"].tolist() == [acc[k][3] for k in sorted(acc)]
+    assert r["mins"].dtype == np.int32
+
+
+@pytest.mark.parametrize("case", GOLD["abi"], ids=lambda c: c["src"])
+def test_c_variant_golden(c_oracle, case):
+    rc, sums, counts = c_variant(c_oracle, case["sym"], case["keys"], case["vals"],
+                                 case["variant"], case["g"])
+    assert rc == case["rc"]
+    if "sums" in case:
+        assert sums.tolist() == case["sums"] and counts.tolist() == case["counts"]
+
+
+def test_c_f64_row_order_bit_exact_vs_bincount(c_oracle):
+    """dense.rs:5-7: row-order f64 sums are bit-exact to np.bincount."""
+    rng = np.random.default_rng(0)
+    n, g = 1 << 16, 1000
+    keys = rng.integers(0, g, n).astype(np.int32)
+    vals = rng.uniform(-1000, 1000, n)
+    sums, counts = np.zeros(g), np.zeros(g, np.int64)
+    assert c_oracle.nf_group_sum_count(keys.ctypes.data, vals.ctypes.data, n,
+                                       sums.ctypes.data, counts.ctypes.data, g) == 0
+    assert np.array_equal(sums.view(np.int64),
+                          np.bincount(keys, weights=vals, minlength=g).view(np.int64))
+    assert np.array_equal(counts, np.bincount(keys, minlength=g))
+
+
+def test_c_rc_prefix_and_precedence(c_oracle):
+    """Hand-derived cases: prefix overflow (final sum fits), first failing
+    row wins between -2 and -4, i32 lane has no V_CHECKED."""
+    # prefix MAX+1 then -1: the final sum MAX fits, the prefix does not
+    assert c_variant(c_oracle, "nf_group_variant_i32", [0, 0, 0], [I32MAX, 1, -1], 0, 1)[0] == -4
+    # same values in the other order never leave int32
+    rc, s, c = c_variant(c_oracle, "nf_group_variant_i32", [0, 0, 0], [I32MAX, -1, 1], 0, 1)
+    assert rc == 0 and s.tolist() == [I32MAX]
+    # overflow at row 1, bad key at row 2 -> -4; bad key at row 1 -> -2
+    assert c_variant(c_oracle, "nf_group_variant_i32", [0, 0, 5], [I32MAX, 1, 0], 0, 2)[0] == -4
+    assert c_variant(c_oracle, "nf_group_variant_i32", [0, 5, 0], [I32MAX, 0, 1], 0, 2)[0] == -2
+    # negative prefix leaves int32 at row 1
+    assert c_variant(c_oracle, "nf_group_variant_i32", [1, 1], [I32MIN, -1], 0, 2)[0] == -4
+    # i64: 2^62 + 2^62 = 2^63 overflows; 2^62 + 2^62 - 1 ... split groups fine
+    assert c_variant(c_oracle, "nf_group_variant_i64", [0, 1], [2 ** 62, 2 ** 62], 1, 2)[0] == 0
+    # V_SORTED: regress at row 2 -> -5; overflow at row 1 before it -> -4
+    assert c_variant(c_oracle, "nf_group_variant_i32", [0, 1, 0], [1, 1, 1], 2, 2)[0] == -5
+    assert c_variant(c_oracle, "nf_group_variant_i32", [0, 0, -0], [I32MAX, 1, 1], 2, 2)[0] == -4
+    assert c_variant(c_oracle, "nf_group_variant_i32", [1, 1, 0], [I32MAX, 1, 1], 2, 2)[0] == -4
+    assert c_variant(c_oracle, "nf_group_variant_i32", [1, 0, 0], [I32MAX, 1, 1], 2, 2)[0] == -5
+    # bad variants
+    assert c_variant(c_oracle, "nf_group_variant_i64", [0], [1], 3, 1)[0] == -3
+    assert c_variant(c_oracle, "nf_group_variant_f64", [0], [1.0], 7, 1)[0] == -3
+
+
+def test_c_int_variants_vs_numpy(c_oracle):
+    """Integer sums commute: every variant equals np.add.at on in-range keys
+    (tests/fast/test_ops_panic_safety.py:113-178 grid, smaller)."""
+    rng = np.random.default_rng(42)
+    for d in (1, 2, 8, 1000):
+        for n in (1, 7, 1000, 20000):
+            keys = rng.integers(0, d, n).astype(np.int32)
+            data = rng.integers(-(2 ** 20), 2 ** 20, n).astype(np.int64)
+            ref = np.zeros(d, np.int64)
+            np.add.at(ref, keys, data)
+            cnt = np.bincount(keys, minlength=d)
+            for v in (0, 1):
+                rc, s, c = c_variant(c_oracle, "nf_group_variant_i64", keys, data, v, d)
+                assert rc == 0 and np.array_equal(s, ref) and np.array_equal(c, cnt)
+            sk = np.sort(keys)
+            ref2 = np.zeros(d, np.int64)
+            np.add.at(ref2, sk, data)
+            rc, s, c = c_variant(c_oracle, "nf_group_variant_i64", sk, data, 2, d)
+            assert rc == 0 and np.array_equal(s, ref2)
+
+
+def test_c_sorted_run_and_carry(c_oracle):
+    rng = np.random.default_rng(7)
+    keys = np.sort(rng.integers(-50, 50, 5000)).astype(np.int32)
+    vals = rng.integers(-1000, 1000, 5000).astype(np.int64)
+    n = keys.size
+    uk, s, c = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+    ng = c_oracle.nf_sorted_run_i64(keys.ctypes.data, vals.ctypes.data, n, uk.ctypes.data,
+                                    s.ctypes.data, c.ctypes.data)
+    ru, starts = np.unique(keys, return_index=True)
+    assert ng == ru.size
+    assert np.array_equal(uk[:ng], ru)
+    assert np.array_equal(s[:ng], np.add.reduceat(vals, starts))
+    assert np.array_equal(c[:ng], np.diff(np.append(starts, n)))
+    bad = keys.copy()
+    bad[10], bad[11] = bad[11] + 1, bad[10]
+    if bad[10] > bad[11]:
+        assert c_oracle.nf_sorted_run_i64(bad.ctypes.data, vals.ctypes.data, n, uk.ctypes.data,
+                                          s.ctypes.data, c.ctypes.data) == -1
+    # carry = flatnonzero compaction (native_cpu.py:295-297)
+    cm = rng.integers(0, 3, 1000).astype(np.int64)
+    sm = rng.integers(-9, 9, 1000).astype(np.int64)
+    uk, cc, ss = np.zeros(1000, np.int64), np.zeros(1000, np.int64), np.zeros(1000, np.int64)
+    ng = c_oracle.nf_carry_build_i64(cm.ctypes.data, sm.ctypes.data, 1000, uk.ctypes.data,
+                                     cc.ctypes.data, ss.ctypes.data)
+    pos = np.flatnonzero(cm)
+    assert ng == pos.size and np.array_equal(uk[:ng], pos)
+    assert np.array_equal(cc[:ng], cm[pos]) and np.array_equal(ss[:ng], sm[pos])
+
+
+def test_c_pattern_golden(c_oracle):
+    from numfast_amd import native
+    for case in GOLD["pattern"]:
+        data, offs = native.build_pattern_buffers(case["rows"])
+        n = offs.size - 1
+        dp = np.frombuffer(data + b"\0", np.uint8)
+        pp = np.frombuffer(case["prefix"].encode() + b"\0", np.uint8)
+        codes, valid = np.zeros(n, np.int32), np.zeros(n, np.uint8)
+        w, e = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        rc = c_oracle.nf_pattern_encode(dp.ctypes.data, len(data), offs.ctypes.data, n,
+                                        pp.ctypes.data, len(case["prefix"]), codes.ctypes.data,
+                                        valid.ctypes.data, w.ctypes.data, e.ctypes.data)
+        assert rc == 0
+        assert codes.tolist() == case["codes"] and valid.tolist() == case["valid"]
+        assert int(w[0]) == case["width"]
+
+
+def test_pipeline_restatement_closed_form():
+    v = np.array([5, -1, 3, 0, 2 ** 30, 7, 3], np.int32)
+    w, rows = ox.filter_derive_sort_limit(v, limit=4)
+    # v*3 with int32 wrap: 2^30*3 = 3*2^30 - 2^32 = -2^30
+    assert w.tolist() == [-(2 ** 30), 9, 9, 15]
+    assert rows.tolist() == [4, 2, 6, 0]
