@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""bench.py -- NumFast's relational hot path on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic input, the
+high-cardinality group-by configuration: 2**28 rows of int32 keys drawn
+Zipf(s=1.1) over 2**24 ids, int32 values uniform in [-2**15, 2**15), 5%
+NULL values; group-by with sum/count/min/max/mean, then unique on the key
+column with its own 5% NULL mask.  Inputs are generated on the GPU and are
+resident in HBM before the timed region; outputs go to device buffers.
+
+Also measured in the same run (reported under "extra"): the stable radix
+argsort of 2**28 int32 keys (uniform full range) and the uniform-2**24
+group-by variant.
+
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one
+GPU (the multi-GPU sharding is a design note only).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+METRIC = ("rows/s + % HBM peak: group-sum & int32 argsort 2^28; learned/batched vs plain")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows-log2", type=int, default=28)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip argsort / uniform legs")
+    return ap.parse_args()
+
+
+def kernel_avg_ms(L, name):
+    ms, cnt = ctypes.c_double(), ctypes.c_uint64()
+    if L.nfh_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)) != 0 or cnt.value == 0:
+        return None, 0
+    return ms.value / cnt.value, int(cnt.value)
+
+
+def timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(n_sample=1 << 22):
+    """The oracle's C restatement of the reference route for this workload
+    (NULL compaction, radix unique on the surviving keys -> codes, then
+    per-code sums/counts (np.bincount) and min/max (np.minimum/maximum.at),
+    mean = sum/count; unique of the key column over its valid rows),
+    single-threaded, on a bounded sample of the same distribution."""
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.executor import load_c_oracle
+    lib, _ = load_c_oracle()
+    rng = np.random.default_rng(0)
+    ids = 1 << 24
+    ranks = np.arange(1, ids + 1, dtype=np.float64)
+    cdf = np.cumsum(ranks ** -1.1)
+    cdf /= cdf[-1]
+    r = np.minimum(np.searchsorted(cdf, rng.random(n_sample), side="right"), ids - 1)
+    keys = rng.permutation(ids).astype(np.int32)[r]
+    vals = rng.integers(-(1 << 15), 1 << 15, n_sample).astype(np.int32)
+    valid = rng.random(n_sample) >= 0.05
+    kvalid = rng.random(n_sample) >= 0.05
+
+    def unique_c(k):
+        n = k.size
+        u, inv, perm = np.empty(n, np.int32), np.empty(n, np.int32), np.empty(n, np.int32)
+        t0, t1, tp = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.int32)
+        ng = lib.nf_unique_inverse_i32(k.ctypes.data, n, u.ctypes.data, inv.ctypes.data,
+                                       perm.ctypes.data, t0.ctypes.data, t1.ctypes.data,
+                                       tp.ctypes.data)
+        return u[:ng], inv
+
+    def step():
+        kk = np.ascontiguousarray(keys[valid])
+        vv = vals[valid]
+        u, inv = unique_c(kk)
+        ng = u.size
+        sums = np.bincount(inv, weights=vv.astype(np.int64), minlength=ng).astype(np.int64)
+        counts = np.bincount(inv, minlength=ng).astype(np.int64)
+        mins = np.full(ng, np.iinfo(np.int32).max, np.int32)
+        maxs = np.full(ng, np.iinfo(np.int32).min, np.int32)
+        np.minimum.at(mins, inv, vv)
+        np.maximum.at(maxs, inv, vv)
+        _ = sums.astype(np.float64) / counts
+        uk, _ = unique_c(np.ascontiguousarray(keys[kvalid]))
+        return ng, uk.size
+
+    step()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        step()
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[len(ts) // 2]
+    return {"value": n_sample / t, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": ("group-by sum/count/min/max/mean + unique, %d rows of the same Zipf(1.1) "
+                       "/ 2^24-id / 5%%-NULL distribution; C restatement of the reference "
+                       "radix unique + NumPy bincount/minimum.at (reference route), median "
+                       "of 3" % n_sample)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; the multi-GPU sharding of the "
+              "group-by / sort is a design note (DESIGN.md).  Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch  # first: the library then shares torch's HIP runtime
+    torch.cuda.init()
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from numfast_amd import native
+    from numfast_amd import workloads as W
+    L = native.lib("hip")
+    n = 1 << args.rows_log2
+    ids = 1 << 24
+
+    keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids)
+    cap = ids
+    outs = [torch.empty(cap, dtype=dt, device="cuda") for dt in
+            (torch.int64, torch.int64, torch.int64, torch.int32, torch.int32, torch.float64)]
+    uniq = torch.empty(ids, dtype=torch.int32, device="cuda")
+    inv = torch.empty(n, dtype=torch.int32, device="cuda")
+    state = {}
+
+    def step_on(k, v, m, km):
+        def step():
+            ng = L.nfh_group_agg_i32(k.data_ptr(), v.data_ptr(), m.data_ptr(), None, n,
+                                     *[o.data_ptr() for o in outs], cap)
+            nu = L.nfh_unique_valid_i32(k.data_ptr(), km.data_ptr(), n, uniq.data_ptr(),
+                                        inv.data_ptr())
+            if ng < 0 or nu < 0:
+                raise RuntimeError("library rc %d / %d" % (ng, nu))
+            state["ng"], state["nu"] = ng, nu
+        return step
+
+    torch.cuda.synchronize()
+    L.nfh_profile_enable(0)
+    step = step_on(keys, vals, valid, kvalid)
+    for _ in range(args.warmup):
+        step()
+    L.nfh_profile_reset()
+    L.nfh_profile_enable(1)
+    t = timed(torch, step, args.steps, 0)
+    L.nfh_profile_enable(0)
+    acc_ms, acc_n = kernel_avg_ms(L, "group_acc")
+    kernels = {}
+    for name in ("key_range", "group_acc", "scan_tiles", "unique_mark", "unique_inv_dense",
+                 "radix_hist", "radix_onesweep"):
+        ms, cnt = kernel_avg_ms(L, name)
+        if ms is not None:
+            kernels[name] = {"avg_ms": round(ms, 4), "launches": cnt}
+    ng, nu = state["ng"], state["nu"]
+    # dominant kernel: the group-by accumulate pass, 9 algorithmic B/row
+    # (key 4 + value 4 + validity 1)
+    acc_bytes = 9.0 * n
+    achieved = acc_bytes / (acc_ms * 1e-3) / 1e9 if acc_ms else None
+    roofline = {"bound": "hbm", "kernel": "k_group_acc", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "alg_bytes_per_launch": acc_bytes, "kernel_avg_ms": round(acc_ms, 4)}
+    # whole-step algorithmic bytes: group-by 9 B/row + 40 B/group, unique
+    # 5 B/row read + 4 B/row inverse + 4 B per distinct key
+    step_bytes = 9.0 * n + 40.0 * ng + 9.0 * n + 4.0 * nu
+    extra = {"groupby_ng": ng, "unique_ng": nu, "kernels": kernels,
+             "step_alg_GBps": round(step_bytes / t / 1e9, 1),
+             "step_alg_frac": round(step_bytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+    del keys, vals, valid, kvalid
+
+    if not args.no_extra:
+        # stable argsort, 2^28 uniform int32 (pass model 60 B/row)
+        k = W.sort_config(n, "uniform")
+        perm = torch.empty(n, dtype=torch.int32, device="cuda")
+        t0b = torch.empty(n, dtype=torch.int32, device="cuda")
+        t1b, tpb = torch.empty_like(t0b), torch.empty_like(t0b)
+
+        def sort_step():
+            rc = L.nf_sort_perm_i32(k.data_ptr(), n, 0, perm.data_ptr(), t0b.data_ptr(),
+                                    t1b.data_ptr(), tpb.data_ptr())
+            if rc != 0:
+                raise RuntimeError("sort rc %d" % rc)
+        L.nfh_profile_reset()
+        L.nfh_profile_enable(1)
+        ts = timed(torch, sort_step, max(3, args.steps // 2), 1)
+        L.nfh_profile_enable(0)
+        one_ms, one_n = kernel_avg_ms(L, "radix_onesweep")
+        extra["argsort_uniform_2^28"] = {
+            "rows_per_s": n / ts, "ms": round(ts * 1e3, 3),
+            "pass_model_GBps": round(60.0 * n / ts / 1e9, 1),
+            "frac_of_peak_60B_model": round(60.0 * n / ts / 1e9 / HBM_PEAK_GBS, 4),
+            "compulsory_frac_8B": round(8.0 * n / ts / 1e9 / HBM_PEAK_GBS, 4),
+            "onesweep_avg_ms": round(one_ms, 4) if one_ms else None}
+        del k, perm, t0b, t1b, tpb
+        # uniform-2^24 group-by variant
+        keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")
+        tu = timed(torch, step_on(keys, vals, valid, kvalid), max(3, args.steps // 2), 1)
+        extra["groupby_unique_uniform_2^24"] = {"rows_per_s": n / tu, "ms": round(tu * 1e3, 3),
+                                                "groupby_ng": state["ng"]}
+        del keys, vals, valid, kvalid
+
+    out = {
+        "metric": METRIC,
+        "value": n / t,
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": "groupby(sum,count,min,max,mean)+unique, 2^%d rows, int32 keys "
+                               "Zipf(1.1) over 2^24 ids, int32 values U[-2^15,2^15), 5%% NULL "
+                               "values, 5%% NULL keys for unique" % args.rows_log2,
+                   "rows": n, "ids": ids, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+        "extra": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

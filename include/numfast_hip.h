@@ -1,0 +1,137 @@
+/*
+ * numfast_hip.h -- C ABI of libnumfast_hip.so (MI355X / gfx950).
+ *
+ * The nf_* symbols are drop-in replacements for the reference crate
+ * numfast-native: same names, argument order and meaning, same return
+ * codes (numfast-native/src/core/errors.rs:13-38), loaded by NumFast's
+ * ctypes binding through NUMFAST_NATIVE_DLL
+ * (src/Drivers/CPU/_lib/native_cpu.py:93-143).  Buffers may be host memory
+ * (staged through device memory) or device memory (used in place); every
+ * call returns when its outputs are written.  Additional return codes:
+ *   -6  device error (no GPU, out of memory, launch failure)
+ *   -7  caller capacity too small (nfh_group_agg_i32 only)
+ * Where the reference aborts the process (an out-of-range dense key
+ * reaching an unchecked lane), these symbols return -2 instead.
+ *
+ * The nfh_* symbols are additive: they have no reference counterpart.
+ */
+#ifndef NUMFAST_HIP_H
+#define NUMFAST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- eager group (native_cpu.py:125-140: all eight or none) ---------- */
+
+/* numfast-native/src/lib.rs:85-103 -- dense f64 sum + count by code. */
+int32_t nf_group_sum_count(const int32_t *keys, const double *values, size_t n,
+                           double *sums, int64_t *counts, size_t g);
+/* lib.rs:117-135 -- SoA multi-column f64 sums + shared counts. */
+int32_t nf_group_multi_sum_count(const int32_t *keys, const double *values, size_t n,
+                                 size_t ncols, double *sums, int64_t *counts, size_t g);
+/* lib.rs:252-272 -- out[i] = k1[i]*m2 + k2[i], -2 on negative / overflow. */
+int32_t nf_pack_i32_direct(const int32_t *k1, const int32_t *k2, int32_t m2, size_t n,
+                           int32_t *out);
+/* lib.rs:340-373 -- prefix+int string encode. */
+int32_t nf_pattern_encode(const uint8_t *data, size_t total, const int32_t *offs, size_t n,
+                          const uint8_t *prefix, size_t prefix_len, int32_t *codes,
+                          uint8_t *valid, int32_t *width_out, int32_t *err_row_out);
+/* lib.rs:399-447 -- run aggregation over nondecreasing keys; ng or -1. */
+int64_t nf_sorted_run_i64(const int32_t *keys, const int64_t *vals, size_t n, int64_t *ukeys,
+                          int64_t *sums, int64_t *counts);
+int64_t nf_sorted_run_f64(const int32_t *keys, const double *vals, size_t n, int64_t *ukeys,
+                          double *sums, int64_t *counts);
+/* lib.rs:462-518 -- dense state -> compact (ukeys, counts, sums); ng. */
+int64_t nf_carry_build_i64(const int64_t *counts_m, const int64_t *sums_m, size_t m,
+                           int64_t *ukeys, int64_t *counts, int64_t *sums);
+int64_t nf_carry_build_f64(const int64_t *counts_m, const double *sums_m, size_t m,
+                           int64_t *ukeys, int64_t *counts, double *sums);
+
+/* ---- variant group (native_cpu.py:324-352) --------------------------- */
+/* lib.rs:539-614; variant 0 dense, 1 checked (not i32), 2 sorted-dense. */
+int32_t nf_group_variant_f64(const int32_t *keys, const double *values, size_t n,
+                             uint32_t variant, double *sums, int64_t *counts, size_t g);
+int32_t nf_group_variant_i64(const int32_t *keys, const int64_t *ticks, size_t n,
+                             uint32_t variant, int64_t *sums, int64_t *counts, size_t g);
+int32_t nf_group_variant_i32(const int32_t *keys, const int32_t *ticks, size_t n,
+                             uint32_t variant, int32_t *sums, int64_t *counts, size_t g);
+
+/* ---- mixed group (native_cpu.py:503-532) ----------------------------- */
+/* lib.rs:150-178 -- sum-only over n_i32 int32 + n_f64 float64 SoA lanes. */
+int32_t nf_group_mixed_sum_only(const int32_t *keys, const int32_t *i32_vals,
+                                const double *f64_vals, size_t n, size_t n_i32, size_t n_f64,
+                                int32_t *i32_sums, double *f64_sums, size_t g);
+/* lib.rs:184-197 -- keys-only counts. */
+int32_t nf_group_count_only(const int32_t *keys, size_t n, int64_t *counts, size_t g);
+
+/* ---- owner group (native_cpu.py:548-574) ----------------------------- */
+/* lib.rs:213-250 -- writes only lanes [lo, hi) of the shared state. */
+int32_t nf_group_owner_2i32_1f64(const int32_t *keys, const int32_t *a, const int32_t *b,
+                                 const double *c, size_t n, size_t lo, size_t hi, int32_t *s1,
+                                 int32_t *s2, double *s3, int64_t *counts, size_t g);
+
+/* ---- fused group (native_cpu.py:621-651) ----------------------------- */
+/* lib.rs:276-331 -- pack k1*m2+k2 fused into the dense sum + count. */
+int32_t nf_pack_sum_count_i32(const int32_t *k1, const int32_t *k2, int32_t m2,
+                              const int32_t *vals, size_t n, int32_t *sums, int64_t *counts,
+                              size_t g);
+int32_t nf_pack_sum_count_f64(const int32_t *k1, const int32_t *k2, int32_t m2,
+                              const double *vals, size_t n, double *sums, int64_t *counts,
+                              size_t g);
+
+/* ---- unique group (native_cpu.py:1399-1431) -------------------------- */
+/* lib.rs:1420-1500 -- np.unique(keys, return_inverse=True); returns ng. */
+int64_t nf_unique_inverse_i32(const int32_t *keys, size_t n, int32_t *uniq_out,
+                              int32_t *inv_out, int32_t *perm, uint32_t *tmp0, uint32_t *tmp1,
+                              int32_t *tmp_p);
+int64_t nf_unique_inverse_i64(const int64_t *keys, size_t n, int64_t *uniq_out,
+                              int32_t *inv_out, int32_t *perm, uint64_t *tmp0, uint64_t *tmp1,
+                              int32_t *tmp_p);
+
+/* ---- sort group (native_cpu.py:1546-1576) ---------------------------- */
+/* lib.rs:1326-1394 -- stable argsort; descending != 0 for descending. */
+int32_t nf_sort_perm_i32(const int32_t *keys, size_t n, uint8_t descending, int32_t *perm_out,
+                         uint32_t *tmp0, uint32_t *tmp1, int32_t *tmp_p);
+int32_t nf_sort_perm_i64(const int64_t *keys, size_t n, uint8_t descending, int32_t *perm_out,
+                         uint64_t *tmp0, uint64_t *tmp1, int32_t *tmp_p);
+
+/* ---- additive surface (no reference counterpart) --------------------- */
+
+/* Order this thread's calls on a caller stream (hipStream_t); NULL
+ * restores the library's own per-thread stream. */
+int32_t nfh_set_stream(void *stream);
+/* 1 when a GPU is usable, else 0. */
+int32_t nfh_device_ok(void);
+int32_t nfh_abi_version(void);
+
+/* Per-kernel HIP-event timing on the launch stream (for benchmarks):
+ * enable, reset, and read the total ms / launch count of a kernel name
+ * (returns -1 for an unknown name). */
+int32_t nfh_profile_enable(int32_t on);
+int32_t nfh_profile_reset(void);
+int32_t nfh_profile_get(const char *name, double *ms, uint64_t *launches);
+
+/* Group-by over raw int32 keys with NULL masks (valid / key_valid may be
+ * NULL): groups of rows with both masks set, ukeys ascending; sums int64,
+ * min/max int32, mean = double(sum)/double(count).  Replaces the
+ * executor's compaction + np.minimum/maximum.at + bincount path
+ * (src/Drivers/CPU/_lib/cpu.py:3507-3563, carry.py:55-62).  Outputs hold
+ * cap lanes (NULL outputs are skipped); returns ng, or -7 if ng > cap. */
+int64_t nfh_group_agg_i32(const int32_t *keys, const int32_t *vals, const uint8_t *valid,
+                          const uint8_t *key_valid, size_t n, int64_t *ukeys, int64_t *counts,
+                          int64_t *sums, int32_t *mins, int32_t *maxs, double *means,
+                          size_t cap);
+
+/* np.unique over valid rows; inv = -1 on NULL rows
+ * (src/Drivers/CPU/_lib/cpu.py:4032-4036).  Returns ng. */
+int64_t nfh_unique_valid_i32(const int32_t *keys, const uint8_t *valid, size_t n,
+                             int32_t *uniq_out, int32_t *inv_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUMFAST_HIP_H */

@@ -1,0 +1,669 @@
+// groupagg.hip -- NULL-aware group-by by raw int32 keys (sum / count / min /
+// max / mean) and unique with validity.  Additive entry points: the
+// reference ABI has no grouped min/max, NULL or mean kernel; its executor
+// does these in NumPy (src/Drivers/CPU/_lib/cpu.py:3507-3563 compaction +
+// np.minimum/maximum.at + bincount, carry.py:55-62 mean = sum / count).
+//
+// Result contract (ColumnCarry, carry.py:21-62; groupindex.py:3-8): groups
+// are the distinct keys of the rows whose key and value are both valid,
+// ascending; counts int64; sums int64 (exact); min/max int32; mean float64
+// = double(sum) / double(count).
+//
+// Strategies (identical results):
+//  * dense (valid key span moderate): one accumulate pass into span-indexed
+//    32-byte slots {sum, count, min, max}.  Hot keys -- found by sampling
+//    2^16 rows, sorting the sample and keeping the keys seen most often --
+//    are privatised per workgroup in LDS (hash table + LDS atomics) and
+//    flushed once per workgroup; every other row goes straight to its
+//    global slot (the high-cardinality spill: two atomics, min/max only
+//    when the row would change them).  One scan over the slots compacts the
+//    present groups in key order.
+//  * sparse (huge span): the valid rows are compacted, radix-unique'd
+//    (unique.hip) and aggregated by code; ukeys = the unique values.
+#include <algorithm>
+
+#include "device.hpp"
+#include "groupagg.hpp"
+#include "runtime.hpp"
+#include "scan.hpp"
+#include "sort.hpp"
+#include "unique.hpp"
+
+namespace nfh {
+
+typedef unsigned long long u64;
+
+struct __align__(32) GroupSlot {
+  u64 sum;
+  uint32_t cnt;
+  int32_t mn;
+  int32_t mx;
+  uint32_t pad[3];
+};
+static_assert(sizeof(GroupSlot) == 32, "slot is one 32-byte sector");
+
+constexpr int HOT_MAX = 1024;    // hot keys privatised per workgroup
+constexpr int HOT_TBL = 2048;    // LDS open-addressing table (load <= 0.5)
+constexpr int HOT_TBL_BITS = 11;
+constexpr int HOT_SAMPLE = 1 << 16;
+constexpr uint32_t HOT_MIN_COUNT = 4;  // sampled occurrences to qualify
+constexpr int HOT_LEN_BUCKETS = 1024;
+constexpr int ACC_BLOCK = 256;
+
+__device__ __forceinline__ uint32_t hot_hash(int32_t k) {
+  return (uint32_t(k) * 0x9E3779B1u) >> (32 - HOT_TBL_BITS);
+}
+
+// Rows are visited in quads (128-bit key/value loads, 32-bit mask loads)
+// when every pointer is suitably aligned, else one by one.
+template <bool VEC, class F>
+__device__ __forceinline__ void for_each_row(const int32_t* __restrict__ keys,
+                                             const int32_t* __restrict__ vals,
+                                             const uint8_t* __restrict__ vm,
+                                             const uint8_t* __restrict__ km, uint32_t n, F&& f) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t tail = 0;
+  if constexpr (VEC) {
+    const uint32_t nq = n / 4;
+    for (uint32_t q = tid; q < nq; q += stride) {
+      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
+      const int4 v4 = vals ? reinterpret_cast<const int4*>(vals)[q] : make_int4(0, 0, 0, 0);
+      const uint32_t a = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
+      const uint32_t b = km ? reinterpret_cast<const uint32_t*>(km)[q] : 0x01010101u;
+      f(k4.x, v4.x, (a & 0xFFu) && (b & 0xFFu));
+      f(k4.y, v4.y, (a & 0xFF00u) && (b & 0xFF00u));
+      f(k4.z, v4.z, (a & 0xFF0000u) && (b & 0xFF0000u));
+      f(k4.w, v4.w, (a & 0xFF000000u) && (b & 0xFF000000u));
+    }
+    tail = nq * 4;
+  }
+  for (uint32_t i = tail + tid; i < n; i += stride) {
+    const bool ok = (vm == nullptr || vm[i] != 0) && (km == nullptr || km[i] != 0);
+    f(keys[i], vals ? vals[i] : 0, ok);
+  }
+}
+
+static bool aligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+static bool vec_ok(const int32_t* k, const int32_t* v, const uint8_t* m1, const uint8_t* m2) {
+  return aligned(k, 16) && (v == nullptr || aligned(v, 16)) && (m1 == nullptr || aligned(m1, 4)) &&
+         (m2 == nullptr || aligned(m2, 4));
+}
+
+// min / max of the valid keys.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_key_range(const int32_t* __restrict__ keys,
+                                                   const uint8_t* __restrict__ v,
+                                                   const uint8_t* __restrict__ kv, uint32_t n,
+                                                   int* __restrict__ out /* min, max */) {
+  int mn = INT32_MAX, mx = INT32_MIN;
+  for_each_row<VEC>(keys, nullptr, v, kv, n, [&](int32_t k, int32_t, bool ok) {
+    if (ok) {
+      mn = k < mn ? k : mn;
+      mx = k > mx ? k : mx;
+    }
+  });
+  for (int off = 32; off > 0; off >>= 1) {
+    const int a = __shfl_down(mn, off), b = __shfl_down(mx, off);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  if (lane_id() == 0) {
+    if (mn != INT32_MAX) atomicMin(&out[0], mn);
+    if (mx != INT32_MIN) atomicMax(&out[1], mx);
+  }
+}
+
+__global__ void k_slot_init(GroupSlot* __restrict__ s, u64 m) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = blockIdx.x * u64(blockDim.x) + threadIdx.x; i < m; i += stride) {
+    GroupSlot z;
+    z.sum = 0;
+    z.cnt = 0;
+    z.mn = INT32_MAX;
+    z.mx = INT32_MIN;
+    z.pad[0] = z.pad[1] = z.pad[2] = 0;
+    s[i] = z;
+  }
+}
+
+// ---------------------------------------------------------------- hot keys
+__global__ void k_sample(const int32_t* __restrict__ keys, uint32_t n, uint32_t S,
+                         int32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S) out[i] = keys[uint32_t((u64(i) * n) / S)];
+}
+
+// Run length at each run head of the sorted sample (0 elsewhere).
+__global__ void k_hot_runs(const int32_t* __restrict__ sk, uint32_t S, uint32_t* __restrict__ len) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  uint32_t L = 0;
+  if (i == 0 || sk[i] != sk[i - 1]) {
+    uint32_t j = i + 1;
+    while (j < S && sk[j] == sk[i]) ++j;
+    L = j - i;
+  }
+  len[i] = L;
+}
+
+// One block: keep the keys sampled most often (at most HOT_MAX, each seen
+// at least HOT_MIN_COUNT times) and build their open-addressing table.
+__global__ __launch_bounds__(1024) void k_hot_pick(const int32_t* __restrict__ sk,
+                                                   const uint32_t* __restrict__ len, uint32_t S,
+                                                   int32_t* __restrict__ hot_keys,
+                                                   int2* __restrict__ table,
+                                                   uint32_t* __restrict__ hot_n) {
+  __shared__ uint32_t hist[HOT_LEN_BUCKETS];
+  __shared__ int32_t t_idx[HOT_TBL];
+  __shared__ int32_t t_key[HOT_TBL];
+  __shared__ uint32_t s_thr, s_cnt;
+  for (int b = threadIdx.x; b < HOT_LEN_BUCKETS; b += blockDim.x) hist[b] = 0;
+  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) {
+    t_idx[b] = -1;
+    t_key[b] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+    const uint32_t L = len[i];
+    if (L >= HOT_MIN_COUNT) atomicAdd(&hist[L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0, thr = 0xFFFFFFFFu;
+    for (int b = HOT_LEN_BUCKETS - 1; b >= int(HOT_MIN_COUNT); --b) {
+      if (acc + hist[b] > uint32_t(HOT_MAX)) break;
+      acc += hist[b];
+      thr = uint32_t(b);
+    }
+    s_thr = thr;
+    s_cnt = 0;
+  }
+  __syncthreads();
+  const uint32_t thr = s_thr;
+  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+    const uint32_t L = len[i];
+    const uint32_t Lb = L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1;
+    if (L >= HOT_MIN_COUNT && Lb >= thr) {
+      const int32_t idx = int32_t(atomicAdd(&s_cnt, 1u));
+      const int32_t key = sk[i];
+      hot_keys[idx] = key;
+      uint32_t h = hot_hash(key);
+      while (atomicCAS(&t_idx[h], -1, idx) != -1) h = (h + 1) & (HOT_TBL - 1);
+      t_key[h] = key;
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) table[b] = make_int2(t_key[b], t_idx[b]);
+  if (threadIdx.x == 0) *hot_n = s_cnt;
+}
+
+__device__ __forceinline__ void slot_add(GroupSlot* p, int32_t x) {
+  atomicAdd(&p->sum, (u64)(long long)x);
+  atomicAdd(&p->cnt, 1u);
+  // min / max only move one way, so a stale read can only cause a
+  // redundant atomic, never a missed one.
+  if (x < p->mn) atomicMin(&p->mn, x);
+  if (x > p->mx) atomicMax(&p->mx, x);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(ACC_BLOCK) void k_group_acc(
+    const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
+    const uint8_t* __restrict__ vm, const uint8_t* __restrict__ km, uint32_t n, int32_t kmin,
+    GroupSlot* __restrict__ slots, const int32_t* __restrict__ code,
+    const int2* __restrict__ table, const int32_t* __restrict__ hot_keys,
+    const uint32_t* __restrict__ hot_n) {
+  __shared__ int32_t t_key[HOT_TBL];
+  __shared__ int32_t t_idx[HOT_TBL];
+  __shared__ u64 a_sum[HOT_MAX];
+  __shared__ uint32_t a_cnt[HOT_MAX];
+  __shared__ int32_t a_mn[HOT_MAX];
+  __shared__ int32_t a_mx[HOT_MAX];
+  const uint32_t nh = table ? *hot_n : 0u;
+  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) {
+    const int2 e = table ? table[b] : make_int2(0, -1);
+    t_key[b] = e.x;
+    t_idx[b] = e.y;
+  }
+  for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
+    a_sum[b] = 0;
+    a_cnt[b] = 0;
+    a_mn[b] = INT32_MAX;
+    a_mx[b] = INT32_MIN;
+  }
+  __syncthreads();
+  auto row = [&](int32_t k, int32_t x, bool ok, uint32_t s) {
+    if (!ok) return;
+    if (nh != 0) {
+      uint32_t h = hot_hash(k);
+      for (;;) {
+        const int32_t idx = t_idx[h];
+        if (idx < 0) break;
+        if (t_key[h] == k) {
+          atomicAdd(&a_sum[idx], (u64)(long long)x);
+          atomicAdd(&a_cnt[idx], 1u);
+          atomicMin(&a_mn[idx], x);
+          atomicMax(&a_mx[idx], x);
+          return;
+        }
+        h = (h + 1) & (HOT_TBL - 1);
+      }
+    }
+    slot_add(slots + s, x);
+  };
+  if (code == nullptr) {
+    for_each_row<VEC>(keys, vals, vm, km, n, [&](int32_t k, int32_t x, bool ok) {
+      row(k, x, ok, uint32_t(k - kmin));
+    });
+  } else {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+      row(keys[i], vals[i], true, uint32_t(code[i]));
+  }
+  if (nh == 0) return;
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x) {
+    const uint32_t c = a_cnt[b];
+    if (c == 0) continue;
+    GroupSlot* p = slots + uint32_t(hot_keys[b] - kmin);
+    atomicAdd(&p->sum, a_sum[b]);
+    atomicAdd(&p->cnt, c);
+    atomicMin(&p->mn, a_mn[b]);
+    atomicMax(&p->mx, a_mx[b]);
+  }
+}
+
+struct SlotCompactOp {
+  const GroupSlot* slots;
+  const int32_t* ukey_of;  // sparse path: unique value per code (else null)
+  int32_t kmin;
+  GroupAggOut o;
+  int64_t* ng;
+  __device__ uint32_t value(uint32_t s) const { return slots[s].cnt != 0 ? 1u : 0u; }
+  __device__ void emit(uint32_t s, uint32_t ex) const {
+    const GroupSlot sl = slots[s];
+    if (sl.cnt == 0) return;
+    const int64_t sum = (int64_t)sl.sum;
+    if (o.ukeys) o.ukeys[ex] = ukey_of ? int64_t(ukey_of[s]) : int64_t(kmin) + int64_t(s);
+    if (o.counts) o.counts[ex] = int64_t(sl.cnt);
+    if (o.sums) o.sums[ex] = sum;
+    if (o.mins) o.mins[ex] = sl.mn;
+    if (o.maxs) o.maxs[ex] = sl.mx;
+    if (o.means) o.means[ex] = double(sum) / double(sl.cnt);
+  }
+  __device__ void total(uint32_t t) const { *ng = int64_t(t); }
+};
+
+// Compact the valid rows (keys + values) in row order.
+struct ValidRowsOp {
+  const int32_t* keys;
+  const int32_t* vals;
+  const uint8_t* v;
+  const uint8_t* kv;
+  int32_t* ck;
+  int32_t* cv;
+  uint32_t* m;
+  __device__ bool ok(uint32_t i) const {
+    return (v == nullptr || v[i] != 0) && (kv == nullptr || kv[i] != 0);
+  }
+  __device__ uint32_t value(uint32_t i) const { return ok(i) ? 1u : 0u; }
+  __device__ void emit(uint32_t i, uint32_t ex) const {
+    if (ok(i)) {
+      ck[ex] = keys[i];
+      if (cv) cv[ex] = vals[i];
+    }
+  }
+  __device__ void total(uint32_t t) const { *m = t; }
+};
+
+bool key_range(const int32_t* keys, const uint8_t* v, const uint8_t* kv, size_t n, int32_t* kmin,
+               int32_t* kmax, hipStream_t s) {
+  int* r = scratch_t<int>(S_MISC, 2);
+  fill_u32(r, uint32_t(INT32_MAX), 1, s);
+  fill_u32(r + 1, uint32_t(INT32_MIN), 1, s);
+  {
+    ProfScope prof("key_range", s);
+    const unsigned grid = grid_for(n / 4 + 1, 256, 8 * num_cus());
+    if (vec_ok(keys, nullptr, v, kv))
+      hipLaunchKernelGGL(k_key_range<true>, dim3(grid), dim3(256), 0, s, keys, v, kv,
+                         uint32_t(n), r);
+    else
+      hipLaunchKernelGGL(k_key_range<false>, dim3(grid), dim3(256), 0, s, keys, v, kv,
+                         uint32_t(n), r);
+    NFH_LAUNCH_CHECK();
+  }
+  int* h = static_cast<int*>(pinned(8));
+  NFH_CHECK(hipMemcpyAsync(h, r, 8, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  *kmin = h[0];
+  *kmax = h[1];
+  return h[0] <= h[1];
+}
+
+size_t dense_span_limit(size_t n) {
+  const size_t lim = std::max<size_t>(size_t(1) << 25, 2 * n);
+  return std::min<size_t>(lim, size_t(1) << 28);
+}
+
+// Hot-key table from a strided sample of the keys; returns false when the
+// input is too small for privatisation to pay.
+static bool build_hot_table(const int32_t* keys, size_t n, int2** table, int32_t** hot_keys,
+                            uint32_t** hot_n, hipStream_t s) {
+  if (n < size_t(HOT_SAMPLE) * 8) return false;
+  const uint32_t S = HOT_SAMPLE;
+  unsigned char* base = static_cast<unsigned char*>(
+      scratch(S_WORK3, size_t(S) * 4 * 7 + HOT_TBL * 8 + HOT_MAX * 4 + 256));
+  int32_t* sample = reinterpret_cast<int32_t*>(base);
+  int32_t* sorted = sample + S;
+  int32_t* perm = sorted + S;
+  uint32_t* t0 = reinterpret_cast<uint32_t*>(perm + S);
+  uint32_t* t1 = t0 + S;
+  int32_t* tp = reinterpret_cast<int32_t*>(t1 + S);
+  uint32_t* len = reinterpret_cast<uint32_t*>(tp + S);
+  *table = reinterpret_cast<int2*>(len + S);
+  *hot_keys = reinterpret_cast<int32_t*>(*table + HOT_TBL);
+  *hot_n = reinterpret_cast<uint32_t*>(*hot_keys + HOT_MAX);
+  hipLaunchKernelGGL(k_sample, dim3(S / 256), dim3(256), 0, s, keys, uint32_t(n), S, sample);
+  NFH_LAUNCH_CHECK();
+  radix_argsort<int32_t>(sample, S, false, perm, t0, t1, tp, sorted, s);
+  hipLaunchKernelGGL(k_hot_runs, dim3(S / 256), dim3(256), 0, s, sorted, S, len);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_hot_pick, dim3(1), dim3(1024), 0, s, sorted, len, S, *hot_keys, *table,
+                     *hot_n);
+  NFH_LAUNCH_CHECK();
+  return true;
+}
+
+int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t* v,
+                         const uint8_t* kv, size_t n, const GroupAggOut& o, hipStream_t s) {
+  if (n == 0) return 0;
+  int32_t kmin, kmax;
+  if (!key_range(keys, v, kv, n, &kmin, &kmax, s)) return 0;
+  const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
+  int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
+  if (span <= dense_span_limit(n)) {
+    GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
+    hipLaunchKernelGGL(k_slot_init, dim3(grid_for(span, 256, 16 * num_cus())), dim3(256), 0, s,
+                       slots, span);
+    NFH_LAUNCH_CHECK();
+    int2* table = nullptr;
+    int32_t* hot_keys = nullptr;
+    uint32_t* hot_n = nullptr;
+    if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
+    {
+      ProfScope prof("group_acc", s);
+      const unsigned grid = grid_for(n / 4 + 1, ACC_BLOCK, 4 * num_cus());
+      if (vec_ok(keys, vals, v, kv))
+        hipLaunchKernelGGL(k_group_acc<true>, dim3(grid), dim3(ACC_BLOCK), 0, s, keys, vals, v,
+                           kv, uint32_t(n), kmin, slots, nullptr, table, hot_keys, hot_n);
+      else
+        hipLaunchKernelGGL(k_group_acc<false>, dim3(grid), dim3(ACC_BLOCK), 0, s, keys, vals, v,
+                           kv, uint32_t(n), kmin, slots, nullptr, table, hot_keys, hot_n);
+      NFH_LAUNCH_CHECK();
+    }
+    scan_tiles(SlotCompactOp{slots, nullptr, kmin, o, ng_d}, span, s);
+    return read_scalar(ng_d, s);
+  }
+  // sparse: compact valid rows, radix unique, aggregate by code
+  int32_t* ck = scratch_t<int32_t>(S_WORK1, n);
+  int32_t* cv = scratch_t<int32_t>(S_WORK3, n);
+  uint32_t* m_d = scratch_t<uint32_t>(S_MISC, 4) + 2;
+  scan_tiles(ValidRowsOp{keys, vals, v, kv, ck, cv, m_d}, n, s);
+  const uint32_t m = read_scalar(m_d, s);
+  int32_t* uniq = scratch_t<int32_t>(S_T0, m);
+  int32_t* inv = scratch_t<int32_t>(S_T1, m);
+  int32_t* perm = scratch_t<int32_t>(S_T2, m);
+  uint32_t* t0 = scratch_t<uint32_t>(S_T3, m);
+  uint32_t* t1 = scratch_t<uint32_t>(S_T4, m);
+  int32_t* tp = scratch_t<int32_t>(S_T5, m);
+  const int64_t ng = unique_inverse_device<int32_t>(ck, m, uniq, inv, perm, t0, t1, tp, s);
+  GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, size_t(ng));
+  hipLaunchKernelGGL(k_slot_init, dim3(grid_for(ng, 256, 16 * num_cus())), dim3(256), 0, s,
+                     slots, u64(ng));
+  NFH_LAUNCH_CHECK();
+  if (m > 0) {
+    hipLaunchKernelGGL(k_group_acc<false>, dim3(grid_for(m, ACC_BLOCK, 4 * num_cus())),
+                       dim3(ACC_BLOCK), 0, s, ck, cv, nullptr, nullptr, m, 0, slots, inv,
+                       nullptr, nullptr, nullptr);
+    NFH_LAUNCH_CHECK();
+  }
+  scan_tiles(SlotCompactOp{slots, uniq, 0, o, ng_d}, size_t(ng), s);
+  return read_scalar(ng_d, s);
+}
+
+// ---------------------------------------------------------------- unique
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
+                                              const uint8_t* __restrict__ v, uint32_t n,
+                                              int32_t kmin, uint32_t* __restrict__ flag) {
+  for_each_row<VEC>(keys, nullptr, v, nullptr, n, [&](int32_t k, int32_t, bool ok) {
+    if (!ok) return;
+    uint32_t* p = flag + uint32_t(k - kmin);
+    if (*p == 0u) *p = 1u;  // set-once flag: most rows only read
+  });
+}
+
+struct RankOp {
+  uint32_t* flag;  // in: presence, out: rank (dense code)
+  int32_t kmin;
+  int32_t* uniq;
+  int64_t* ng;
+  __device__ uint32_t value(uint32_t s) const { return flag[s]; }
+  __device__ void emit(uint32_t s, uint32_t ex) const {
+    if (flag[s]) uniq[ex] = kmin + int32_t(s);
+    flag[s] = ex;
+  }
+  __device__ void total(uint32_t t) const { *ng = int64_t(t); }
+};
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_inv_dense(const int32_t* __restrict__ keys,
+                                                   const uint8_t* __restrict__ v, uint32_t n,
+                                                   int32_t kmin, const uint32_t* __restrict__ rank,
+                                                   int32_t* __restrict__ inv) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t tail = 0;
+  if constexpr (VEC) {
+    const uint32_t nq = n / 4;
+    for (uint32_t q = tid; q < nq; q += stride) {
+      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
+      const uint32_t m = v ? reinterpret_cast<const uint32_t*>(v)[q] : 0x01010101u;
+      int4 o;
+      o.x = (m & 0xFFu) ? int32_t(rank[uint32_t(k4.x - kmin)]) : -1;
+      o.y = (m & 0xFF00u) ? int32_t(rank[uint32_t(k4.y - kmin)]) : -1;
+      o.z = (m & 0xFF0000u) ? int32_t(rank[uint32_t(k4.z - kmin)]) : -1;
+      o.w = (m & 0xFF000000u) ? int32_t(rank[uint32_t(k4.w - kmin)]) : -1;
+      reinterpret_cast<int4*>(inv)[q] = o;
+    }
+    tail = nq * 4;
+  }
+  for (uint32_t i = tail + tid; i < n; i += stride)
+    inv[i] = (v == nullptr || v[i]) ? int32_t(rank[uint32_t(keys[i] - kmin)]) : -1;
+}
+
+__global__ void k_inv_scatter(const uint8_t* __restrict__ v, uint32_t n,
+                              const int32_t* __restrict__ cinv, const uint32_t* __restrict__ pos,
+                              int32_t* __restrict__ inv) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    inv[i] = v[i] ? cinv[pos[i]] : -1;
+}
+
+struct ValidPosOp {
+  const int32_t* keys;
+  const uint8_t* v;
+  int32_t* ck;
+  uint32_t* pos;
+  uint32_t* m;
+  __device__ uint32_t value(uint32_t i) const { return v[i] ? 1u : 0u; }
+  __device__ void emit(uint32_t i, uint32_t ex) const {
+    pos[i] = ex;
+    if (v[i]) ck[ex] = keys[i];
+  }
+  __device__ void total(uint32_t t) const { *m = t; }
+};
+
+int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int32_t* uniq,
+                            int32_t* inv, hipStream_t s) {
+  if (n == 0) return 0;
+  int32_t kmin, kmax;
+  if (!key_range(keys, v, nullptr, n, &kmin, &kmax, s)) {
+    fill_u32(inv, 0xFFFFFFFFu, n, s);
+    return 0;
+  }
+  const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
+  int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
+  if (span <= dense_span_limit(n)) {
+    uint32_t* flag = scratch_t<uint32_t>(S_WORK0, span);
+    fill_u32(flag, 0, span, s);
+    const bool vec = vec_ok(keys, nullptr, v, nullptr) && aligned(inv, 16);
+    const unsigned grid = grid_for(n / 4 + 1, 256, 8 * num_cus());
+    {
+      ProfScope prof("unique_mark", s);
+      if (vec)
+        hipLaunchKernelGGL(k_mark<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
+                           flag);
+      else
+        hipLaunchKernelGGL(k_mark<false>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n),
+                           kmin, flag);
+      NFH_LAUNCH_CHECK();
+    }
+    scan_tiles(RankOp{flag, kmin, uniq, ng_d}, span, s);
+    {
+      ProfScope prof("unique_inv_dense", s);
+      if (vec)
+        hipLaunchKernelGGL(k_inv_dense<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n),
+                           kmin, flag, inv);
+      else
+        hipLaunchKernelGGL(k_inv_dense<false>, dim3(grid), dim3(256), 0, s, keys, v,
+                           uint32_t(n), kmin, flag, inv);
+      NFH_LAUNCH_CHECK();
+    }
+    return read_scalar(ng_d, s);
+  }
+  int32_t* perm = scratch_t<int32_t>(S_T2, n);
+  uint32_t* t0 = scratch_t<uint32_t>(S_T3, n);
+  uint32_t* t1 = scratch_t<uint32_t>(S_T4, n);
+  int32_t* tp = scratch_t<int32_t>(S_T5, n);
+  if (v == nullptr) return unique_inverse_device<int32_t>(keys, n, uniq, inv, perm, t0, t1, tp, s);
+  int32_t* ck = scratch_t<int32_t>(S_WORK1, n);
+  uint32_t* pos = scratch_t<uint32_t>(S_WORK3, n);
+  uint32_t* m_d = scratch_t<uint32_t>(S_MISC, 4) + 2;
+  scan_tiles(ValidPosOp{keys, v, ck, pos, m_d}, n, s);
+  const uint32_t m = read_scalar(m_d, s);
+  int32_t* cinv = scratch_t<int32_t>(S_WORK0, m);
+  const int64_t ng = unique_inverse_device<int32_t>(ck, m, uniq, cinv, perm, t0, t1, tp, s);
+  hipLaunchKernelGGL(k_inv_scatter, dim3(grid_for(n, 256, 16 * num_cus())), dim3(256), 0, s, v,
+                     uint32_t(n), cinv, pos, inv);
+  NFH_LAUNCH_CHECK();
+  return ng;
+}
+
+}  // namespace nfh
+
+using namespace nfh;
+
+// Group-by over raw int32 keys with NULL masks (valid / key_valid may be
+// null = all valid).  Outputs hold `cap` lanes; any output pointer may be
+// null.  Returns ng >= 0, -1 null input, -3 n > i32::MAX, -6 device error,
+// -7 cap < ng (outputs unwritten).
+NF_EXPORT int64_t nfh_group_agg_i32(const int32_t* keys, const int32_t* vals,
+                                    const uint8_t* valid, const uint8_t* key_valid, size_t n,
+                                    int64_t* ukeys, int64_t* counts, int64_t* sums,
+                                    int32_t* mins, int32_t* maxs, double* means, size_t cap) {
+  if (n == 0) return 0;
+  if (!keys || !vals) return NULL_OR_ABORT;
+  if (n > size_t(INT32_MAX)) return MALFORMED;
+  NFH_GUARD_BEGIN
+  Stage st;
+  const int32_t* k = st.in(keys, n, S_IN0);
+  const int32_t* v = st.in(vals, n, S_IN1);
+  const uint8_t* vm = valid ? st.in(valid, n, S_IN2) : nullptr;
+  const uint8_t* km = key_valid ? st.in(key_valid, n, S_IN3) : nullptr;
+  GroupAggOut d{};
+  if (cap >= n) {
+    // ng <= n fits: write straight into the caller's (or staged) buffers.
+    d.ukeys = st.out(ukeys, n, S_OUT0);
+    d.counts = st.out(counts, n, S_OUT1);
+    d.sums = st.out(sums, n, S_OUT2);
+    d.mins = st.out(mins, n, S_OUT3);
+    d.maxs = st.out(maxs, n, S_OUT4);
+    d.means = st.out(means, n, S_OUT5);
+    const int64_t ng = group_agg_device(k, v, vm, km, n, d, st.s);
+    st.set_out_bytes(ukeys, size_t(ng) * 8);
+    st.set_out_bytes(counts, size_t(ng) * 8);
+    st.set_out_bytes(sums, size_t(ng) * 8);
+    st.set_out_bytes(mins, size_t(ng) * 4);
+    st.set_out_bytes(maxs, size_t(ng) * 4);
+    st.set_out_bytes(means, size_t(ng) * 8);
+    st.finish();
+    return ng;
+  }
+  // Device outputs of smaller capacity: the kernels write them directly
+  // when cap covers the group bound known from the key range.
+  const bool all_dev = (!ukeys || is_device_ptr(ukeys)) && (!counts || is_device_ptr(counts)) &&
+                       (!sums || is_device_ptr(sums)) && (!mins || is_device_ptr(mins)) &&
+                       (!maxs || is_device_ptr(maxs)) && (!means || is_device_ptr(means));
+  int32_t kmin = 0, kmax = -1;
+  const bool any = key_range(k, vm, km, n, &kmin, &kmax, st.s);
+  const u64 bound = any ? u64(int64_t(kmax) - int64_t(kmin) + 1) : 0;
+  if (all_dev && bound <= cap) {
+    d.ukeys = ukeys;
+    d.counts = counts;
+    d.sums = sums;
+    d.mins = mins;
+    d.maxs = maxs;
+    d.means = means;
+    const int64_t ng = group_agg_device(k, v, vm, km, n, d, st.s);
+    st.finish();
+    return ng;
+  }
+  // Otherwise results land in device scratch first, then move to the
+  // caller's buffers once ng <= cap is known.
+  d.ukeys = ukeys ? scratch_t<int64_t>(S_OUT0, n) : nullptr;
+  d.counts = counts ? scratch_t<int64_t>(S_OUT1, n) : nullptr;
+  d.sums = sums ? scratch_t<int64_t>(S_OUT2, n) : nullptr;
+  d.mins = mins ? scratch_t<int32_t>(S_OUT3, n) : nullptr;
+  d.maxs = maxs ? scratch_t<int32_t>(S_OUT4, n) : nullptr;
+  d.means = means ? scratch_t<double>(S_OUT5, n) : nullptr;
+  const int64_t ng = group_agg_device(k, v, vm, km, n, d, st.s);
+  if (size_t(ng) > cap) {
+    st.finish();
+    return TOO_SMALL;
+  }
+  auto copy = [&](void* dst, const void* src, size_t bytes) {
+    if (dst && bytes) NFH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st.s));
+  };
+  copy(ukeys, d.ukeys, size_t(ng) * 8);
+  copy(counts, d.counts, size_t(ng) * 8);
+  copy(sums, d.sums, size_t(ng) * 8);
+  copy(mins, d.mins, size_t(ng) * 4);
+  copy(maxs, d.maxs, size_t(ng) * 4);
+  copy(means, d.means, size_t(ng) * 8);
+  st.finish();
+  return ng;
+  NFH_GUARD_END(int64_t)
+}
+
+// np.unique over the valid rows of int32 keys: uniq ascending (first ng
+// lanes), inv[row] = code, -1 on NULL rows (cpu.py:4032-4036).  valid may
+// be null.  Returns ng, -1 null input, -3 n > i32::MAX, -6 device error.
+NF_EXPORT int64_t nfh_unique_valid_i32(const int32_t* keys, const uint8_t* valid, size_t n,
+                                       int32_t* uniq_out, int32_t* inv_out) {
+  if (n == 0) return 0;
+  if (!keys || !uniq_out || !inv_out) return NULL_OR_ABORT;
+  if (n > size_t(INT32_MAX)) return MALFORMED;
+  NFH_GUARD_BEGIN
+  Stage st;
+  const int32_t* k = st.in(keys, n, S_IN0);
+  const uint8_t* vm = valid ? st.in(valid, n, S_IN1) : nullptr;
+  int32_t* u = st.out(uniq_out, n, S_OUT0);
+  int32_t* iv = st.out(inv_out, n, S_OUT1);
+  const int64_t ng = unique_valid_device(k, vm, n, u, iv, st.s);
+  st.set_out_bytes(uniq_out, size_t(ng) * 4);
+  st.finish();
+  return ng;
+  NFH_GUARD_END(int64_t)
+}

@@ -1,5 +1,8 @@
 // runtime.hip -- per-thread device context, scratch pool and staging.
+#include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
 
 #include "runtime.hpp"
 
@@ -155,6 +158,57 @@ void Stage::finish() {
       NFH_CHECK(hipMemcpyAsync(q.host, q.dev, q.bytes, hipMemcpyDeviceToHost, s));
   pend_.clear();
   NFH_CHECK(hipStreamSynchronize(s));
+  prof_collect();
+}
+
+// ---------------------------------------------------------------- profiling
+namespace {
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+struct ProfAcc {
+  double ms = 0;
+  uint64_t launches = 0;
+};
+std::map<std::string, ProfAcc> g_prof;
+struct Pending {
+  const char* name;
+  hipEvent_t a, b;
+};
+thread_local std::vector<Pending> t_pending;
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), s_(s) {
+  if (!g_prof_on) return;
+  NFH_CHECK(hipEventCreate(&a_));
+  NFH_CHECK(hipEventCreate(&b_));
+  NFH_CHECK(hipEventRecord(a_, s_));
+}
+
+ProfScope::~ProfScope() {
+  if (a_ == nullptr) return;
+  if (hipEventRecord(b_, s_) == hipSuccess) {
+    t_pending.push_back({name_, a_, b_});
+  } else {
+    (void)hipEventDestroy(a_);
+    (void)hipEventDestroy(b_);
+  }
+}
+
+void prof_collect() {
+  if (t_pending.empty()) return;
+  std::vector<Pending> p;
+  p.swap(t_pending);
+  for (auto& e : p) {
+    float ms = 0;
+    if (hipEventSynchronize(e.b) == hipSuccess && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      std::lock_guard<std::mutex> g(g_prof_mu);
+      ProfAcc& acc = g_prof[e.name];
+      acc.ms += ms;
+      acc.launches += 1;
+    }
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
 }
 
 // ---------------------------------------------------------------- fills
@@ -212,3 +266,31 @@ NF_EXPORT int32_t nfh_device_ok(void) {
 
 // ABI version of this library's additive surface.
 NF_EXPORT int32_t nfh_abi_version(void) { return 1; }
+
+// Per-kernel event timing on the launch stream: on != 0 enables it.
+NF_EXPORT int32_t nfh_profile_enable(int32_t on) {
+  std::lock_guard<std::mutex> g(nfh::g_prof_mu);
+  nfh::g_prof_on = on != 0;
+  return 0;
+}
+
+NF_EXPORT int32_t nfh_profile_reset(void) {
+  std::lock_guard<std::mutex> g(nfh::g_prof_mu);
+  nfh::g_prof.clear();
+  return 0;
+}
+
+// Total milliseconds and launch count recorded for kernel `name`; returns 0
+// when the name is known, -1 otherwise.
+NF_EXPORT int32_t nfh_profile_get(const char* name, double* ms, uint64_t* launches) {
+  try {
+    nfh::prof_collect();
+  } catch (...) {
+  }
+  std::lock_guard<std::mutex> g(nfh::g_prof_mu);
+  auto it = nfh::g_prof.find(name ? std::string(name) : std::string());
+  if (it == nfh::g_prof.end()) return -1;
+  if (ms) *ms = it->second.ms;
+  if (launches) *launches = it->second.launches;
+  return 0;
+}

@@ -148,6 +148,21 @@ void fill_u64(void* p, uint64_t v, size_t count, hipStream_t s);
 
 int num_cus();
 
+// Optional per-kernel timing with HIP events on the launch stream
+// (nfh_profile_enable).  A scope brackets one kernel launch; durations are
+// collected after the stream is synchronised (Stage::finish / prof_collect).
+class ProfScope {
+ public:
+  ProfScope(const char* name, hipStream_t s);
+  ~ProfScope();
+
+ private:
+  const char* name_;
+  hipStream_t s_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+void prof_collect();
+
 }  // namespace nfh
 
 // Wrap an entry point body: every exception becomes a return code.

@@ -74,6 +74,7 @@ void scan_tiles(const Op& op, size_t n, hipStream_t s) {
   const uint32_t epoch = lookback_epoch(s);
   uint32_t* ctr = scratch_t<uint32_t>(S_META2, 8) + 4;
   fill_u32(ctr, 0, 1, s);
+  ProfScope prof("scan_tiles", s);
   hipLaunchKernelGGL(k_scan_tiles<Op>, dim3(tiles), dim3(SC_BLOCK), 0, s, op, uint32_t(n), lb,
                      ctr, epoch);
   NFH_LAUNCH_CHECK();

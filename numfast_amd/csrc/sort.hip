@@ -251,6 +251,7 @@ static void launch_pass(const K* keys, const typename Key<K>::W* win, const int3
                         int shift, bool desc, hipStream_t s) {
   const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
   const uint32_t epoch = lookback_epoch(s);
+  ProfScope prof("radix_onesweep", s);
   hipLaunchKernelGGL((k_onesweep<K, FIRST, LAST>), dim3(tiles), dim3(SORT_BLOCK), 0, s, keys,
                      win, iin, wout, iout, kout, bin_start, lb, ctr, n, shift, desc, epoch);
   NFH_LAUNCH_CHECK();
@@ -272,8 +273,11 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
   uint32_t* triv = ctrs + P;
   fill_u32(meta, 0, size_t(P) * 512 + 2 * P, s);
   const unsigned hb = grid_for(n, 256, 4u * unsigned(num_cus()));
-  hipLaunchKernelGGL((k_hist<K>), dim3(hb), dim3(256), 0, s, keys, n, desc, hist);
-  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("radix_hist", s);
+    hipLaunchKernelGGL((k_hist<K>), dim3(hb), dim3(256), 0, s, keys, n, desc, hist);
+    NFH_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_bins, dim3(P), dim3(256), 0, s, hist, n, bins, triv);
   NFH_LAUNCH_CHECK();
   uint32_t* h_triv = static_cast<uint32_t*>(pinned(P * sizeof(uint32_t)));

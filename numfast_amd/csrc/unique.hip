@@ -98,9 +98,12 @@ int64_t unique_inverse_device(const K* keys, size_t n_, K* uniq, int32_t* inv, i
   uint32_t* meta = scratch_t<uint32_t>(S_META2, 4);
   fill_u32(meta, 0, 4, s);
   int64_t* ng_dev = reinterpret_cast<int64_t*>(meta + 2);
-  hipLaunchKernelGGL((k_run_encode<K>), dim3(tiles), dim3(RE_BLOCK), 0, s, sorted, perm, n,
-                     uniq, inv, lb, meta, epoch, ng_dev);
-  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("unique_run_encode", s);
+    hipLaunchKernelGGL((k_run_encode<K>), dim3(tiles), dim3(RE_BLOCK), 0, s, sorted, perm, n,
+                       uniq, inv, lb, meta, epoch, ng_dev);
+    NFH_LAUNCH_CHECK();
+  }
   return read_scalar(ng_dev, s);
 }
 

@@ -79,6 +79,11 @@ GROUPS = {
         ("nfh_set_stream", [_vp], _i32),
         ("nfh_device_ok", [], _i32),
         ("nfh_abi_version", [], _i32),
+        ("nfh_group_agg_i32", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
+        ("nfh_unique_valid_i32", [_vp, _vp, _sz, _vp, _vp], _i64),
+        ("nfh_profile_enable", [_i32], _i32),
+        ("nfh_profile_reset", [], _i32),
+        ("nfh_profile_get", [ctypes.c_char_p, _vp, _vp], _i32),
     ],
 }
 
@@ -522,3 +527,39 @@ def sort_perm_step(keys, descending):
         if ok_d and st_d:
             return np.arange(n - 1, -1, -1, dtype=np.int32), "rev"
     return sort_perm_raw(k, descending), "native"
+
+
+# ------------------------------------------------------------ additive surface
+def group_agg(keys, vals, valid=None, key_valid=None):
+    """Group-by with sum/count/min/max/mean and NULL masks (nfh_group_agg_i32).
+
+    Returns a dict shaped like ColumnCarry (carry.py:21-62): ukeys int64
+    ascending, counts int64, sums int64, mins/maxs int32, means float64.
+    """
+    keys, vals = _c(keys, np.int32), _c(vals, np.int32)
+    n = keys.size
+    L = lib("hip")
+    vm = None if valid is None else _c(valid, np.uint8)
+    km = None if key_valid is None else _c(key_valid, np.uint8)
+    out = {"ukeys": np.zeros(n, np.int64), "counts": np.zeros(n, np.int64),
+           "sums": np.zeros(n, np.int64), "mins": np.zeros(n, np.int32),
+           "maxs": np.zeros(n, np.int32), "means": np.zeros(n, np.float64)}
+    ng = int(L.nfh_group_agg_i32(_p(keys), _p(vals), _p(vm), _p(km), n, _p(out["ukeys"]),
+                                 _p(out["counts"]), _p(out["sums"]), _p(out["mins"]),
+                                 _p(out["maxs"]), _p(out["means"]), n))
+    if ng < 0:
+        raise RuntimeError("nfh_group_agg_i32 rc=%d" % ng)
+    return {k: v[:ng] for k, v in out.items()}
+
+
+def unique_valid(keys, valid=None):
+    """np.unique over valid rows, inv = -1 on NULL rows (nfh_unique_valid_i32)."""
+    keys = _c(keys, np.int32)
+    n = keys.size
+    L = lib("hip")
+    vm = None if valid is None else _c(valid, np.uint8)
+    uniq, inv = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    ng = int(L.nfh_unique_valid_i32(_p(keys), _p(vm), n, _p(uniq), _p(inv)))
+    if ng < 0:
+        raise RuntimeError("nfh_unique_valid_i32 rc=%d" % ng)
+    return uniq[:ng], inv
