@@ -109,6 +109,7 @@ static int64_t sorted_run_entry(const int32_t* keys, const V* vals, size_t n, in
   sp.hi = size_t(ng);
   sp.counts = ct;
   sp.presorted = true;
+  sp.f64_seq_first = true;
   if constexpr (std::is_same<V, int64_t>::value) {
     sp.ni64 = 1;
     sp.v64[0] = v;

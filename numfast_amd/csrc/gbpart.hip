@@ -1,0 +1,422 @@
+// gbpart.hip -- partitioned dense group-by (sum / count / min / max) for
+// large key spans, with no per-row global atomics.
+//
+// Device-scope atomics on MI355X execute beside memory at a fixed rate
+// (about 25 G/s whatever the footprint), so a row-per-atomic scatter
+// cannot approach HBM bandwidth.  Instead:
+//   1. hot keys (sampled, see groupagg.hip) are aggregated per workgroup in
+//      LDS during the bucket-histogram pass and merged once at the end;
+//   2. every other valid row becomes a record (o = key - kmin, value) and
+//      one or two onesweep passes (decoupled look-back, 7-bit digits of
+//      o >> 11) order the records by 2048-key bucket (LSD, stable);
+//   3. one workgroup per bucket aggregates its records in LDS and writes the
+//      bucket's 2048 slots with plain coalesced stores;
+//   4. the hot aggregates are added to their slots; a scan compacts the
+//      present groups (groupagg.hip).
+// Histogram of buckets first, so every pass knows its global bin starts.
+#include <algorithm>
+
+#include "device.hpp"
+#include "gbpart.hpp"
+#include "runtime.hpp"
+#include "sort.hpp"
+
+namespace nfh {
+
+typedef unsigned long long u64;
+
+constexpr int GP_BLOCK = 256;
+constexpr int GP_WAVES = GP_BLOCK / WAVE;
+constexpr int GP_IPT = 16;
+constexpr int GP_TILE = GP_BLOCK * GP_IPT;
+constexpr int GP_HOT_MAX = 512;
+constexpr int GP_HOT_TBL = 1024;
+constexpr int GP_HOT_BITS = 10;
+
+__device__ __forceinline__ uint32_t gp_hash(int32_t k) {
+  return (uint32_t(k) * 0x9E3779B1u) >> (32 - GP_HOT_BITS);
+}
+
+// LDS hot table: returns the hot index of k or -1.
+__device__ __forceinline__ int gp_hot_find(const int32_t* t_key, const int32_t* t_idx, int32_t k) {
+  uint32_t h = gp_hash(k);
+  for (;;) {
+    const int32_t idx = t_idx[h];
+    if (idx < 0) return -1;
+    if (t_key[h] == k) return idx;
+    h = (h + 1) & (GP_HOT_TBL - 1);
+  }
+}
+
+__device__ __forceinline__ void gp_load_table(const int2* table, int32_t* t_key, int32_t* t_idx) {
+  for (int b = threadIdx.x; b < GP_HOT_TBL; b += blockDim.x) {
+    const int2 e = table ? table[b] : make_int2(0, -1);
+    t_key[b] = e.x;
+    t_idx[b] = e.y;
+  }
+}
+
+// ------------------------------------------------------------ histogram
+struct HotAcc {
+  u64 sum;
+  uint32_t cnt;
+  int32_t mn;
+  int32_t mx;
+  uint32_t pad;
+};
+
+// One persistent pass over the rows: hot rows aggregate in LDS (flushed
+// once per workgroup), every other valid row is counted in its 2048-key
+// bucket.
+__global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
+    const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
+    const uint8_t* __restrict__ vm, uint32_t n, int32_t kmin, uint32_t nb,
+    const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc) {
+  extern __shared__ uint32_t s_h[];  // nb counters
+  __shared__ int32_t t_key[GP_HOT_TBL], t_idx[GP_HOT_TBL];
+  __shared__ u64 h_sum[GP_HOT_MAX];
+  __shared__ uint32_t h_cnt[GP_HOT_MAX];
+  __shared__ int32_t h_mn[GP_HOT_MAX], h_mx[GP_HOT_MAX];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) s_h[b] = 0;
+  gp_load_table(table, t_key, t_idx);
+  for (int b = threadIdx.x; b < GP_HOT_MAX; b += blockDim.x) {
+    h_sum[b] = 0;
+    h_cnt[b] = 0;
+    h_mn[b] = INT32_MAX;
+    h_mx[b] = INT32_MIN;
+  }
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (vm && !vm[i]) continue;
+    const int32_t k = keys[i];
+    const int hi = table ? gp_hot_find(t_key, t_idx, k) : -1;
+    if (hi >= 0) {
+      const int32_t x = vals[i];
+      atomicAdd(&h_sum[hi], (u64)(long long)x);
+      atomicAdd(&h_cnt[hi], 1u);
+      atomicMin(&h_mn[hi], x);
+      atomicMax(&h_mx[hi], x);
+    } else {
+      atomicAdd(&s_h[uint32_t(k - kmin) >> GP_BUCKET_BITS], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
+  if (table) {
+    for (int b = threadIdx.x; b < GP_HOT_MAX; b += blockDim.x) {
+      const uint32_t c = h_cnt[b];
+      if (c == 0) continue;
+      atomicAdd(&hot_acc[b].sum, h_sum[b]);
+      atomicAdd(&hot_acc[b].cnt, c);
+      atomicMin(&hot_acc[b].mn, h_mn[b]);
+      atomicMax(&hot_acc[b].mx, h_mx[b]);
+    }
+  }
+}
+
+// Bucket starts (exclusive scan of hist) and the per-digit bin starts of
+// the partition passes (digit p = bits [7p, 7p+7) of the bucket id).
+__global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ hist, uint32_t nb,
+                                                  uint32_t* __restrict__ bstart,
+                                                  uint32_t* __restrict__ dstart /* 2 x 128 */) {
+  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_d[2][128];
+  if (threadIdx.x < 256) s_d[threadIdx.x >> 7][threadIdx.x & 127] = 0;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += blockDim.x) {
+    const uint32_t b = base + threadIdx.x;
+    const uint32_t c = b < nb ? hist[b] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<16>(c, s_tmp, &tot);
+    if (b < nb) {
+      bstart[b] = s_carry + ex;
+      if (c) {
+        atomicAdd(&s_d[0][b & 127], c);
+        atomicAdd(&s_d[1][(b >> 7) & 127], c);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bstart[nb] = s_carry;
+  if (threadIdx.x < 2) {
+    uint32_t run = 0;
+    for (int d = 0; d < 128; ++d) {
+      const uint32_t c = s_d[threadIdx.x][d];
+      dstart[threadIdx.x * 128 + d] = run;
+      run += c;
+    }
+  }
+}
+
+// ------------------------------------------------------------ partition pass
+
+// One onesweep pass over records ordered by digit (o >> (11 + 7p)) & 127.
+// FIRST reads raw rows: NULL rows and hot rows (already aggregated by
+// k_gp_hist) are dropped, the others become records.
+template <bool FIRST>
+__global__ __launch_bounds__(GP_BLOCK) void k_gp_part(
+    const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
+    const uint8_t* __restrict__ vm, uint32_t n_rows, int32_t kmin,
+    const uint32_t* __restrict__ oin, const int32_t* __restrict__ vin, uint32_t n_rec,
+    uint32_t* __restrict__ oout, int32_t* __restrict__ vout, int shift,
+    const uint32_t* __restrict__ dstart, uint64_t* __restrict__ lookback,
+    uint32_t* __restrict__ tile_ctr, uint32_t epoch, const int2* __restrict__ table) {
+  __shared__ uint32_t s_o[GP_TILE];
+  __shared__ int32_t s_v[GP_TILE];
+  __shared__ uint32_t s_wh[GP_WAVES][128];
+  __shared__ uint32_t s_tstart[128];
+  __shared__ uint32_t s_gbase[128];
+  __shared__ uint32_t s_tmp[GP_WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ int32_t t_key[FIRST ? GP_HOT_TBL : 1], t_idx[FIRST ? GP_HOT_TBL : 1];
+
+  const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  for (int w = 0; w < GP_WAVES; ++w)
+    if (tid < 128) s_wh[w][tid] = 0;
+  if constexpr (FIRST) gp_load_table(table, t_key, t_idx);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t n = FIRST ? n_rows : n_rec;
+  const uint32_t base = tile * uint32_t(GP_TILE);
+  const uint32_t wbase = base + uint32_t(wave) * (WAVE * GP_IPT);
+
+  uint32_t ov[GP_IPT];
+  int32_t vv[GP_IPT];
+  uint32_t live = 0;  // bit j: item j is a record of this pass
+#pragma unroll
+  for (int j = 0; j < GP_IPT; ++j) {
+    const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+    ov[j] = 0;
+    vv[j] = 0;
+    if (pos < n) {
+      if constexpr (FIRST) {
+        const bool ok = vm == nullptr || vm[pos] != 0;
+        if (ok) {
+          const int32_t k = keys[pos];
+          if (!table || gp_hot_find(t_key, t_idx, k) < 0) {
+            ov[j] = uint32_t(k - kmin);
+            vv[j] = vals[pos];
+            live |= 1u << j;
+          }
+        }
+      } else {
+        ov[j] = oin[pos];
+        vv[j] = vin[pos];
+        live |= 1u << j;
+      }
+    }
+  }
+
+  uint32_t rk[GP_IPT];
+#pragma unroll
+  for (int j = 0; j < GP_IPT; ++j) {
+    const bool valid = (live >> j) & 1u;
+    const uint32_t d = (ov[j] >> shift) & 127u;
+    const uint64_t peers = match_digit8(d, valid);
+    const uint32_t r = __popcll(peers & lanemask_lt());
+    const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+    uint32_t old = 0;
+    if (valid && r == 0) {
+      old = s_wh[wave][d];
+      s_wh[wave][d] = old + uint32_t(__popcll(peers));
+    }
+    old = __shfl(old, leader);
+    rk[j] = old + r;
+  }
+  __syncthreads();
+
+  uint32_t cnt = 0;
+  if (tid < 128) {
+#pragma unroll
+    for (int w = 0; w < GP_WAVES; ++w) {
+      const uint32_t c = s_wh[w][tid];
+      s_wh[w][tid] = cnt;
+      cnt += c;
+    }
+    lb_store(lookback + size_t(tile) * 128 + tid,
+             lb_pack(epoch, tile == 0 ? LB_PREFIX : LB_AGG, cnt));
+  }
+  uint32_t tile_total;
+  const uint32_t tstart = block_excl_scan<GP_WAVES>(tid < 128 ? cnt : 0u, s_tmp, &tile_total);
+  if (tid < 128) s_tstart[tid] = tstart;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < GP_IPT; ++j) {
+    if ((live >> j) & 1u) {
+      const uint32_t d = (ov[j] >> shift) & 127u;
+      const uint32_t p = s_tstart[d] + s_wh[wave][d] + rk[j];
+      s_o[p] = ov[j];
+      s_v[p] = vv[j];
+    }
+  }
+  if (tid < 128) {
+    uint32_t excl = 0;
+    if (tile > 0) {
+      excl = lb_lookback(lookback + tid, tile, 128, epoch);
+      lb_store(lookback + size_t(tile) * 128 + tid, lb_pack(epoch, LB_PREFIX, excl + cnt));
+    }
+    s_gbase[tid] = dstart[tid] + excl - tstart;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < tile_total; i += GP_BLOCK) {
+    const uint32_t o = s_o[i];
+    const uint32_t g = s_gbase[(o >> shift) & 127u] + i;
+    oout[g] = o;
+    vout[g] = s_v[i];
+  }
+}
+
+// ------------------------------------------------------------ bucket aggregation
+// One workgroup per 2048-key bucket: LDS aggregation of its records, then
+// plain stores of all of the bucket's slots (empty ones included).
+__global__ __launch_bounds__(GP_BLOCK) void k_gp_bucket(const uint32_t* __restrict__ o,
+                                                        const int32_t* __restrict__ v,
+                                                        const uint32_t* __restrict__ bstart,
+                                                        uint64_t span, GpSlot* __restrict__ slots) {
+  constexpr int W = 1 << GP_BUCKET_BITS;
+  __shared__ u64 a_sum[W];
+  __shared__ uint32_t a_cnt[W];
+  __shared__ int32_t a_mn[W], a_mx[W];
+  const uint32_t b = blockIdx.x;
+  for (int i = threadIdx.x; i < W; i += GP_BLOCK) {
+    a_sum[i] = 0;
+    a_cnt[i] = 0;
+    a_mn[i] = INT32_MAX;
+    a_mx[i] = INT32_MIN;
+  }
+  __syncthreads();
+  const uint32_t lo = bstart[b], hi = bstart[b + 1];
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += GP_BLOCK) {
+    const uint32_t s = o[i] & (W - 1);
+    const int32_t x = v[i];
+    atomicAdd(&a_sum[s], (u64)(long long)x);
+    atomicAdd(&a_cnt[s], 1u);
+    atomicMin(&a_mn[s], x);
+    atomicMax(&a_mx[s], x);
+  }
+  __syncthreads();
+  const uint64_t first = uint64_t(b) << GP_BUCKET_BITS;
+  for (int i = threadIdx.x; i < W; i += GP_BLOCK) {
+    const uint64_t g = first + uint64_t(i);
+    if (g >= span) break;
+    GpSlot sl;
+    sl.sum = a_sum[i];
+    sl.cnt = a_cnt[i];
+    sl.mn = a_mn[i];
+    sl.mx = a_mx[i];
+    sl.pad[0] = sl.pad[1] = sl.pad[2] = 0;
+    slots[g] = sl;
+  }
+}
+
+__global__ void k_gp_hotmerge(const HotAcc* __restrict__ hot_acc,
+                              const int32_t* __restrict__ hot_keys,
+                              const uint32_t* __restrict__ hot_n, int32_t kmin,
+                              GpSlot* __restrict__ slots) {
+  const uint32_t nh = *hot_n;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+    const HotAcc a = hot_acc[h];
+    if (a.cnt == 0) continue;
+    GpSlot* p = slots + uint32_t(hot_keys[h] - kmin);
+    p->sum += a.sum;
+    p->cnt += a.cnt;
+    p->mn = a.mn < p->mn ? a.mn : p->mn;
+    p->mx = a.mx > p->mx ? a.mx : p->mx;
+  }
+}
+
+__global__ void k_gp_hotinit(HotAcc* __restrict__ h, uint32_t m) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    HotAcc z;
+    z.sum = 0;
+    z.cnt = 0;
+    z.mn = INT32_MAX;
+    z.mx = INT32_MIN;
+    z.pad = 0;
+    h[i] = z;
+  }
+}
+
+bool gp_supported(uint64_t span) { return span <= (uint64_t(1) << (GP_BUCKET_BITS + 13)); }
+
+void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, size_t n_,
+                  int32_t kmin, uint64_t span, const int2* table, const int32_t* hot_keys,
+                  const uint32_t* hot_n, GpSlot* slots, hipStream_t s) {
+  const uint32_t n = uint32_t(n_);
+  const uint32_t nb = uint32_t((span + (1u << GP_BUCKET_BITS) - 1) >> GP_BUCKET_BITS);
+  const int passes = nb <= 1 ? 0 : (nb <= 128 ? 1 : 2);
+  // scratch: hist[nb] | bstart[nb+1] | dstart[256] | ctr[2] | hot_acc
+  const size_t meta_words = size_t(nb) * 2 + 1 + 256 + 4;
+  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + GP_HOT_MAX * 6);
+  uint32_t* hist = meta;
+  uint32_t* bstart = hist + nb;
+  uint32_t* dstart = bstart + nb + 1;
+  uint32_t* ctr = dstart + 256;
+  HotAcc* hot_acc = reinterpret_cast<HotAcc*>(ctr + 4 + (4 - ((meta_words) & 3)));
+  fill_u32(meta, 0, meta_words, s);
+  hipLaunchKernelGGL(k_gp_hotinit, dim3(2), dim3(256), 0, s, hot_acc, uint32_t(GP_HOT_MAX));
+  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("gp_hist", s);
+    const size_t lds = size_t(nb) * 4;
+    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n, GP_BLOCK, 2 * num_cus())), dim3(GP_BLOCK),
+                       lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc);
+    NFH_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart, dstart);
+  NFH_LAUNCH_CHECK();
+  // record buffers (ping-pong): at most n records
+  uint32_t* o0 = scratch_t<uint32_t>(S_T0, n);
+  int32_t* v0 = scratch_t<int32_t>(S_T1, n);
+  uint32_t* o1 = scratch_t<uint32_t>(S_T2, n);
+  int32_t* v1 = scratch_t<int32_t>(S_T3, n);
+  const uint32_t tiles = (n + GP_TILE - 1) / GP_TILE;
+  uint64_t* lb = lookback_words(size_t(tiles) * 128, s);
+  // pass 1 always runs: it drops NULL and hot rows (digit 0 when nb <= 1)
+  {
+    ProfScope prof("gp_part_first", s);
+    const uint32_t epoch = lookback_epoch(s);
+    hipLaunchKernelGGL(k_gp_part<true>, dim3(tiles), dim3(GP_BLOCK), 0, s, keys, vals, vm, n,
+                       kmin, nullptr, nullptr, 0u, o0, v0, GP_BUCKET_BITS, dstart, lb, ctr, epoch,
+                       table);
+    NFH_LAUNCH_CHECK();
+  }
+  const uint32_t* fo = o0;
+  const int32_t* fv = v0;
+  if (passes == 2) {
+    // records of pass 1 are all non-hot valid rows: their count is bstart[nb]
+    uint32_t m = 0;
+    NFH_CHECK(hipMemcpyAsync(&m, bstart + nb, 4, hipMemcpyDeviceToHost, s));
+    NFH_CHECK(hipStreamSynchronize(s));
+    if (m > 0) {
+      ProfScope prof("gp_part", s);
+      const uint32_t t2 = (m + GP_TILE - 1) / GP_TILE;
+      const uint32_t epoch = lookback_epoch(s);
+      hipLaunchKernelGGL(k_gp_part<false>, dim3(t2), dim3(GP_BLOCK), 0, s, nullptr, nullptr,
+                         nullptr, 0u, kmin, o0, v0, m, o1, v1, GP_BUCKET_BITS + 7, dstart + 128,
+                         lb, ctr + 1, epoch, nullptr);
+      NFH_LAUNCH_CHECK();
+    }
+    fo = o1;
+    fv = v1;
+  }
+  {
+    ProfScope prof("gp_bucket", s);
+    hipLaunchKernelGGL(k_gp_bucket, dim3(nb), dim3(GP_BLOCK), 0, s, fo, fv, bstart, span, slots);
+    NFH_LAUNCH_CHECK();
+  }
+  if (table) {
+    hipLaunchKernelGGL(k_gp_hotmerge, dim3(2), dim3(256), 0, s, hot_acc, hot_keys, hot_n, kmin,
+                       slots);
+    NFH_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace nfh

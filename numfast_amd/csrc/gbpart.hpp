@@ -1,0 +1,29 @@
+// gbpart.hpp -- partitioned dense group-by (no per-row global atomics).
+#pragma once
+#include "runtime.hpp"
+
+namespace nfh {
+
+constexpr int GP_BUCKET_BITS = 11;  // 2048 keys per LDS bucket
+
+// One 32-byte group state slot, indexed by key - kmin.
+struct __align__(32) GpSlot {
+  unsigned long long sum;
+  uint32_t cnt;
+  int32_t mn;
+  int32_t mx;
+  uint32_t pad[3];
+};
+static_assert(sizeof(GpSlot) == 32, "slot is one 32-byte sector");
+
+// Spans this path handles (two 7-bit partition passes at most).
+bool gp_supported(uint64_t span);
+
+// Fill slots[0, span) with {sum, count, min, max} of the valid rows (vm may
+// be null).  `table` / `hot_keys` / `hot_n` describe the sampled hot keys
+// (table may be null).  Asynchronous on s except one small readback.
+void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, size_t n,
+                  int32_t kmin, uint64_t span, const int2* table, const int32_t* hot_keys,
+                  const uint32_t* hot_n, GpSlot* slots, hipStream_t s);
+
+}  // namespace nfh

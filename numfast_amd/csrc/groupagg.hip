@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include "device.hpp"
+#include "gbpart.hpp"
 #include "groupagg.hpp"
 #include "runtime.hpp"
 #include "scan.hpp"
@@ -33,18 +34,12 @@ namespace nfh {
 
 typedef unsigned long long u64;
 
-struct __align__(32) GroupSlot {
-  u64 sum;
-  uint32_t cnt;
-  int32_t mn;
-  int32_t mx;
-  uint32_t pad[3];
-};
-static_assert(sizeof(GroupSlot) == 32, "slot is one 32-byte sector");
+using GroupSlot = GpSlot;
 
-constexpr int HOT_MAX = 1024;    // hot keys privatised per workgroup
-constexpr int HOT_TBL = 2048;    // LDS open-addressing table (load <= 0.5)
-constexpr int HOT_TBL_BITS = 11;
+// The hot-key table layout is shared with gbpart.hip (same hash, sizes).
+constexpr int HOT_MAX = 512;     // hot keys privatised per workgroup
+constexpr int HOT_TBL = 1024;    // LDS open-addressing table (load <= 0.5)
+constexpr int HOT_TBL_BITS = 10;
 constexpr int HOT_SAMPLE = 1 << 16;
 constexpr uint32_t HOT_MIN_COUNT = 4;  // sampled occurrences to qualify
 constexpr int HOT_LEN_BUCKETS = 1024;
@@ -382,6 +377,17 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
   if (!key_range(keys, v, kv, n, &kmin, &kmax, s)) return 0;
   const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
   int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
+  if (span <= dense_span_limit(n) && gp_supported(span) && kv == nullptr) {
+    // Partitioned path: no per-row global atomics.
+    GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
+    int2* table = nullptr;
+    int32_t* hot_keys = nullptr;
+    uint32_t* hot_n = nullptr;
+    if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
+    gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, slots, s);
+    scan_tiles(SlotCompactOp{slots, nullptr, kmin, o, ng_d}, span, s);
+    return read_scalar(ng_d, s);
+  }
   if (span <= dense_span_limit(n)) {
     GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
     hipLaunchKernelGGL(k_slot_init, dim3(grid_for(span, 256, 16 * num_cus())), dim3(256), 0, s,

@@ -47,6 +47,8 @@ struct KJob {
   // accumulator array indices (arrays of R words)
   int i_s32[MAXC], i_a32[MAXC], i_s64[MAXC], i_a64h[MAXC], i_a64l[MAXC], i_fc[MAXC];
   int i_ff[MAXC];  // into the 32-bit arrays (array 0 is the count)
+  int i_fb[MAXC];  // f64: sum of ceil(|v| * 2^(30-E)), a bound on sum |v|
+  bool f64_seq_first;  // sequential f64 sums start from the first value
   int n64arr, n32arr;
   u64* acc64;       // n64arr * R
   uint32_t* acc32;  // n32arr * R
@@ -132,6 +134,8 @@ __global__ __launch_bounds__(256) void k_dense_acc(KJob J) {
 #pragma unroll
         for (int q = 0; q < XS_CHUNKS; ++q)
           if (ch[q] != 0) atomicAdd(&a64[size_t(J.i_fc[c] + q) * J.R + r], (u64)ch[q]);
+        const u64 b = (u64)ceil(ldexp(fabs(v), 30 - E[c]));
+        if (b != 0) atomicAdd(&a64[size_t(J.i_fb[c]) * J.R + r], b);
       } else {
         atomicOr(&a32[size_t(J.i_ff[c]) * J.R + r], xs_special_flag(v));
       }
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(256) void k_dense_fin(KJob J, FinOut O) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 r = blockIdx.x * u64(blockDim.x) + threadIdx.x; r < J.R; r += stride) {
     const u64 k = J.lo + r;
-    bool amb = false;
+    bool amb = false, ambf = false;
     if (O.counts) O.counts[k] = (int64_t)J.acc32[r];
     for (int c = 0; c < J.ni32; ++c) {
       const long long s = (long long)J.acc64[size_t(J.i_s32[c]) * J.R + r];
@@ -208,10 +212,21 @@ __global__ __launch_bounds__(256) void k_dense_fin(KJob J, FinOut O) {
 #pragma unroll
       for (int q = 0; q < XS_CHUNKS; ++q) S[q] = (int64_t)J.acc64[size_t(J.i_fc[c] + q) * J.R + r];
       const uint32_t fl = J.acc32[size_t(J.i_ff[c]) * J.R + r];
-      if (O.of[c]) O.of[c][k] = xs_finish(S, xs_exponent_of_bits(J.emax[c]), fl);
+      const int E = xs_exponent_of_bits(J.emax[c]);
+      const double sum = xs_finish(S, E, fl);
+      if (O.of[c]) O.of[c][k] = sum;
+      // The row-order sum of the reference differs from the exact sum by at
+      // most (cnt-1) * 2^-53 * sum|v|.  When that bound could exceed half the
+      // 1e-12 tolerance the group's sum is redone in row order (bit-exact).
+      if (fl == 0 && O.amb) {
+        const double B = ldexp(double(J.acc64[size_t(J.i_fb[c]) * J.R + r]), E - 30) *
+                         (1.0 + 0x1p-40);
+        const double cnt = double(J.acc32[r]);
+        if (cnt * 0x1p-53 * B > 0.5e-12 * fmax(1.0, fabs(sum))) ambf = true;
+      }
     }
-    if (O.amb) O.amb[r] = amb ? 1 : 0;
-    if (amb) atomicOr(O.anyamb, 1u);
+    if (O.amb) O.amb[r] = uint8_t((amb ? 1 : 0) | (ambf ? 2 : 0));
+    if (amb || ambf) atomicOr(O.anyamb, 1u);
   }
 }
 
@@ -270,7 +285,7 @@ __global__ __launch_bounds__(256) void k_ordered_scan(KJob J, const uint8_t* __r
   const u64 waves = u64(gridDim.x) * (blockDim.x / WAVE);
   const u64 w0 = u64(blockIdx.x) * (blockDim.x / WAVE) + wave_id();
   for (u64 r = w0; r < J.R; r += waves) {
-    if (!amb[r]) continue;
+    if (!(amb[r] & 1)) continue;
     const uint32_t st = start[r];
     const uint32_t cnt = J.acc32[r];
     for (int c = 0; c < J.ni32 + J.ni64; ++c) {
@@ -314,6 +329,44 @@ __global__ __launch_bounds__(256) void k_ordered_scan(KJob J, const uint8_t* __r
         hi = __shfl(hi, WAVE - 1);
         carry = ((__int128)hi << 64) | (unsigned long long)lo;
       }
+    }
+  }
+}
+
+// One wave per slot flagged for a row-order float64 sum: the wave loads 64
+// values at a time and every lane adds them in row order, so the result is
+// the reference's sequential sum bit for bit (dense.rs:32-36 starts from
+// 0.0; the sorted-run lanes, sorted.rs:36-52, start from the first value).
+__global__ __launch_bounds__(256) void k_ordered_f64(KJob J, const uint8_t* __restrict__ amb,
+                                                     const uint32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ rows,
+                                                     const int32_t* __restrict__ perm,
+                                                     double* o0, double* o1, double* o2,
+                                                     double* o3) {
+  const int lane = lane_id();
+  const u64 waves = u64(gridDim.x) * (blockDim.x / WAVE);
+  const u64 w0 = u64(blockIdx.x) * (blockDim.x / WAVE) + wave_id();
+  double* outs[MAXC] = {o0, o1, o2, o3};
+  for (u64 r = w0; r < J.R; r += waves) {
+    if (!(amb[r] & 2)) continue;
+    const uint32_t st = start[r];
+    const uint32_t cnt = J.acc32[r];
+    for (int c = 0; c < J.nf64; ++c) {
+      double acc = 0.0;
+      for (uint32_t t0 = 0; t0 < cnt; t0 += WAVE) {
+        const uint32_t t = t0 + lane;
+        double v = 0.0;
+        if (t < cnt) {
+          const uint32_t pos = st + t;
+          v = J.vf[c][rows[perm ? perm[pos] : pos]];
+        }
+        const uint32_t m = (cnt - t0) < uint32_t(WAVE) ? (cnt - t0) : uint32_t(WAVE);
+        for (uint32_t l = 0; l < m; ++l) {
+          const double x = __shfl(v, int(l));
+          acc = (t0 == 0 && l == 0 && J.f64_seq_first) ? x : acc + x;
+        }
+      }
+      if (lane == 0 && outs[c]) outs[c][J.lo + r] = acc;
     }
   }
 }
@@ -377,8 +430,10 @@ AggResult dense_aggregate(const DenseSpec& sp, hipStream_t s) {
     J.vf[c] = sp.vf[c];
     J.i_fc[c] = a64;
     a64 += XS_CHUNKS;
+    J.i_fb[c] = a64++;
     J.i_ff[c] = a32++;
   }
+  J.f64_seq_first = sp.f64_seq_first;
   J.n64arr = a64;
   J.n32arr = a32;
   const u64 R = J.R;
@@ -421,7 +476,8 @@ AggResult dense_aggregate(const DenseSpec& sp, hipStream_t s) {
   for (int c = 0; c < sp.ni64; ++c) O.o64[c] = sp.o64[c].s64;
   for (int c = 0; c < sp.nf64; ++c) O.of[c] = sp.of[c].f64;
   O.counts = sp.counts;
-  const bool need_check = (sp.check_i32 && sp.ni32 > 0) || (sp.check_i64 && sp.ni64 > 0);
+  const bool need_check =
+      (sp.check_i32 && sp.ni32 > 0) || (sp.check_i64 && sp.ni64 > 0) || sp.nf64 > 0;
   uint8_t* amb = need_check ? scratch_t<uint8_t>(S_WORK1, R) : nullptr;
   O.amb = amb;
   O.anyamb = anyamb;
@@ -459,9 +515,16 @@ AggResult dense_aggregate(const DenseSpec& sp, hipStream_t s) {
     scan_tiles(SlotStartOp{amb, J.acc32, start}, R, s);
     u64* ovf = scratch_t<u64>(S_WORK2, 1);
     fill_u64(ovf, ~0ull, 1, s);
-    hipLaunchKernelGGL(k_ordered_scan, dim3(grid_for(R, 4, 4 * num_cus())), dim3(256), 0, s, J,
-                       amb, start, rows, perm, ovf);
-    NFH_LAUNCH_CHECK();
+    if (J.ni32 + J.ni64 > 0) {
+      hipLaunchKernelGGL(k_ordered_scan, dim3(grid_for(R, 4, 4 * num_cus())), dim3(256), 0, s,
+                         J, amb, start, rows, perm, ovf);
+      NFH_LAUNCH_CHECK();
+    }
+    if (J.nf64 > 0) {
+      hipLaunchKernelGGL(k_ordered_f64, dim3(grid_for(R, 4, 4 * num_cus())), dim3(256), 0, s, J,
+                         amb, start, rows, perm, O.of[0], O.of[1], O.of[2], O.of[3]);
+      NFH_LAUNCH_CHECK();
+    }
     ovf_row = read_scalar(ovf, s);
   }
   if (err_row == ~0ull && ovf_row == ~0ull) return res;
@@ -667,6 +730,7 @@ static int32_t variant_entry(const int32_t* keys, const V* vals, size_t n, uint3
   if (variant == 2) {
     nn = first_regress_row(sp.keys, n, st.s);
     sp.presorted = true;
+    sp.f64_seq_first = true;
   }
   sp.n = nn;
   AggResult r = dense_aggregate(sp, st.s);

@@ -32,6 +32,7 @@ struct DenseSpec {
   bool check_i32 = false;     // -4 on a row-order prefix leaving int32
   bool check_i64 = false;     // -4 on a row-order prefix leaving int64
   bool presorted = false;     // keys nondecreasing over [0, n) (V_SORTED)
+  bool f64_seq_first = false; // row-order f64 sums start from the first value
 };
 
 // rc (0, -2 bad key, -4 prefix overflow) of the first failing row, in row

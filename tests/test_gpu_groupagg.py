@@ -51,6 +51,7 @@ def test_group_agg_golden():
 @pytest.mark.parametrize("n,ids,kind", [
     (1, 1, "uniform"), (7, 3, "uniform"), (100_000, 10, "uniform"),
     (1 << 20, 1 << 16, "zipf"), (1 << 20, 1 << 20, "uniform"),
+    (1 << 22, 1 << 24, "zipf"), (1 << 22, 1 << 24, "uniform"), (1 << 21, 1 << 11, "zipf"),
     (1 << 18, 1 << 30, "wide"), (1 << 18, 1 << 31, "extremes"),
 ])
 def test_group_agg_parity(n, ids, kind):
