@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--rows-log2", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip argsort / uniform legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (reference kernel for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE scaling)")
     return ap.parse_args()
 
 
@@ -44,6 +47,41 @@ def kernel_avg_ms(L, name):
     if L.nfh_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)) != 0 or cnt.value == 0:
         return None, 0
     return ms.value / cnt.value, int(cnt.value)
+
+
+def kernel_table(L):
+    """Per kernel (HIP events on the library's stream): launches, average
+    ms, algorithmic bytes per launch and the achieved GB/s."""
+    buf = ctypes.create_string_buffer(8192)
+    L.nfh_profile_names(buf, 8192)
+    out = {}
+    for name in buf.value.decode().split():
+        ms, cnt, by = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
+        if L.nfh_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)) != 0:
+            continue
+        L.nfh_profile_bytes(name.encode(), ctypes.byref(by))
+        c = max(1, cnt.value)
+        row = {"launches": int(cnt.value), "avg_ms": round(ms.value / c, 4),
+               "total_ms": round(ms.value, 4)}
+        if by.value > 0 and ms.value > 0:
+            row["alg_bytes_per_launch"] = by.value / c
+            row["GBps"] = round(by.value / (ms.value * 1e-3) / 1e9, 1)
+        out[name] = row
+    return out
+
+
+def roofline_of(kernels, steps):
+    """Roofline object for the kernel with the largest total time that
+    carries an algorithmic byte count."""
+    cands = [(v["total_ms"], k) for k, v in kernels.items() if "GBps" in v]
+    if not cands:
+        return None
+    _, name = max(cands)
+    v = kernels[name]
+    return {"bound": "hbm", "kernel": name, "achieved": v["GBps"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(v["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "alg_bytes_per_launch": v["alg_bytes_per_launch"], "kernel_avg_ms": v["avg_ms"],
+            "launches_per_step": round(v["launches"] / max(1, steps), 2)}
 
 
 def timed(torch, fn, steps, warmup):
@@ -131,6 +169,13 @@ def main():
     L = native.lib("hip")
     n = 1 << args.rows_log2
     ids = 1 << 24
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids)
     cap = ids
@@ -160,22 +205,9 @@ def main():
     L.nfh_profile_enable(1)
     t = timed(torch, step, args.steps, 0)
     L.nfh_profile_enable(0)
-    acc_ms, acc_n = kernel_avg_ms(L, "group_acc")
-    kernels = {}
-    for name in ("key_range", "group_acc", "scan_tiles", "unique_mark", "unique_inv_dense",
-                 "radix_hist", "radix_onesweep"):
-        ms, cnt = kernel_avg_ms(L, name)
-        if ms is not None:
-            kernels[name] = {"avg_ms": round(ms, 4), "launches": cnt}
+    kernels = kernel_table(L)
     ng, nu = state["ng"], state["nu"]
-    # dominant kernel: the group-by accumulate pass, 9 algorithmic B/row
-    # (key 4 + value 4 + validity 1)
-    acc_bytes = 9.0 * n
-    achieved = acc_bytes / (acc_ms * 1e-3) / 1e9 if acc_ms else None
-    roofline = {"bound": "hbm", "kernel": "k_group_acc", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None,
-                "alg_bytes_per_launch": acc_bytes, "kernel_avg_ms": round(acc_ms, 4)}
+    roofline = roofline_of(kernels, args.steps)
     # whole-step algorithmic bytes: group-by 9 B/row + 40 B/group, unique
     # 5 B/row read + 4 B/row inverse + 4 B per distinct key
     step_bytes = 9.0 * n + 40.0 * ng + 9.0 * n + 4.0 * nu
@@ -198,15 +230,16 @@ def main():
                 raise RuntimeError("sort rc %d" % rc)
         L.nfh_profile_reset()
         L.nfh_profile_enable(1)
-        ts = timed(torch, sort_step, max(3, args.steps // 2), 1)
+        sort_steps = max(3, args.steps // 2)
+        ts = timed(torch, sort_step, sort_steps, 1)
         L.nfh_profile_enable(0)
-        one_ms, one_n = kernel_avg_ms(L, "radix_onesweep")
+        sk = kernel_table(L)
         extra["argsort_uniform_2^28"] = {
             "rows_per_s": n / ts, "ms": round(ts * 1e3, 3),
             "pass_model_GBps": round(60.0 * n / ts / 1e9, 1),
             "frac_of_peak_60B_model": round(60.0 * n / ts / 1e9 / HBM_PEAK_GBS, 4),
             "compulsory_frac_8B": round(8.0 * n / ts / 1e9 / HBM_PEAK_GBS, 4),
-            "onesweep_avg_ms": round(one_ms, 4) if one_ms else None}
+            "kernels": sk, "roofline": roofline_of(sk, sort_steps)}
         del k, perm, t0b, t1b, tpb
         # uniform-2^24 group-by variant
         keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")

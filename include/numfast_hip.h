@@ -114,6 +114,10 @@ int32_t nfh_abi_version(void);
 int32_t nfh_profile_enable(int32_t on);
 int32_t nfh_profile_reset(void);
 int32_t nfh_profile_get(const char *name, double *ms, uint64_t *launches);
+/* Algorithmic bytes recorded for a kernel name (sum over its launches). */
+int32_t nfh_profile_bytes(const char *name, double *bytes);
+/* Recorded kernel names, newline-separated, into buf; returns the length. */
+int64_t nfh_profile_names(char *buf, size_t cap);
 
 /* Group-by over raw int32 keys with NULL masks (valid / key_valid may be
  * NULL): groups of rows with both masks set, ukeys ascending; sums int64,

@@ -97,7 +97,9 @@ static int64_t sorted_run_entry(const int32_t* keys, const V* vals, size_t n, in
     st.finish();
     return -1;
   }
-  int32_t* rid = scratch_t<int32_t>(S_T0, n);
+  // S_IN5: not used by this entry's staging nor by dense_aggregate, whose
+  // row-order path reuses the S_T* slots.
+  int32_t* rid = scratch_t<int32_t>(S_IN5, n);
   int64_t* ng_d = scratch_t<int64_t>(S_MISC, 1);
   scan_tiles(RunHeadOp{k, rid, uk, ng_d}, n, st.s);
   const int64_t ng = read_scalar(ng_d, st.s);

@@ -104,21 +104,33 @@ __device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
 
 // Walk back from tile `tile-1` summing aggregates until a prefix is found.
 // `stride` is the distance between consecutive tiles' words.
+// The first probe reads one word (usually the predecessor's prefix); once
+// a run of aggregates shows up, eight words are requested back to back per
+// round, so a long walk costs one memory round trip per eight tiles.
 __device__ __forceinline__ uint32_t lb_lookback(const uint64_t* lb, uint32_t tile,
                                                 uint32_t stride, uint32_t epoch) {
+  constexpr int K = 8;
   uint32_t excl = 0;
   int64_t j = int64_t(tile) - 1;
+  int width = 1;
   while (j >= 0) {
-    const uint64_t w = lb_load(lb + size_t(j) * stride);
-    const uint32_t ep = uint32_t(w >> 34);
-    const uint32_t st = uint32_t(w >> 32) & 3u;
-    if (ep != epoch || st == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
+    const int cnt = j + 1 < width ? int(j + 1) : width;
+    uint64_t w[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) w[k] = k < cnt ? lb_load(lb + size_t(j - k) * stride) : 0ull;
+    int k = 0;
+    for (; k < cnt; ++k) {
+      const uint32_t ep = uint32_t(w[k] >> 34);
+      const uint32_t st = uint32_t(w[k] >> 32) & 3u;
+      if (ep != epoch || st == 0) break;  // not published yet: poll again
+      excl += uint32_t(w[k]);
+      if (st == LB_PREFIX) return excl;
     }
-    excl += uint32_t(w);
-    if (st == LB_PREFIX) break;
-    --j;
+    j -= k;
+    if (k < cnt)
+      __builtin_amdgcn_s_sleep(1);
+    else
+      width = K;
   }
   return excl;
 }

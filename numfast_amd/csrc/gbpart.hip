@@ -15,6 +15,7 @@
 //      present groups (groupagg.hip).
 // Histogram of buckets first, so every pass knows its global bin starts.
 #include <algorithm>
+#include <mutex>
 
 #include "device.hpp"
 #include "gbpart.hpp"
@@ -26,9 +27,14 @@ namespace nfh {
 typedef unsigned long long u64;
 
 constexpr int GP_BLOCK = 256;
-constexpr int GP_WAVES = GP_BLOCK / WAVE;
+// Partition passes use 16384-row tiles (1024 threads, ~150 KB of LDS) so a
+// digit's run of records fills whole 128-byte lines when written out.
+constexpr int GP_PBLOCK = 1024;
+constexpr int GP_WAVES = GP_PBLOCK / WAVE;
 constexpr int GP_IPT = 16;
-constexpr int GP_TILE = GP_BLOCK * GP_IPT;
+constexpr int GP_TILE = GP_PBLOCK * GP_IPT;
+constexpr size_t GP_PART_LDS = size_t(GP_TILE) * 8 + GP_WAVES * 128 * 4 + 128 * 4 * 2 +
+                               GP_WAVES * 4 + 16;
 constexpr int GP_HOT_MAX = 512;
 constexpr int GP_HOT_TBL = 1024;
 constexpr int GP_HOT_BITS = 10;
@@ -71,7 +77,8 @@ struct HotAcc {
 __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n, int32_t kmin, uint32_t nb,
-    const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc) {
+    const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc,
+    bool vec) {
   extern __shared__ uint32_t s_h[];  // nb counters
   __shared__ int32_t t_key[GP_HOT_TBL], t_idx[GP_HOT_TBL];
   __shared__ u64 h_sum[GP_HOT_MAX];
@@ -86,13 +93,10 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
     h_mx[b] = INT32_MIN;
   }
   __syncthreads();
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    if (vm && !vm[i]) continue;
-    const int32_t k = keys[i];
+  auto row = [&](int32_t k, int32_t x, bool ok) {
+    if (!ok) return;
     const int hi = table ? gp_hot_find(t_key, t_idx, k) : -1;
     if (hi >= 0) {
-      const int32_t x = vals[i];
       atomicAdd(&h_sum[hi], (u64)(long long)x);
       atomicAdd(&h_cnt[hi], 1u);
       atomicMin(&h_mn[hi], x);
@@ -100,7 +104,43 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
     } else {
       atomicAdd(&s_h[uint32_t(k - kmin) >> GP_BUCKET_BITS], 1u);
     }
+  };
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t tail = 0;
+  if (vec) {
+    // quads: 128-bit key / value loads, a 32-bit load of four mask bytes,
+    // two quads in flight per thread
+    const uint32_t nq = n / 4;
+    uint32_t q = tid;
+    for (; q + stride < nq; q += 2 * stride) {
+      const int4 ka = reinterpret_cast<const int4*>(keys)[q];
+      const int4 kb = reinterpret_cast<const int4*>(keys)[q + stride];
+      const int4 va = reinterpret_cast<const int4*>(vals)[q];
+      const int4 vb = reinterpret_cast<const int4*>(vals)[q + stride];
+      const uint32_t ma = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
+      const uint32_t mb = vm ? reinterpret_cast<const uint32_t*>(vm)[q + stride] : 0x01010101u;
+      row(ka.x, va.x, ma & 0xFFu);
+      row(ka.y, va.y, ma & 0xFF00u);
+      row(ka.z, va.z, ma & 0xFF0000u);
+      row(ka.w, va.w, ma & 0xFF000000u);
+      row(kb.x, vb.x, mb & 0xFFu);
+      row(kb.y, vb.y, mb & 0xFF00u);
+      row(kb.z, vb.z, mb & 0xFF0000u);
+      row(kb.w, vb.w, mb & 0xFF000000u);
+    }
+    for (; q < nq; q += stride) {
+      const int4 ka = reinterpret_cast<const int4*>(keys)[q];
+      const int4 va = reinterpret_cast<const int4*>(vals)[q];
+      const uint32_t ma = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
+      row(ka.x, va.x, ma & 0xFFu);
+      row(ka.y, va.y, ma & 0xFF00u);
+      row(ka.z, va.z, ma & 0xFF0000u);
+      row(ka.w, va.w, ma & 0xFF000000u);
+    }
+    tail = nq * 4;
   }
+  for (uint32_t i = tail + tid; i < n; i += stride) row(keys[i], vals[i], !vm || vm[i]);
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
@@ -160,29 +200,29 @@ __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ h
 // FIRST reads raw rows: NULL rows and hot rows (already aggregated by
 // k_gp_hist) are dropped, the others become records.
 template <bool FIRST>
-__global__ __launch_bounds__(GP_BLOCK) void k_gp_part(
+__global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n_rows, int32_t kmin,
     const uint32_t* __restrict__ oin, const int32_t* __restrict__ vin, uint32_t n_rec,
     uint32_t* __restrict__ oout, int32_t* __restrict__ vout, int shift,
     const uint32_t* __restrict__ dstart, uint64_t* __restrict__ lookback,
     uint32_t* __restrict__ tile_ctr, uint32_t epoch, const int2* __restrict__ table) {
-  __shared__ uint32_t s_o[GP_TILE];
-  __shared__ int32_t s_v[GP_TILE];
-  __shared__ uint32_t s_wh[GP_WAVES][128];
-  __shared__ uint32_t s_tstart[128];
-  __shared__ uint32_t s_gbase[128];
-  __shared__ uint32_t s_tmp[GP_WAVES];
-  __shared__ uint32_t s_tile;
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  uint32_t* s_o = reinterpret_cast<uint32_t*>(s_raw);
+  int32_t* s_v = reinterpret_cast<int32_t*>(s_o + GP_TILE);
+  uint32_t(*s_wh)[128] = reinterpret_cast<uint32_t(*)[128]>(s_v + GP_TILE);
+  uint32_t* s_tstart = &s_wh[GP_WAVES][0];
+  uint32_t* s_gbase = s_tstart + 128;
+  uint32_t* s_tmp = s_gbase + 128;
+  uint32_t* s_tile = s_tmp + GP_WAVES;
   __shared__ int32_t t_key[FIRST ? GP_HOT_TBL : 1], t_idx[FIRST ? GP_HOT_TBL : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-  for (int w = 0; w < GP_WAVES; ++w)
-    if (tid < 128) s_wh[w][tid] = 0;
+  if (tid == 0) *s_tile = atomicAdd(tile_ctr, 1u);
+  for (int i = tid; i < GP_WAVES * 128; i += GP_PBLOCK) s_wh[i >> 7][i & 127] = 0;
   if constexpr (FIRST) gp_load_table(table, t_key, t_idx);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = *s_tile;
   const uint32_t n = FIRST ? n_rows : n_rec;
   const uint32_t base = tile * uint32_t(GP_TILE);
   const uint32_t wbase = base + uint32_t(wave) * (WAVE * GP_IPT);
@@ -190,27 +230,36 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_part(
   uint32_t ov[GP_IPT];
   int32_t vv[GP_IPT];
   uint32_t live = 0;  // bit j: item j is a record of this pass
+  if constexpr (FIRST) {
+    // all loads first (independent, coalesced per item), then the filter
+    uint32_t okm = 0;
 #pragma unroll
-  for (int j = 0; j < GP_IPT; ++j) {
-    const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-    ov[j] = 0;
-    vv[j] = 0;
-    if (pos < n) {
-      if constexpr (FIRST) {
-        const bool ok = vm == nullptr || vm[pos] != 0;
-        if (ok) {
-          const int32_t k = keys[pos];
-          if (!table || gp_hot_find(t_key, t_idx, k) < 0) {
-            ov[j] = uint32_t(k - kmin);
-            vv[j] = vals[pos];
-            live |= 1u << j;
-          }
+    for (int j = 0; j < GP_IPT; ++j) {
+      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+      const bool in = pos < n;
+      ov[j] = in ? uint32_t(keys[pos]) : 0u;
+      vv[j] = in ? vals[pos] : 0;
+      okm |= uint32_t(in && (vm == nullptr || vm[pos] != 0)) << j;
+    }
+#pragma unroll
+    for (int j = 0; j < GP_IPT; ++j) {
+      if ((okm >> j) & 1u) {
+        const int32_t k = int32_t(ov[j]);
+        if (!table || gp_hot_find(t_key, t_idx, k) < 0) {
+          ov[j] = uint32_t(k - kmin);
+          live |= 1u << j;
         }
-      } else {
-        ov[j] = oin[pos];
-        vv[j] = vin[pos];
-        live |= 1u << j;
       }
+    }
+  }
+  if constexpr (!FIRST) {
+#pragma unroll
+    for (int j = 0; j < GP_IPT; ++j) {
+      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+      const bool in = pos < n;
+      ov[j] = in ? oin[pos] : 0u;
+      vv[j] = in ? vin[pos] : 0;
+      live |= uint32_t(in) << j;
     }
   }
 
@@ -265,7 +314,7 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_part(
     s_gbase[tid] = dstart[tid] + excl - tstart;
   }
   __syncthreads();
-  for (uint32_t i = tid; i < tile_total; i += GP_BLOCK) {
+  for (uint32_t i = tid; i < tile_total; i += GP_PBLOCK) {
     const uint32_t o = s_o[i];
     const uint32_t g = s_gbase[(o >> shift) & 127u] + i;
     oout[g] = o;
@@ -363,15 +412,26 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   fill_u32(meta, 0, meta_words, s);
   hipLaunchKernelGGL(k_gp_hotinit, dim3(2), dim3(256), 0, s, hot_acc, uint32_t(GP_HOT_MAX));
   NFH_LAUNCH_CHECK();
+  const double mask_b = vm ? 1.0 : 0.0;
   {
-    ProfScope prof("gp_hist", s);
+    // algorithmic bytes: key + value + validity per row
+    ProfScope prof("gp_hist", s, double(n) * (8.0 + mask_b));
     const size_t lds = size_t(nb) * 4;
-    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n, GP_BLOCK, 2 * num_cus())), dim3(GP_BLOCK),
-                       lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc);
+    const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16) == 0 &&
+                     (reinterpret_cast<uintptr_t>(vals) % 16) == 0 &&
+                     (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
+    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_BLOCK, 3 * num_cus())),
+                       dim3(GP_BLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
+                       vec);
     NFH_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart, dstart);
   NFH_LAUNCH_CHECK();
+  // records = valid non-hot rows (bstart[nb])
+  uint32_t* h_m = static_cast<uint32_t*>(pinned(4));
+  NFH_CHECK(hipMemcpyAsync(h_m, bstart + nb, 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  const uint32_t m = *h_m;
   // record buffers (ping-pong): at most n records
   uint32_t* o0 = scratch_t<uint32_t>(S_T0, n);
   int32_t* v0 = scratch_t<int32_t>(S_T1, n);
@@ -379,11 +439,20 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   int32_t* v1 = scratch_t<int32_t>(S_T3, n);
   const uint32_t tiles = (n + GP_TILE - 1) / GP_TILE;
   uint64_t* lb = lookback_words(size_t(tiles) * 128, s);
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
+    NFH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gp_part<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(GP_PART_LDS)));
+    NFH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gp_part<false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(GP_PART_LDS)));
+  });
   // pass 1 always runs: it drops NULL and hot rows (digit 0 when nb <= 1)
   {
-    ProfScope prof("gp_part_first", s);
+    // reads key + validity of every row and the value of each record,
+    // writes (o, value) per record
+    ProfScope prof("gp_part_first", s, double(n) * (4.0 + mask_b) + double(m) * 12.0);
     const uint32_t epoch = lookback_epoch(s);
-    hipLaunchKernelGGL(k_gp_part<true>, dim3(tiles), dim3(GP_BLOCK), 0, s, keys, vals, vm, n,
+    hipLaunchKernelGGL(k_gp_part<true>, dim3(tiles), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n,
                        kmin, nullptr, nullptr, 0u, o0, v0, GP_BUCKET_BITS, dstart, lb, ctr, epoch,
                        table);
     NFH_LAUNCH_CHECK();
@@ -391,15 +460,11 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   const uint32_t* fo = o0;
   const int32_t* fv = v0;
   if (passes == 2) {
-    // records of pass 1 are all non-hot valid rows: their count is bstart[nb]
-    uint32_t m = 0;
-    NFH_CHECK(hipMemcpyAsync(&m, bstart + nb, 4, hipMemcpyDeviceToHost, s));
-    NFH_CHECK(hipStreamSynchronize(s));
     if (m > 0) {
-      ProfScope prof("gp_part", s);
+      ProfScope prof("gp_part", s, double(m) * 16.0);
       const uint32_t t2 = (m + GP_TILE - 1) / GP_TILE;
       const uint32_t epoch = lookback_epoch(s);
-      hipLaunchKernelGGL(k_gp_part<false>, dim3(t2), dim3(GP_BLOCK), 0, s, nullptr, nullptr,
+      hipLaunchKernelGGL(k_gp_part<false>, dim3(t2), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
                          nullptr, 0u, kmin, o0, v0, m, o1, v1, GP_BUCKET_BITS + 7, dstart + 128,
                          lb, ctr + 1, epoch, nullptr);
       NFH_LAUNCH_CHECK();
@@ -408,7 +473,7 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
     fv = v1;
   }
   {
-    ProfScope prof("gp_bucket", s);
+    ProfScope prof("gp_bucket", s, double(m) * 8.0 + double(span) * sizeof(GpSlot));
     hipLaunchKernelGGL(k_gp_bucket, dim3(nb), dim3(GP_BLOCK), 0, s, fo, fv, bstart, span, slots);
     NFH_LAUNCH_CHECK();
   }

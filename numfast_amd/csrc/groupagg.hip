@@ -135,9 +135,17 @@ __global__ void k_hot_runs(const int32_t* __restrict__ sk, uint32_t S, uint32_t*
   if (i >= S) return;
   uint32_t L = 0;
   if (i == 0 || sk[i] != sk[i - 1]) {
-    uint32_t j = i + 1;
-    while (j < S && sk[j] == sk[i]) ++j;
-    L = j - i;
+    // end of the run: first position holding a larger key (binary search)
+    const int32_t k = sk[i];
+    uint32_t lo = i + 1, hi = S;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sk[mid] == k)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    L = lo - i;
   }
   len[i] = L;
 }
@@ -290,6 +298,93 @@ struct SlotCompactOp {
   __device__ void total(uint32_t t) const { *ng = int64_t(t); }
 };
 
+// Compaction of the present slots in key order with coalesced output: each
+// wave takes 64 consecutive slots per step (striped), ballots the present
+// ones and writes them to consecutive positions; tiles of 4096 slots chain
+// their counts with a decoupled look-back.
+constexpr int SC2_BLOCK = 256, SC2_IPT = 16, SC2_TILE = SC2_BLOCK * SC2_IPT;
+
+__global__ __launch_bounds__(SC2_BLOCK) void k_slot_compact(
+    const GroupSlot* __restrict__ slots, uint64_t m, const int32_t* __restrict__ ukey_of,
+    int32_t kmin, GroupAggOut o, uint64_t* __restrict__ lookback, uint32_t* __restrict__ tile_ctr,
+    uint32_t epoch, int64_t* __restrict__ ng) {
+  __shared__ uint32_t s_wtot[SC2_BLOCK / WAVE];
+  __shared__ uint32_t s_tile, s_prefix;
+  const int lane = lane_id(), wave = wave_id();
+  if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t wbase = uint64_t(tile) * SC2_TILE + uint64_t(wave) * (WAVE * SC2_IPT);
+  uint64_t bal[SC2_IPT];
+  uint32_t wtot = 0;
+#pragma unroll
+  for (int j = 0; j < SC2_IPT; ++j) {
+    const uint64_t s = wbase + uint64_t(j * WAVE + lane);
+    const bool present = s < m && slots[s].cnt != 0;
+    bal[j] = __ballot(present);
+    wtot += uint32_t(__popcll(bal[j]));
+  }
+  if (lane == 0) s_wtot[wave] = wtot;
+  __syncthreads();
+  uint32_t woff = 0, ttot = 0;
+#pragma unroll
+  for (int w = 0; w < SC2_BLOCK / WAVE; ++w) {
+    if (w < wave) woff += s_wtot[w];
+    ttot += s_wtot[w];
+  }
+  if (threadIdx.x == 0) {
+    uint64_t* my = lookback + tile;
+    if (tile == 0) {
+      lb_store(my, lb_pack(epoch, LB_PREFIX, ttot));
+      s_prefix = 0;
+    } else {
+      lb_store(my, lb_pack(epoch, LB_AGG, ttot));
+      const uint32_t ex = lb_lookback(lookback, tile, 1, epoch);
+      lb_store(my, lb_pack(epoch, LB_PREFIX, ex + ttot));
+      s_prefix = ex;
+    }
+  }
+  __syncthreads();
+  uint32_t pos = s_prefix + woff;
+#pragma unroll
+  for (int j = 0; j < SC2_IPT; ++j) {
+    const uint64_t s = wbase + uint64_t(j * WAVE + lane);
+    if ((bal[j] >> lane) & 1ull) {
+      const uint32_t ex = pos + uint32_t(__popcll(bal[j] & lanemask_lt()));
+      const GroupSlot sl = slots[s];
+      const int64_t sum = (int64_t)sl.sum;
+      if (o.ukeys) o.ukeys[ex] = ukey_of ? int64_t(ukey_of[s]) : int64_t(kmin) + int64_t(s);
+      if (o.counts) o.counts[ex] = int64_t(sl.cnt);
+      if (o.sums) o.sums[ex] = sum;
+      if (o.mins) o.mins[ex] = sl.mn;
+      if (o.maxs) o.maxs[ex] = sl.mx;
+      if (o.means) o.means[ex] = double(sum) / double(sl.cnt);
+    }
+    pos += uint32_t(__popcll(bal[j]));
+  }
+  const uint32_t ntiles = uint32_t((m + SC2_TILE - 1) / SC2_TILE);
+  if (tile == ntiles - 1 && threadIdx.x == 0) *ng = int64_t(s_prefix + ttot);
+}
+
+static int64_t compact_slots(const GroupSlot* slots, uint64_t m, const int32_t* ukey_of,
+                             int32_t kmin, const GroupAggOut& o, hipStream_t s) {
+  if (m == 0) return 0;
+  const uint32_t tiles = uint32_t((m + SC2_TILE - 1) / SC2_TILE);
+  uint64_t* lb = lookback_words(tiles, s);
+  const uint32_t epoch = lookback_epoch(s);
+  uint32_t* ctr = scratch_t<uint32_t>(S_META2, 8) + 4;
+  fill_u32(ctr, 0, 1, s);
+  int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
+  {
+    // reads the slots, writes 40 B per present group
+    ProfScope prof("slot_compact", s, double(m) * sizeof(GroupSlot));
+    hipLaunchKernelGGL(k_slot_compact, dim3(tiles), dim3(SC2_BLOCK), 0, s, slots, m, ukey_of,
+                       kmin, o, lb, ctr, epoch, ng_d);
+    NFH_LAUNCH_CHECK();
+  }
+  return read_scalar(ng_d, s);
+}
+
 // Compact the valid rows (keys + values) in row order.
 struct ValidRowsOp {
   const int32_t* keys;
@@ -318,7 +413,7 @@ bool key_range(const int32_t* keys, const uint8_t* v, const uint8_t* kv, size_t 
   fill_u32(r, uint32_t(INT32_MAX), 1, s);
   fill_u32(r + 1, uint32_t(INT32_MIN), 1, s);
   {
-    ProfScope prof("key_range", s);
+    ProfScope prof("key_range", s, double(n) * (4.0 + (v ? 1.0 : 0.0) + (kv ? 1.0 : 0.0)));
     const unsigned grid = grid_for(n / 4 + 1, 256, 8 * num_cus());
     if (vec_ok(keys, nullptr, v, kv))
       hipLaunchKernelGGL(k_key_range<true>, dim3(grid), dim3(256), 0, s, keys, v, kv,
@@ -371,10 +466,17 @@ static bool build_hot_table(const int32_t* keys, size_t n, int2** table, int32_t
 }
 
 int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t* v,
-                         const uint8_t* kv, size_t n, const GroupAggOut& o, hipStream_t s) {
+                         const uint8_t* kv, size_t n, const GroupAggOut& o, hipStream_t s,
+                         const int32_t* known_range) {
   if (n == 0) return 0;
   int32_t kmin, kmax;
-  if (!key_range(keys, v, kv, n, &kmin, &kmax, s)) return 0;
+  if (known_range) {
+    kmin = known_range[0];
+    kmax = known_range[1];
+    if (kmin > kmax) return 0;
+  } else if (!key_range(keys, v, kv, n, &kmin, &kmax, s)) {
+    return 0;
+  }
   const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
   int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
   if (span <= dense_span_limit(n) && gp_supported(span) && kv == nullptr) {
@@ -385,8 +487,7 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
     uint32_t* hot_n = nullptr;
     if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
     gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, slots, s);
-    scan_tiles(SlotCompactOp{slots, nullptr, kmin, o, ng_d}, span, s);
-    return read_scalar(ng_d, s);
+    return compact_slots(slots, span, nullptr, kmin, o, s);
   }
   if (span <= dense_span_limit(n)) {
     GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
@@ -398,7 +499,7 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
     uint32_t* hot_n = nullptr;
     if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
     {
-      ProfScope prof("group_acc", s);
+      ProfScope prof("group_acc", s, double(n) * (8.0 + (v ? 1.0 : 0.0) + (kv ? 1.0 : 0.0)));
       const unsigned grid = grid_for(n / 4 + 1, ACC_BLOCK, 4 * num_cus());
       if (vec_ok(keys, vals, v, kv))
         hipLaunchKernelGGL(k_group_acc<true>, dim3(grid), dim3(ACC_BLOCK), 0, s, keys, vals, v,
@@ -408,8 +509,7 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
                            kv, uint32_t(n), kmin, slots, nullptr, table, hot_keys, hot_n);
       NFH_LAUNCH_CHECK();
     }
-    scan_tiles(SlotCompactOp{slots, nullptr, kmin, o, ng_d}, span, s);
-    return read_scalar(ng_d, s);
+    return compact_slots(slots, span, nullptr, kmin, o, s);
   }
   // sparse: compact valid rows, radix unique, aggregate by code
   int32_t* ck = scratch_t<int32_t>(S_WORK1, n);
@@ -434,8 +534,7 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
                        nullptr, nullptr, nullptr);
     NFH_LAUNCH_CHECK();
   }
-  scan_tiles(SlotCompactOp{slots, uniq, 0, o, ng_d}, size_t(ng), s);
-  return read_scalar(ng_d, s);
+  return compact_slots(slots, u64(ng), uniq, 0, o, s);
 }
 
 // ---------------------------------------------------------------- unique
@@ -527,7 +626,7 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
     const bool vec = vec_ok(keys, nullptr, v, nullptr) && aligned(inv, 16);
     const unsigned grid = grid_for(n / 4 + 1, 256, 8 * num_cus());
     {
-      ProfScope prof("unique_mark", s);
+      ProfScope prof("unique_mark", s, double(n) * (4.0 + (v ? 1.0 : 0.0)));
       if (vec)
         hipLaunchKernelGGL(k_mark<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
                            flag);
@@ -538,7 +637,7 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
     }
     scan_tiles(RankOp{flag, kmin, uniq, ng_d}, span, s);
     {
-      ProfScope prof("unique_inv_dense", s);
+      ProfScope prof("unique_inv_dense", s, double(n) * (8.0 + (v ? 1.0 : 0.0)));
       if (vec)
         hipLaunchKernelGGL(k_inv_dense<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n),
                            kmin, flag, inv);
@@ -622,7 +721,8 @@ NF_EXPORT int64_t nfh_group_agg_i32(const int32_t* keys, const int32_t* vals,
     d.mins = mins;
     d.maxs = maxs;
     d.means = means;
-    const int64_t ng = group_agg_device(k, v, vm, km, n, d, st.s);
+    const int32_t range[2] = {kmin, kmax};
+    const int64_t ng = group_agg_device(k, v, vm, km, n, d, st.s, range);
     st.finish();
     return ng;
   }

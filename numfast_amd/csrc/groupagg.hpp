@@ -15,9 +15,11 @@ struct GroupAggOut {
 
 // Groups of the rows whose key and value are valid (masks may be null).
 // Outputs need ng lanes (ng <= n).  Returns ng.  Synchronous.
+// `known_range` ({kmin, kmax}) skips the key-range pass when the caller
+// already ran it for the same rows and masks.
 int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t* valid,
                          const uint8_t* key_valid, size_t n, const GroupAggOut& out,
-                         hipStream_t s);
+                         hipStream_t s, const int32_t* known_range = nullptr);
 
 // np.unique over valid rows; inv = -1 on NULL rows.  Returns ng.
 int64_t unique_valid_device(const int32_t* keys, const uint8_t* valid, size_t n, int32_t* uniq,

@@ -168,16 +168,19 @@ bool g_prof_on = false;
 struct ProfAcc {
   double ms = 0;
   uint64_t launches = 0;
+  double bytes = 0;
 };
 std::map<std::string, ProfAcc> g_prof;
 struct Pending {
   const char* name;
   hipEvent_t a, b;
+  double bytes;
 };
 thread_local std::vector<Pending> t_pending;
 }  // namespace
 
-ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), s_(s) {
+ProfScope::ProfScope(const char* name, hipStream_t s, double bytes)
+    : name_(name), s_(s), bytes_(bytes) {
   if (!g_prof_on) return;
   NFH_CHECK(hipEventCreate(&a_));
   NFH_CHECK(hipEventCreate(&b_));
@@ -187,7 +190,7 @@ ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), s_(s) {
 ProfScope::~ProfScope() {
   if (a_ == nullptr) return;
   if (hipEventRecord(b_, s_) == hipSuccess) {
-    t_pending.push_back({name_, a_, b_});
+    t_pending.push_back({name_, a_, b_, bytes_});
   } else {
     (void)hipEventDestroy(a_);
     (void)hipEventDestroy(b_);
@@ -205,6 +208,7 @@ void prof_collect() {
       ProfAcc& acc = g_prof[e.name];
       acc.ms += ms;
       acc.launches += 1;
+      acc.bytes += e.bytes;
     }
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
@@ -293,4 +297,30 @@ NF_EXPORT int32_t nfh_profile_get(const char* name, double* ms, uint64_t* launch
   if (ms) *ms = it->second.ms;
   if (launches) *launches = it->second.launches;
   return 0;
+}
+
+// Algorithmic bytes recorded for kernel `name` (sum over its launches).
+NF_EXPORT int32_t nfh_profile_bytes(const char* name, double* bytes) {
+  std::lock_guard<std::mutex> g(nfh::g_prof_mu);
+  auto it = nfh::g_prof.find(name ? std::string(name) : std::string());
+  if (it == nfh::g_prof.end()) return -1;
+  if (bytes) *bytes = it->second.bytes;
+  return 0;
+}
+
+// Names of every kernel recorded so far, '\n'-separated, into buf (cap
+// bytes, NUL-terminated); returns the length needed.
+NF_EXPORT int64_t nfh_profile_names(char* buf, size_t cap) {
+  std::lock_guard<std::mutex> g(nfh::g_prof_mu);
+  std::string all;
+  for (auto& kv : nfh::g_prof) {
+    all += kv.first;
+    all += '\n';
+  }
+  if (buf && cap) {
+    const size_t k = all.size() < cap - 1 ? all.size() : cap - 1;
+    std::memcpy(buf, all.data(), k);
+    buf[k] = 0;
+  }
+  return int64_t(all.size());
 }

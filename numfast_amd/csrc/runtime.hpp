@@ -153,12 +153,14 @@ int num_cus();
 // collected after the stream is synchronised (Stage::finish / prof_collect).
 class ProfScope {
  public:
-  ProfScope(const char* name, hipStream_t s);
+  // `bytes`: algorithmic bytes the launch must move (for roofline reports).
+  ProfScope(const char* name, hipStream_t s, double bytes = 0.0);
   ~ProfScope();
 
  private:
   const char* name_;
   hipStream_t s_;
+  double bytes_;
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 void prof_collect();

@@ -19,6 +19,8 @@
 // The first active pass reads the raw keys and generates row ids; the last
 // one writes only the permutation (and, for unique, the sorted keys).  So
 // the traffic per key is 4 (histogram) + 12 + 16 + 16 + 12 bytes for int32.
+#include <mutex>
+
 #include "device.hpp"
 #include "runtime.hpp"
 #include "sort.hpp"
@@ -54,10 +56,19 @@ struct Key<int64_t> {
   }
 };
 
-constexpr int SORT_BLOCK = 256;
-constexpr int SORT_WAVES = SORT_BLOCK / WAVE;
+// Tiles are large (16384 int32 keys, 8192 int64 keys) so that each digit's
+// run inside a tile spans whole 128-byte lines when written out: smaller
+// tiles leave partial lines, which cost a fill read plus a partial
+// write-back per line (measured with FETCH_SIZE / WRITE_SIZE).
 constexpr int SORT_IPT = 16;
-constexpr int SORT_TILE = SORT_BLOCK * SORT_IPT;
+template <class K>
+struct SortGeom {
+  static constexpr int BLOCK = sizeof(K) == 4 ? 1024 : 512;
+  static constexpr int WAVES = BLOCK / WAVE;
+  static constexpr int TILE = BLOCK * SORT_IPT;
+  static constexpr size_t LDS =
+      size_t(TILE) * (sizeof(K) + 4) + size_t(WAVES) * 256 * 4 + 256 * 4 * 2 + WAVES * 4 + 16;
+};
 
 // ------------------------------------------------------------ histograms
 template <class K>
@@ -98,7 +109,7 @@ __global__ __launch_bounds__(256) void k_bins(const uint32_t* __restrict__ hist,
 // FIRST: read raw keys (convert to words) and generate row ids.
 // LAST: write only the permutation (+ sorted keys when kout != nullptr).
 template <class K, bool FIRST, bool LAST>
-__global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
+__global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     const K* __restrict__ keys, const typename Key<K>::W* __restrict__ win,
     const int32_t* __restrict__ iin, typename Key<K>::W* __restrict__ wout,
     int32_t* __restrict__ iout, K* __restrict__ kout, const uint32_t* __restrict__ bin_start,
@@ -106,22 +117,24 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
     bool desc, uint32_t epoch) {
   using T = Key<K>;
   using W = typename T::W;
-  __shared__ W s_w[SORT_TILE];
-  __shared__ uint32_t s_i[SORT_TILE];
-  __shared__ uint32_t s_wh[SORT_WAVES][256];
-  __shared__ uint32_t s_tstart[256];
-  __shared__ uint32_t s_gbase[256];
-  __shared__ uint32_t s_tmp[SORT_WAVES];
-  __shared__ uint32_t s_tile;
+  using G = SortGeom<K>;
+  constexpr int SORT_BLOCK = G::BLOCK, SORT_WAVES = G::WAVES, SORT_TILE = G::TILE;
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  W* s_w = reinterpret_cast<W*>(s_raw);
+  uint32_t* s_i = reinterpret_cast<uint32_t*>(s_w + SORT_TILE);
+  uint32_t(*s_wh)[256] = reinterpret_cast<uint32_t(*)[256]>(s_i + SORT_TILE);
+  uint32_t* s_tstart = &s_wh[SORT_WAVES][0];
+  uint32_t* s_gbase = s_tstart + 256;
+  uint32_t* s_tmp = s_gbase + 256;
+  uint32_t* s_tile = s_tmp + SORT_WAVES;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = wave_id();
-  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-#pragma unroll
-  for (int w = 0; w < SORT_WAVES; ++w) s_wh[w][tid] = 0;
+  if (tid == 0) *s_tile = atomicAdd(tile_ctr, 1u);
+  for (int i = tid; i < SORT_WAVES * 256; i += SORT_BLOCK) s_wh[i >> 8][i & 255] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = *s_tile;
   const uint32_t base = tile * uint32_t(SORT_TILE);
   const uint32_t wbase = base + uint32_t(wave) * (WAVE * SORT_IPT);
 
@@ -165,19 +178,23 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
   }
   __syncthreads();
 
-  // Per digit (thread == digit): wave-exclusive offsets and tile count.
+  // Per digit (threads 0..255, thread == digit): wave-exclusive offsets and
+  // the tile's count, published at once for later tiles' look-back.
+  const bool digit_thread = tid < 256;
   uint32_t cnt = 0;
+  uint64_t* my_lb = lookback + size_t(tile) * 256 + (digit_thread ? tid : 0);
+  if (digit_thread) {
 #pragma unroll
-  for (int w = 0; w < SORT_WAVES; ++w) {
-    const uint32_t c = s_wh[w][tid];
-    s_wh[w][tid] = cnt;
-    cnt += c;
+    for (int w = 0; w < SORT_WAVES; ++w) {
+      const uint32_t c = s_wh[w][tid];
+      s_wh[w][tid] = cnt;
+      cnt += c;
+    }
+    lb_store(my_lb, lb_pack(epoch, tile == 0 ? LB_PREFIX : LB_AGG, cnt));
   }
-  uint64_t* my_lb = lookback + size_t(tile) * 256 + tid;
-  lb_store(my_lb, lb_pack(epoch, tile == 0 ? LB_PREFIX : LB_AGG, cnt));
   uint32_t tile_total;
   const uint32_t tstart = block_excl_scan<SORT_WAVES>(cnt, s_tmp, &tile_total);
-  s_tstart[tid] = tstart;
+  if (digit_thread) s_tstart[tid] = tstart;
   __syncthreads();
 
   // Reorder the tile in LDS by digit (stable).
@@ -193,12 +210,14 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
   }
 
   // Decoupled look-back for this digit's offset among earlier tiles.
-  uint32_t excl = 0;
-  if (tile > 0) {
-    excl = lb_lookback(lookback + tid, tile, 256, epoch);
-    lb_store(my_lb, lb_pack(epoch, LB_PREFIX, excl + cnt));
+  if (digit_thread) {
+    uint32_t excl = 0;
+    if (tile > 0) {
+      excl = lb_lookback(lookback + tid, tile, 256, epoch);
+      lb_store(my_lb, lb_pack(epoch, LB_PREFIX, excl + cnt));
+    }
+    s_gbase[tid] = bin_start[tid] + excl - tstart;
   }
-  s_gbase[tid] = bin_start[tid] + excl - tstart;
   __syncthreads();
 
   const uint32_t nvalid = (n - base) < uint32_t(SORT_TILE) ? (n - base) : uint32_t(SORT_TILE);
@@ -249,10 +268,21 @@ static void launch_pass(const K* keys, const typename Key<K>::W* win, const int3
                         typename Key<K>::W* wout, int32_t* iout, K* kout,
                         const uint32_t* bin_start, uint64_t* lb, uint32_t* ctr, uint32_t n,
                         int shift, bool desc, hipStream_t s) {
-  const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+  using G = SortGeom<K>;
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
+    NFH_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_onesweep<K, FIRST, LAST>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(G::LDS)));
+  });
+  const uint32_t tiles = (n + G::TILE - 1) / G::TILE;
   const uint32_t epoch = lookback_epoch(s);
-  ProfScope prof("radix_onesweep", s);
-  hipLaunchKernelGGL((k_onesweep<K, FIRST, LAST>), dim3(tiles), dim3(SORT_BLOCK), 0, s, keys,
+  // pass model: first pass reads keys, later ones (word, row id); the last
+  // writes row ids (+ keys for unique), the others (word, row id)
+  const double in_b = FIRST ? sizeof(K) : sizeof(typename Key<K>::W) + 4.0;
+  const double out_b = LAST ? 4.0 + (kout ? sizeof(K) : 0.0) : sizeof(typename Key<K>::W) + 4.0;
+  ProfScope prof("radix_onesweep", s, double(n) * (in_b + out_b));
+  hipLaunchKernelGGL((k_onesweep<K, FIRST, LAST>), dim3(tiles), dim3(G::BLOCK), G::LDS, s, keys,
                      win, iin, wout, iout, kout, bin_start, lb, ctr, n, shift, desc, epoch);
   NFH_LAUNCH_CHECK();
 }
@@ -274,7 +304,7 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
   fill_u32(meta, 0, size_t(P) * 512 + 2 * P, s);
   const unsigned hb = grid_for(n, 256, 4u * unsigned(num_cus()));
   {
-    ProfScope prof("radix_hist", s);
+    ProfScope prof("radix_hist", s, double(n) * sizeof(K));
     hipLaunchKernelGGL((k_hist<K>), dim3(hb), dim3(256), 0, s, keys, n, desc, hist);
     NFH_LAUNCH_CHECK();
   }
@@ -293,7 +323,7 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
     NFH_LAUNCH_CHECK();
     return;
   }
-  const size_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+  const size_t tiles = (n + SortGeom<K>::TILE - 1) / SortGeom<K>::TILE;
   uint64_t* lb = lookback_words(tiles * 256, s);
   // Intermediate pairs: A = (t1, tp), B = (t0, perm).  The last pass must
   // read A so that it can write perm; k = na-1 intermediate outputs.

@@ -99,7 +99,8 @@ int64_t unique_inverse_device(const K* keys, size_t n_, K* uniq, int32_t* inv, i
   fill_u32(meta, 0, 4, s);
   int64_t* ng_dev = reinterpret_cast<int64_t*>(meta + 2);
   {
-    ProfScope prof("unique_run_encode", s);
+    // reads sorted key + row id, writes the inverse (+ the distinct keys)
+    ProfScope prof("unique_run_encode", s, double(n) * (sizeof(K) + 8.0));
     hipLaunchKernelGGL((k_run_encode<K>), dim3(tiles), dim3(RE_BLOCK), 0, s, sorted, perm, n,
                        uniq, inv, lb, meta, epoch, ng_dev);
     NFH_LAUNCH_CHECK();

@@ -84,6 +84,8 @@ GROUPS = {
         ("nfh_profile_enable", [_i32], _i32),
         ("nfh_profile_reset", [], _i32),
         ("nfh_profile_get", [ctypes.c_char_p, _vp, _vp], _i32),
+        ("nfh_profile_bytes", [ctypes.c_char_p, _vp], _i32),
+        ("nfh_profile_names", [_vp, _sz], _i64),
     ],
 }
 
