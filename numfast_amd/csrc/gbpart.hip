@@ -27,9 +27,11 @@ namespace nfh {
 typedef unsigned long long u64;
 
 constexpr int GP_BLOCK = 256;
-// Partition passes use 16384-row tiles (1024 threads, ~150 KB of LDS) so a
-// digit's run of records fills whole 128-byte lines when written out.
-constexpr int GP_PBLOCK = 1024;
+// Partition passes use 8192-row tiles (512 threads, ~70 KB of LDS) so a
+// digit's run of records fills whole 128-byte lines when written out; 512
+// threads leave the kernel a 256-VGPR budget (1024 would cap it at 128 and
+// spill).
+constexpr int GP_PBLOCK = 512;
 constexpr int GP_WAVES = GP_PBLOCK / WAVE;
 constexpr int GP_IPT = 16;
 constexpr int GP_TILE = GP_PBLOCK * GP_IPT;
@@ -231,14 +233,14 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
   int32_t vv[GP_IPT];
   uint32_t live = 0;  // bit j: item j is a record of this pass
   if constexpr (FIRST) {
-    // all loads first (independent, coalesced per item), then the filter
+    // key and mask loads first (independent, coalesced per item), then the
+    // filter; values are loaded at the LDS scatter, for records only
     uint32_t okm = 0;
 #pragma unroll
     for (int j = 0; j < GP_IPT; ++j) {
       const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
       const bool in = pos < n;
       ov[j] = in ? uint32_t(keys[pos]) : 0u;
-      vv[j] = in ? vals[pos] : 0;
       okm |= uint32_t(in && (vm == nullptr || vm[pos] != 0)) << j;
     }
 #pragma unroll
@@ -302,7 +304,10 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
       const uint32_t d = (ov[j] >> shift) & 127u;
       const uint32_t p = s_tstart[d] + s_wh[wave][d] + rk[j];
       s_o[p] = ov[j];
-      s_v[p] = vv[j];
+      if constexpr (FIRST)
+        s_v[p] = vals[wbase + uint32_t(j * WAVE + lane)];
+      else
+        s_v[p] = vv[j];
     }
   }
   if (tid < 128) {
