@@ -158,7 +158,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the multi-GPU sharding of the "
-              "group-by / sort is a design note (DESIGN.md).  Run with --gpus 1.",
+              "group-by / sort is a design note only.  Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     import torch  # first: the library then shares torch's HIP runtime

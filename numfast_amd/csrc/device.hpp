@@ -27,6 +27,32 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {
   return peers;
 }
 
+// Stable in-wave rank of digit d through a wave-private LDS counter array
+// `ctr` (one counter per digit), returning the count of this wave's earlier
+// items with the same digit.  Calls for successive items of a wave see each
+// other's increments (LDS operations of a wave complete in issue order);
+// within one call, lanes that hit the same counter receive ds_add_rtn's old
+// values in ascending lane order -- a property of the LDS that
+// lds_rank_supported() verifies once per device (the ballot-match ranking
+// above is used when it does not hold).  Invalid lanes must be the highest
+// lanes of the wave (a tail).  When every valid lane has the same digit the
+// wave updates the counter once instead of serialising on one address.
+__device__ __forceinline__ uint32_t rank_lds(uint32_t* ctr, uint32_t d, bool valid) {
+  const uint64_t act = __ballot(valid);
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+  const uint64_t same = __ballot(valid && d == d0);
+  if (same == act) {
+    uint32_t base = 0;
+    if (lane_id() == 0) {
+      base = ctr[d0];
+      ctr[d0] = base + uint32_t(__popcll(act));
+    }
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + uint32_t(__popcll(act & lanemask_lt()));
+  }
+  return valid ? atomicAdd(&ctr[d], 1u) : 0u;
+}
+
 // Inclusive scan of a 32-bit value across the wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const int lane = lane_id();

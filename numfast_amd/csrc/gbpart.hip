@@ -201,7 +201,7 @@ __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ h
 // One onesweep pass over records ordered by digit (o >> (11 + 7p)) & 127.
 // FIRST reads raw rows: NULL rows and hot rows (already aggregated by
 // k_gp_hist) are dropped, the others become records.
-template <bool FIRST>
+template <bool FIRST, bool LR>
 __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n_rows, int32_t kmin,
@@ -265,21 +265,29 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     }
   }
 
+  // Stable in-wave ranks (position order: item j of lane l, then lane l+1,
+  // then item j+1).  LR: one lane-ordered LDS atomic per record, else the
+  // ballot digit match.  Dropped rows are not a tail of the wave, so the
+  // atomic form is used directly (no uniform-digit shortcut).
   uint32_t rk[GP_IPT];
 #pragma unroll
   for (int j = 0; j < GP_IPT; ++j) {
     const bool valid = (live >> j) & 1u;
     const uint32_t d = (ov[j] >> shift) & 127u;
-    const uint64_t peers = match_digit8(d, valid);
-    const uint32_t r = __popcll(peers & lanemask_lt());
-    const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
-    uint32_t old = 0;
-    if (valid && r == 0) {
-      old = s_wh[wave][d];
-      s_wh[wave][d] = old + uint32_t(__popcll(peers));
+    if constexpr (LR) {
+      rk[j] = valid ? atomicAdd(&s_wh[wave][d], 1u) : 0u;
+    } else {
+      const uint64_t peers = match_digit8(d, valid);
+      const uint32_t r = __popcll(peers & lanemask_lt());
+      const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+      uint32_t old = 0;
+      if (valid && r == 0) {
+        old = s_wh[wave][d];
+        s_wh[wave][d] = old + uint32_t(__popcll(peers));
+      }
+      old = __shfl(old, leader);
+      rk[j] = old + r;
     }
-    old = __shfl(old, leader);
-    rk[j] = old + r;
   }
   __syncthreads();
 
@@ -446,18 +454,24 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   uint64_t* lb = lookback_words(size_t(tiles) * 128, s);
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
-    NFH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gp_part<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(GP_PART_LDS)));
-    NFH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gp_part<false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(GP_PART_LDS)));
+    const void* fns[4] = {reinterpret_cast<const void*>(&k_gp_part<true, true>),
+                          reinterpret_cast<const void*>(&k_gp_part<true, false>),
+                          reinterpret_cast<const void*>(&k_gp_part<false, true>),
+                          reinterpret_cast<const void*>(&k_gp_part<false, false>)};
+    for (const void* f : fns)
+      NFH_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(GP_PART_LDS)));
   });
+  const bool lr = lds_rank_supported();
+  const auto part_first = lr ? &k_gp_part<true, true> : &k_gp_part<true, false>;
+  const auto part_next = lr ? &k_gp_part<false, true> : &k_gp_part<false, false>;
   // pass 1 always runs: it drops NULL and hot rows (digit 0 when nb <= 1)
   {
     // reads key + validity of every row and the value of each record,
     // writes (o, value) per record
     ProfScope prof("gp_part_first", s, double(n) * (4.0 + mask_b) + double(m) * 12.0);
     const uint32_t epoch = lookback_epoch(s);
-    hipLaunchKernelGGL(k_gp_part<true>, dim3(tiles), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n,
+    hipLaunchKernelGGL(part_first, dim3(tiles), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n,
                        kmin, nullptr, nullptr, 0u, o0, v0, GP_BUCKET_BITS, dstart, lb, ctr, epoch,
                        table);
     NFH_LAUNCH_CHECK();
@@ -469,7 +483,7 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
       ProfScope prof("gp_part", s, double(m) * 16.0);
       const uint32_t t2 = (m + GP_TILE - 1) / GP_TILE;
       const uint32_t epoch = lookback_epoch(s);
-      hipLaunchKernelGGL(k_gp_part<false>, dim3(t2), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
+      hipLaunchKernelGGL(part_next, dim3(t2), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
                          nullptr, 0u, kmin, o0, v0, m, o1, v1, GP_BUCKET_BITS + 7, dstart + 128,
                          lb, ctr + 1, epoch, nullptr);
       NFH_LAUNCH_CHECK();

@@ -50,7 +50,11 @@ __device__ __forceinline__ uint32_t hot_hash(int32_t k) {
 }
 
 // Rows are visited in quads (128-bit key/value loads, 32-bit mask loads)
-// when every pointer is suitably aligned, else one by one.
+// when every pointer is suitably aligned, else one by one.  Each thread
+// issues the loads of RQ quads before using any of them, so enough bytes
+// are in flight per CU to cover HBM latency.
+constexpr int RQ = 4;
+
 template <bool VEC, class F>
 __device__ __forceinline__ void for_each_row(const int32_t* __restrict__ keys,
                                              const int32_t* __restrict__ vals,
@@ -61,7 +65,27 @@ __device__ __forceinline__ void for_each_row(const int32_t* __restrict__ keys,
   uint32_t tail = 0;
   if constexpr (VEC) {
     const uint32_t nq = n / 4;
-    for (uint32_t q = tid; q < nq; q += stride) {
+    uint32_t q = tid;
+    for (; q + (RQ - 1) * stride < nq; q += RQ * stride) {
+      int4 k4[RQ], v4[RQ];
+      uint32_t a[RQ], b[RQ];
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        const uint32_t qq = q + uint32_t(u) * stride;
+        k4[u] = reinterpret_cast<const int4*>(keys)[qq];
+        v4[u] = vals ? reinterpret_cast<const int4*>(vals)[qq] : make_int4(0, 0, 0, 0);
+        a[u] = vm ? reinterpret_cast<const uint32_t*>(vm)[qq] : 0x01010101u;
+        b[u] = km ? reinterpret_cast<const uint32_t*>(km)[qq] : 0x01010101u;
+      }
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        f(k4[u].x, v4[u].x, (a[u] & 0xFFu) && (b[u] & 0xFFu));
+        f(k4[u].y, v4[u].y, (a[u] & 0xFF00u) && (b[u] & 0xFF00u));
+        f(k4[u].z, v4[u].z, (a[u] & 0xFF0000u) && (b[u] & 0xFF0000u));
+        f(k4[u].w, v4[u].w, (a[u] & 0xFF000000u) && (b[u] & 0xFF000000u));
+      }
+    }
+    for (; q < nq; q += stride) {
       const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
       const int4 v4 = vals ? reinterpret_cast<const int4*>(vals)[q] : make_int4(0, 0, 0, 0);
       const uint32_t a = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
@@ -542,11 +566,54 @@ template <bool VEC>
 __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
                                               const uint8_t* __restrict__ v, uint32_t n,
                                               int32_t kmin, uint32_t* __restrict__ flag) {
-  for_each_row<VEC>(keys, nullptr, v, nullptr, n, [&](int32_t k, int32_t, bool ok) {
-    if (!ok) return;
-    uint32_t* p = flag + uint32_t(k - kmin);
-    if (*p == 0u) *p = 1u;  // set-once flag: most rows only read
-  });
+  // Set-once flags: most rows only read.  In the quad path all 4*RQ flag
+  // reads of a thread are issued before any store (rows with equal keys
+  // may both store 1).
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t tail = 0;
+  if constexpr (VEC) {
+    const uint32_t nq = n / 4;
+    uint32_t q = tid;
+    for (; q + (RQ - 1) * stride < nq; q += RQ * stride) {
+      uint32_t off[4 * RQ], f[4 * RQ];
+      uint32_t ok = 0;
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        const uint32_t qq = q + uint32_t(u) * stride;
+        const int4 k4 = reinterpret_cast<const int4*>(keys)[qq];
+        const uint32_t m = v ? reinterpret_cast<const uint32_t*>(v)[qq] : 0x01010101u;
+        off[4 * u + 0] = uint32_t(k4.x - kmin);
+        off[4 * u + 1] = uint32_t(k4.y - kmin);
+        off[4 * u + 2] = uint32_t(k4.z - kmin);
+        off[4 * u + 3] = uint32_t(k4.w - kmin);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ok |= uint32_t((m >> (8 * c)) & 0xFFu ? 1u : 0u) << (4 * u + c);
+      }
+#pragma unroll
+      for (int j = 0; j < 4 * RQ; ++j) f[j] = ((ok >> j) & 1u) ? flag[off[j]] : 1u;
+#pragma unroll
+      for (int j = 0; j < 4 * RQ; ++j)
+        if (f[j] == 0u) flag[off[j]] = 1u;
+    }
+    for (; q < nq; q += stride) {
+      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
+      const uint32_t m = v ? reinterpret_cast<const uint32_t*>(v)[q] : 0x01010101u;
+      const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (!((m >> (8 * c)) & 0xFFu)) continue;
+        uint32_t* p = flag + uint32_t(kk[c] - kmin);
+        if (*p == 0u) *p = 1u;
+      }
+    }
+    tail = nq * 4;
+  }
+  for (uint32_t i = tail + tid; i < n; i += stride) {
+    if (v != nullptr && v[i] == 0) continue;
+    uint32_t* p = flag + uint32_t(keys[i] - kmin);
+    if (*p == 0u) *p = 1u;
+  }
 }
 
 struct RankOp {
@@ -572,7 +639,29 @@ __global__ __launch_bounds__(256) void k_inv_dense(const int32_t* __restrict__ k
   uint32_t tail = 0;
   if constexpr (VEC) {
     const uint32_t nq = n / 4;
-    for (uint32_t q = tid; q < nq; q += stride) {
+    uint32_t q = tid;
+    // RQ quads per thread: key loads, then all rank gathers, then stores
+    for (; q + (RQ - 1) * stride < nq; q += RQ * stride) {
+      int4 k4[RQ];
+      uint32_t m[RQ];
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        const uint32_t qq = q + uint32_t(u) * stride;
+        k4[u] = reinterpret_cast<const int4*>(keys)[qq];
+        m[u] = v ? reinterpret_cast<const uint32_t*>(v)[qq] : 0x01010101u;
+      }
+      int4 o[RQ];
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        o[u].x = (m[u] & 0xFFu) ? int32_t(rank[uint32_t(k4[u].x - kmin)]) : -1;
+        o[u].y = (m[u] & 0xFF00u) ? int32_t(rank[uint32_t(k4[u].y - kmin)]) : -1;
+        o[u].z = (m[u] & 0xFF0000u) ? int32_t(rank[uint32_t(k4[u].z - kmin)]) : -1;
+        o[u].w = (m[u] & 0xFF000000u) ? int32_t(rank[uint32_t(k4[u].w - kmin)]) : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) reinterpret_cast<int4*>(inv)[q + uint32_t(u) * stride] = o[u];
+    }
+    for (; q < nq; q += stride) {
       const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
       const uint32_t m = v ? reinterpret_cast<const uint32_t*>(v)[q] : 0x01010101u;
       int4 o;

@@ -4,6 +4,7 @@
 #include <mutex>
 #include <string>
 
+#include "device.hpp"
 #include "runtime.hpp"
 
 namespace nfh {
@@ -57,6 +58,55 @@ void ensure_device() {
 int num_cus() {
   ensure_device();
   return g_cus > 0 ? g_cus : 256;
+}
+
+// Check of the LDS property rank_lds relies on: the old values that
+// ds_add_rtn returns to lanes of one instruction hitting one address rise
+// with the lane index.  Compares against the ballot-match rank over digit
+// patterns with 2..256 distinct values per wave; counts mismatches.
+__global__ __launch_bounds__(1024) void k_lds_rank_check(uint32_t trials,
+                                                         unsigned long long* bad) {
+  __shared__ uint32_t cnt[16][256];
+  const int lane = lane_id(), wave = wave_id();
+  for (int i = threadIdx.x; i < 16 * 256; i += blockDim.x) (&cnt[0][0])[i] = 0;
+  __syncthreads();
+  unsigned long long v = 0;
+  for (uint32_t t = 0; t < trials; ++t) {
+    uint32_t h = (t * 2654435761u) ^ (blockIdx.x * 40503u + uint32_t(wave) * 7919u + lane);
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    const uint32_t mask = (t % 4 == 0) ? 255u : (t % 4 == 1) ? 1u : (t % 4 == 2) ? 3u : 31u;
+    const uint32_t d = (h >> 7) & mask;
+    const uint32_t old = atomicAdd(&cnt[wave][d], 1u);
+    const uint64_t peers = match_digit8(d, true);
+    const uint32_t r = uint32_t(__popcll(peers & lanemask_lt()));
+    const uint32_t lead_old = __shfl(old, __ffsll((unsigned long long)peers) - 1);
+    v += (old != lead_old + r) ? 1u : 0u;
+  }
+  if (v) atomicAdd(bad, v);
+}
+
+bool lds_rank_supported() {
+  static std::once_flag once;
+  static bool ok = false;
+  std::call_once(once, [] {
+    ensure_device();
+    unsigned long long* d = nullptr;
+    if (hipMalloc(&d, sizeof(unsigned long long)) != hipSuccess) return;
+    unsigned long long h = 1;
+    hipStream_t s = stream();
+    fill_u64(d, 0, 1, s);
+    {
+      hipLaunchKernelGGL(k_lds_rank_check, dim3(64), dim3(1024), 0, s, 256u, d);
+      if (hipGetLastError() == hipSuccess &&
+          hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
+          hipStreamSynchronize(s) == hipSuccess)
+        ok = (h == 0);
+    }
+    (void)hipFree(d);
+  });
+  return ok;
 }
 
 hipStream_t stream() {
@@ -128,6 +178,8 @@ bool is_device_ptr(const void* p) {
 Stage::Stage() {
   ensure_device();
   s = stream();
+  // one-time device checks run here, before any caller could be capturing
+  (void)lds_rank_supported();
 }
 
 const void* Stage::in_raw(const void* p, size_t bytes, int slot) {

@@ -148,6 +148,11 @@ void fill_u64(void* p, uint64_t v, size_t count, hipStream_t s);
 
 int num_cus();
 
+// True when LDS atomics return old values in ascending lane order for
+// lanes of one instruction that hit one address (checked once per process
+// on the current device; the stable rank_lds ranking depends on it).
+bool lds_rank_supported();
+
 // Optional per-kernel timing with HIP events on the launch stream
 // (nfh_profile_enable).  A scope brackets one kernel launch; durations are
 // collected after the stream is synchronised (Stage::finish / prof_collect).

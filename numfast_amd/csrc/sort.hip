@@ -10,10 +10,11 @@
 //   2. k_bins: exclusive scan per digit position -> global bin starts, and
 //      a flag for digit positions where every key shares one bin (such a
 //      pass is the identity and is skipped).
-//   3. k_onesweep per active pass: a workgroup claims a 4096-key tile
+//   3. k_onesweep per active pass: a workgroup claims a 16384-key tile
 //      (atomic tile counter, so every earlier tile is already resident),
-//      ranks keys stably inside the tile with wave64 8-ballot digit match
-//      and per-wave LDS counters, publishes its per-digit counts and walks
+//      ranks keys stably inside the tile with per-wave LDS digit counters
+//      (one lane-ordered LDS atomic per key, or a wave64 8-ballot digit
+//      match where that LDS property is absent), publishes its per-digit counts and walks
 //      back over earlier tiles (decoupled look-back) for its global offsets,
 //      reorders the tile in LDS and writes it out in digit runs.
 // The first active pass reads the raw keys and generates row ids; the last
@@ -108,7 +109,8 @@ __global__ __launch_bounds__(256) void k_bins(const uint32_t* __restrict__ hist,
 // ------------------------------------------------------------ one pass
 // FIRST: read raw keys (convert to words) and generate row ids.
 // LAST: write only the permutation (+ sorted keys when kout != nullptr).
-template <class K, bool FIRST, bool LAST>
+// LR: rank with lane-ordered LDS atomics (rank_lds).
+template <class K, bool FIRST, bool LAST, bool LR>
 __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     const K* __restrict__ keys, const typename Key<K>::W* __restrict__ win,
     const int32_t* __restrict__ iin, typename Key<K>::W* __restrict__ wout,
@@ -158,23 +160,28 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
   }
 
   // Stable in-wave ranks: item j of lane l precedes item j of lane l+1,
-  // which precedes item j+1 of lane 0 (that is the position order).
+  // which precedes item j+1 of lane 0 (that is the position order).  LR:
+  // one lane-ordered LDS atomic per item; else the 8-ballot digit match.
   uint32_t rk[SORT_IPT];
 #pragma unroll
   for (int j = 0; j < SORT_IPT; ++j) {
     const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
     const bool valid = pos < n;
     const uint32_t d = uint32_t(kw[j] >> shift) & 255u;
-    const uint64_t peers = match_digit8(d, valid);
-    const uint32_t r = __popcll(peers & lanemask_lt());
-    const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
-    uint32_t old = 0;
-    if (valid && r == 0) {
-      old = s_wh[wave][d];
-      s_wh[wave][d] = old + uint32_t(__popcll(peers));
+    if constexpr (LR) {
+      rk[j] = rank_lds(s_wh[wave], d, valid);
+    } else {
+      const uint64_t peers = match_digit8(d, valid);
+      const uint32_t r = __popcll(peers & lanemask_lt());
+      const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+      uint32_t old = 0;
+      if (valid && r == 0) {
+        old = s_wh[wave][d];
+        s_wh[wave][d] = old + uint32_t(__popcll(peers));
+      }
+      old = __shfl(old, leader);
+      rk[j] = old + r;
     }
-    old = __shfl(old, leader);
-    rk[j] = old + r;
   }
   __syncthreads();
 
@@ -272,9 +279,13 @@ static void launch_pass(const K* keys, const typename Key<K>::W* win, const int3
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
     NFH_CHECK(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_onesweep<K, FIRST, LAST>),
+        reinterpret_cast<const void*>(&k_onesweep<K, FIRST, LAST, true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(G::LDS)));
+    NFH_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_onesweep<K, FIRST, LAST, false>),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(G::LDS)));
   });
+  const bool lr = lds_rank_supported();
   const uint32_t tiles = (n + G::TILE - 1) / G::TILE;
   const uint32_t epoch = lookback_epoch(s);
   // pass model: first pass reads keys, later ones (word, row id); the last
@@ -282,8 +293,13 @@ static void launch_pass(const K* keys, const typename Key<K>::W* win, const int3
   const double in_b = FIRST ? sizeof(K) : sizeof(typename Key<K>::W) + 4.0;
   const double out_b = LAST ? 4.0 + (kout ? sizeof(K) : 0.0) : sizeof(typename Key<K>::W) + 4.0;
   ProfScope prof("radix_onesweep", s, double(n) * (in_b + out_b));
-  hipLaunchKernelGGL((k_onesweep<K, FIRST, LAST>), dim3(tiles), dim3(G::BLOCK), G::LDS, s, keys,
-                     win, iin, wout, iout, kout, bin_start, lb, ctr, n, shift, desc, epoch);
+  if (lr)
+    hipLaunchKernelGGL((k_onesweep<K, FIRST, LAST, true>), dim3(tiles), dim3(G::BLOCK), G::LDS, s,
+                       keys, win, iin, wout, iout, kout, bin_start, lb, ctr, n, shift, desc, epoch);
+  else
+    hipLaunchKernelGGL((k_onesweep<K, FIRST, LAST, false>), dim3(tiles), dim3(G::BLOCK), G::LDS,
+                       s, keys, win, iin, wout, iout, kout, bin_start, lb, ctr, n, shift, desc,
+                       epoch);
   NFH_LAUNCH_CHECK();
 }
 

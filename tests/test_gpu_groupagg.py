@@ -72,6 +72,23 @@ def test_group_agg_parity(n, ids, kind):
     _check(native.group_agg(keys, vals, valid, kv), ox.group_agg(keys, vals, valid, kv))
 
 
+@pytest.mark.parametrize("n,ids", [(1 << 20, 1 << 16), (1 << 22, 1 << 24)])
+def test_group_agg_mixed_value_ranges(n, ids):
+    """Hot keys see values both inside and outside [-2^15, 2^15) (the two
+    accumulator forms of the hot path), including the exact boundaries."""
+    rng = np.random.default_rng(7)
+    keys = zipf_keys(rng, n, ids, 1.1)
+    vals = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
+    wide = rng.random(n) < 0.2
+    vals[wide] = rng.integers(I32MIN, I32MAX, int(wide.sum()), dtype=np.int64,
+                              endpoint=True).astype(np.int32)
+    edge = rng.random(n) < 0.02
+    vals[edge] = rng.choice(np.array([-32769, -32768, 32767, 32768, I32MIN, I32MAX], np.int32),
+                            int(edge.sum()))
+    valid = rng.random(n) > 0.05
+    _check(native.group_agg(keys, vals, valid), ox.group_agg(keys, vals, valid))
+
+
 def test_group_agg_extreme_values():
     keys = np.array([3, 3, 3, 4, 4, 9], np.int32)
     vals = np.array([I32MAX, I32MAX, I32MAX, I32MIN, I32MIN, 0], np.int32)
