@@ -154,6 +154,78 @@ def cpu_baseline(n_sample=1 << 22):
                        "of 3" % n_sample)}
 
 
+def sort_pipeline_leg(torch, L, W, native, n, steps):
+    """Per distribution: plain vs learned argsort (model trained once on the
+    column, training time reported apart) and the filter -> derive -> sort
+    -> limit(1000) pipeline with each sort; outputs compared for identity."""
+    import ctypes
+    out = {}
+    perm0 = torch.empty(n, dtype=torch.int32, device="cuda")
+    perm1 = torch.empty_like(perm0)
+    t0b, t1b, tpb = (torch.empty_like(perm0) for _ in range(3))
+    lim = 1000
+    w0 = torch.empty(lim, dtype=torch.int32, device="cuda")
+    r0, w1, r1 = torch.empty_like(w0), torch.empty_like(w0), torch.empty_like(w0)
+    for dist in ("uniform", "swaps", "distinct1024"):
+        k = W.sort_config(n, dist)
+        torch.cuda.synchronize()
+        model = native.CdfModel()
+        loss = ctypes.c_float()
+        t = time.perf_counter()
+        rc = L.nfh_cdf_train_i32(k.data_ptr(), n, 100, ctypes.byref(model), ctypes.byref(loss))
+        train_s = time.perf_counter() - t
+        if rc != 0:
+            raise RuntimeError("train rc %d" % rc)
+        stats = (ctypes.c_uint64 * 4)()
+
+        def plain():
+            if L.nf_sort_perm_i32(k.data_ptr(), n, 0, perm0.data_ptr(), t0b.data_ptr(),
+                                  t1b.data_ptr(), tpb.data_ptr()) != 0:
+                raise RuntimeError("sort")
+
+        def learned():
+            if L.nfh_sort_perm_learned_i32(k.data_ptr(), n, ctypes.byref(model), perm1.data_ptr(),
+                                           stats) != 0:
+                raise RuntimeError("learned sort")
+        tp = timed(torch, plain, steps, 1)
+        L.nfh_profile_reset()
+        L.nfh_profile_enable(1)
+        tl = timed(torch, learned, steps, 1)
+        L.nfh_profile_enable(0)
+        lk = kernel_table(L)
+        same = bool(torch.equal(perm0, perm1))
+        # pipeline: model for the derived column w = v * 3 (int32 wrap)
+        wcol = (k[k > 0].to(torch.int64) * 3).to(torch.int32)
+        pmodel = native.CdfModel()
+        if L.nfh_cdf_train_i32(wcol.data_ptr(), wcol.numel(), 100, ctypes.byref(pmodel),
+                               None) != 0:
+            raise RuntimeError("train rc")
+        del wcol
+
+        def pipe(m, wo, ro):
+            def f():
+                r = L.nfh_filter_derive_sort_limit_i32(k.data_ptr(), n, 3, lim, m, wo.data_ptr(),
+                                                       ro.data_ptr())
+                if r < 0:
+                    raise RuntimeError("pipeline rc %d" % r)
+            return f
+        tpp = timed(torch, pipe(None, w0, r0), steps, 1)
+        tpl = timed(torch, pipe(ctypes.byref(pmodel), w1, r1), steps, 1)
+        out[dist] = {
+            "plain_ms": round(tp * 1e3, 3), "learned_ms": round(tl * 1e3, 3),
+            "speedup": round(tp / tl, 3), "identical_perm": same,
+            "plain_rows_per_s": n / tp, "learned_rows_per_s": n / tl,
+            "plain_frac_60B_model": round(60.0 * n / tp / 1e9 / HBM_PEAK_GBS, 4),
+            "learned_frac_44B_model": round(44.0 * n / tl / 1e9 / HBM_PEAK_GBS, 4),
+            "train_ms": round(train_s * 1e3, 2), "train_loss": float(loss.value),
+            "buckets": [int(x) for x in stats], "learned_kernels": lk,
+            "pipeline_plain_ms": round(tpp * 1e3, 3), "pipeline_learned_ms": round(tpl * 1e3, 3),
+            "pipeline_speedup": round(tpp / tpl, 3),
+            "pipeline_identical": bool(torch.equal(w0, w1) and torch.equal(r0, r1))}
+        del k
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -241,6 +313,10 @@ def main():
             "compulsory_frac_8B": round(8.0 * n / ts / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": sk, "roofline": roofline_of(sk, sort_steps)}
         del k, perm, t0b, t1b, tpb
+        # config-3 shapes: plain LSD vs learned-CDF sort, and the
+        # filter(v>0) -> derive(v*3) -> sort -> limit(1000) pipeline
+        extra["sort_pipeline_2^28"] = sort_pipeline_leg(torch, L, W, native, n,
+                                                        max(3, args.steps // 2))
         # uniform-2^24 group-by variant
         keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")
         tu = timed(torch, step_on(keys, vals, valid, kvalid), max(3, args.steps // 2), 1)

@@ -99,6 +99,22 @@ int32_t nf_sort_perm_i32(const int32_t *keys, size_t n, uint8_t descending, int3
 int32_t nf_sort_perm_i64(const int64_t *keys, size_t n, uint8_t descending, int32_t *perm_out,
                          uint64_t *tmp0, uint64_t *tmp1, int32_t *tmp_p);
 
+/* ---- select / mask (IR filter): lib.rs:616-1062, series/select.rs ---- */
+
+/* lib.rs:630-641 -- rows with mask != 0; -2 on a null pointer. */
+int64_t nf_select_count(const uint8_t *mask, size_t n);
+/* lib.rs:643-700 -- order-preserving compaction of src where mask != 0
+ * into out[0..m); returns m, -2 on a null pointer. */
+int64_t nf_select_scatter_i32(const int32_t *src, const uint8_t *mask, size_t n, int32_t *out);
+int64_t nf_select_scatter_i64(const int64_t *src, const uint8_t *mask, size_t n, int64_t *out);
+int64_t nf_select_scatter_f32(const float *src, const uint8_t *mask, size_t n, float *out);
+int64_t nf_select_scatter_f64(const double *src, const uint8_t *mask, size_t n, double *out);
+int64_t nf_select_scatter_u8(const uint8_t *src, const uint8_t *mask, size_t n, uint8_t *out);
+/* lib.rs:1014-1062 -- elementwise mask algebra, output 0/1; 0 or -1. */
+int32_t nf_mask_and(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out);
+int32_t nf_mask_or(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out);
+int32_t nf_mask_not(const uint8_t *a, size_t n, uint8_t *out);
+
 /* ---- additive surface (no reference counterpart) --------------------- */
 
 /* Order this thread's calls on a caller stream (hipStream_t); NULL
@@ -134,6 +150,43 @@ int64_t nfh_group_agg_i32(const int32_t *keys, const int32_t *vals, const uint8_
  * (src/Drivers/CPU/_lib/cpu.py:4032-4036).  Returns ng. */
 int64_t nfh_unique_valid_i32(const int32_t *keys, const uint8_t *valid, size_t n,
                              int32_t *uniq_out, int32_t *inv_out);
+
+/* Learned-CDF model: a 1 -> 32 -> 32 -> 1 fp32 ReLU network over the
+ * normalised key word (w - wmin) / span, w = (uint32)key ^ 0x80000000.
+ * p = w1[32] b1[32] w2[32][32] b2[32] w3[32] b3. */
+typedef struct nfh_cdf_model {
+  float p[1153];
+  uint32_t wmin;
+  uint32_t span;
+  uint32_t pad;
+} nfh_cdf_model;
+
+/* Train a model on a column (2^16-key sample sorted by the radix argsort,
+ * plus the exact CDF at top-byte boundaries from the radix histogram
+ * kernel; Adam on the device for `steps` full-batch steps).  *loss_out
+ * (may be NULL): final mean squared CDF error.  0, -1, -3 or -6. */
+int32_t nfh_cdf_train_i32(const int32_t *keys, size_t n, uint32_t steps,
+                          nfh_cdf_model *model_out, float *loss_out);
+/* Stable ascending argsort through the model's bucket code; the result
+ * equals nf_sort_perm_i32(keys, n, 0, ...) for any model.  stats (may be
+ * NULL, 4 words): non-empty buckets, largest bucket, buckets over the LDS
+ * capacity, buckets re-sorted by the overflow fix-up. */
+int32_t nfh_sort_perm_learned_i32(const int32_t *keys, size_t n, const nfh_cdf_model *model,
+                                  int32_t *perm_out, uint64_t *stats);
+/* Mean-squared-error gradient of the network at params[1153] over (x, y),
+ * computed by the training kernels (grad_out[1153]; loss_out, pred_out[n]
+ * may be NULL). */
+int32_t nfh_mlp_grad(const float *params, const float *x, const float *y, size_t n,
+                     float *grad_out, float *loss_out, float *pred_out);
+
+/* filter(v > 0) -> derive(w = v * mul, int32 wrap) -> stable sort by w ->
+ * limit: the first min(limit, m) rows as (w, source row index); returns
+ * that count.  model NULL: LSD radix sort; else the learned-CDF sort
+ * (identical output).  cpu.py:3034-3035, :3074, :1174-1225;
+ * src/Semantic/TableExpr/_lib/chain.py:978-1008. */
+int64_t nfh_filter_derive_sort_limit_i32(const int32_t *v, size_t n, int32_t mul, size_t limit,
+                                         const nfh_cdf_model *model, int32_t *w_out,
+                                         int32_t *rows_out);
 
 #ifdef __cplusplus
 }

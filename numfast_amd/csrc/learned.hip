@@ -1,0 +1,1018 @@
+// learned.hip -- learned-CDF bucket sort (exact) and its GPU training.
+//
+// Model: a 1 -> 32 -> 32 -> 1 fp32 MLP with ReLU hidden layers maps the
+// normalised key word x = (w - wmin) / span to an estimate of the key CDF.
+//
+// Training (cdf_train_i32): a strided sample of 2^16 keys is sorted by the
+// plain radix argsort, whose histogram kernel also runs over the full
+// column; the training set is the sample's empirical CDF points plus the
+// exact CDF at every top-byte boundary from that histogram.  The network
+// starts as a 32-piece piecewise-linear interpolant of the sample CDF
+// (layer 1: ReLU ramps at greedily placed knots of the empirical CDF,
+// layer 2: identity, layer 3: slope changes) and is refined by full-batch
+// Adam on the device: k_mlp_grad (forward + backward, one sample per
+// thread, per-workgroup partial gradients) and k_mlp_adam (fixed-order
+// reduction + update) with backtracking: a step that raises the training
+// loss is undone and the step size halved, so the result is never worse
+// than the initial fit.  Nothing
+// about correctness depends on the fit.
+//
+// Sort (learned_argsort_i32):
+//   1. the model is tabulated at 2049 knots into integer codes scaled to
+//      [0, 2^bits * 16], made non-decreasing by a running max;
+//   2. the bucket code of a key word interpolates the knots in integer
+//      arithmetic, so code(w) is non-decreasing in w: every key of bucket b
+//      is <= every key of bucket b+1;
+//   3. one or two stable onesweep passes (the plain sort's pass kernel with
+//      the code's bytes as digits) order (word, row) pairs by code;
+//   4. bucket starts come from a binary search per code; one workgroup per
+//      bucket sorts it in LDS (stable LSD over the bucket-relative key,
+//      9-bit digits, as many passes as the bucket's key range needs) and
+//      writes the rows;
+//   5. buckets beyond the LDS capacity are either all-equal (already in
+//      stable order: rows copied) or re-sorted by the plain radix argsort
+//      (the overflow fix-up).
+// The result is the unique stable ascending order, identical to the plain
+// LSD argsort's, whatever the model predicts.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "device.hpp"
+#include "learned.hpp"
+#include "onesweep.hpp"
+#include "runtime.hpp"
+#include "sort.hpp"
+
+namespace nfh {
+
+constexpr int KNOTS = 2048;
+constexpr int LS_SAMPLE = 1 << 16;
+constexpr int LS_BLOCK = 512;
+constexpr int LS_WAVES = LS_BLOCK / WAVE;
+constexpr int LS_IPT = 10;
+constexpr int LS_CAP = LS_BLOCK * LS_IPT;  // 5120 keys per bucket in LDS
+constexpr int LS_DBITS = 9;                // in-bucket digit width
+constexpr int LS_DIG = 1 << LS_DBITS;
+constexpr int LS_CHUNK = 4096;             // rows per workgroup, large buckets
+
+// ------------------------------------------------------------ code map
+// 32-bit integer arithmetic: with u = clamp(w - wmin, 0, span),
+// p = umulhi(u << shl, mul) = floor(u * 2^21 / (span + 1)) up to a relative
+// 2^-21 (shl = clz(span) normalises, mul = 2^53 / ((span + 1) << shl) in
+// [2^21, 2^22]), a knot index . 10-bit fraction below 2^21; knots are in
+// units of code / 16 (<= 2^20), so every product fits 32 bits.
+constexpr int KN_FRAC = 10;  // fraction bits of the knot position
+constexpr int KN_UNIT = 4;   // knot values carry 4 fractional code bits
+struct LearnedMap {
+  const uint32_t* knots;  // KNOTS + 1 non-decreasing, units of code / 16
+  uint32_t wmin, span;
+  uint32_t shl, mul;
+  uint32_t maxcode;
+};
+
+// Non-decreasing in w: every step is a non-decreasing map (clamp, shift,
+// multiply-high by mul >= 0, then linear interpolation between
+// non-decreasing knots, whose value at fraction 2^KN_FRAC - 1 of knot k
+// never exceeds knot k + 1).
+__device__ __forceinline__ uint32_t lcode(uint32_t w, const uint32_t* kn, const LearnedMap& m) {
+  uint32_t u = w > m.wmin ? w - m.wmin : 0u;
+  u = u < m.span ? u : m.span;
+  const uint32_t p = __umulhi(u << m.shl, m.mul);
+  const uint32_t k = p >> KN_FRAC;
+  const uint32_t f = p & ((1u << KN_FRAC) - 1u);
+  const uint32_t t0 = kn[k], t1 = kn[k + 1];
+  const uint32_t c = ((t0 << KN_FRAC) + (t1 - t0) * f) >> (KN_FRAC + KN_UNIT);
+  return c < m.maxcode ? c : m.maxcode;
+}
+
+struct LearnedDigit {
+  LearnedMap m;
+  int shift;
+  static constexpr size_t EXTRA_LDS = ((KNOTS + 1) * 4 + 15) / 16 * 16;
+  __device__ void init(unsigned char* s) const {
+    uint32_t* k = reinterpret_cast<uint32_t*>(s);
+    for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) k[i] = m.knots[i];
+  }
+  template <class W>
+  __device__ uint32_t digit(W w, const unsigned char* s) const {
+    return (lcode(uint32_t(w), reinterpret_cast<const uint32_t*>(s), m) >> shift) & 255u;
+  }
+};
+
+// ------------------------------------------------------------ MLP
+__device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
+
+__device__ float mlp_eval(const float* P, float x) {
+  float h1[MLP_H];
+#pragma unroll
+  for (int k = 0; k < MLP_H; ++k) h1[k] = relu(fmaf(P[MLP_W1 + k], x, P[MLP_B1 + k]));
+  float out = P[MLP_B3];
+  for (int i = 0; i < MLP_H; ++i) {
+    float a = P[MLP_B2 + i];
+#pragma unroll
+    for (int j = 0; j < MLP_H; ++j) a = fmaf(P[MLP_W2 + i * MLP_H + j], h1[j], a);
+    out = fmaf(P[MLP_W3 + i], relu(a), out);
+  }
+  return out;
+}
+
+constexpr int TG_BLOCK = 64;  // one wave per workgroup
+constexpr int TG_GRID = 128;  // workgroups of the gradient kernel (at most)
+
+// Forward + backward of the mean squared error over samples (x, y).  Each
+// workgroup loops over chunks of TG_BLOCK samples (one per thread).  A
+// thread keeps its sample's activations as a row of LDS arrays (h1, h2,
+// dh2, dh1); per-chunk gradient sums run over the rows in a fixed order and
+// accumulate in registers.  The workgroup then writes its gradient
+// (MLP_P floats) and loss share to part[blockIdx.x * (MLP_P + 1) ...].
+// pred (optional) gets the outputs.  Deterministic for a given grid.
+__global__ __launch_bounds__(TG_BLOCK) void k_mlp_grad(const float* __restrict__ Pg,
+                                                       const float* __restrict__ xs,
+                                                       const float* __restrict__ ys, uint32_t S,
+                                                       float inv_s, float* __restrict__ part,
+                                                       float* __restrict__ pred) {
+  __shared__ float P[MLP_P];
+  __shared__ float s_h1[TG_BLOCK][MLP_H + 1];
+  __shared__ float s_h2[TG_BLOCK][MLP_H + 1];
+  __shared__ float s_d2[TG_BLOCK][MLP_H + 1];
+  __shared__ float s_d1[TG_BLOCK][MLP_H + 1];
+  __shared__ float s_x[TG_BLOCK], s_dout[TG_BLOCK], s_r2[TG_BLOCK];
+  constexpr int NW2 = MLP_H * MLP_H / TG_BLOCK;  // W2 entries per thread
+  const int t = threadIdx.x;
+  for (int q = t; q < MLP_P; q += TG_BLOCK) P[q] = Pg[q];
+  __syncthreads();
+  float acc_w2[NW2];
+#pragma unroll
+  for (int e = 0; e < NW2; ++e) acc_w2[e] = 0.f;
+  float acc_a0 = 0.f, acc_a1 = 0.f, acc_b0 = 0.f, acc_b1 = 0.f;
+  for (uint32_t c0 = blockIdx.x * TG_BLOCK; c0 < S; c0 += gridDim.x * TG_BLOCK) {
+    const uint32_t sidx = c0 + t;
+    const bool valid = sidx < S;
+    const float x = valid ? xs[sidx] : 0.f;
+    const float y = valid ? ys[sidx] : 0.f;
+    for (int k = 0; k < MLP_H; ++k) s_h1[t][k] = relu(fmaf(P[MLP_W1 + k], x, P[MLP_B1 + k]));
+    float out = P[MLP_B3];
+#pragma unroll 1
+    for (int i = 0; i < MLP_H; ++i) {
+      float a = P[MLP_B2 + i];
+#pragma unroll 8
+      for (int jj = 0; jj < MLP_H; ++jj) a = fmaf(P[MLP_W2 + i * MLP_H + jj], s_h1[t][jj], a);
+      const float h2 = relu(a);
+      s_h2[t][i] = h2;
+      out = fmaf(P[MLP_W3 + i], h2, out);
+    }
+    if (valid && pred) pred[sidx] = out;
+    const float r = out - y;
+    const float dout = valid ? 2.f * r * inv_s : 0.f;
+    for (int i = 0; i < MLP_H; ++i) s_d2[t][i] = s_h2[t][i] > 0.f ? dout * P[MLP_W3 + i] : 0.f;
+#pragma unroll 1
+    for (int jj = 0; jj < MLP_H; ++jj) {
+      float g = 0.f;
+#pragma unroll 8
+      for (int i = 0; i < MLP_H; ++i) g = fmaf(P[MLP_W2 + i * MLP_H + jj], s_d2[t][i], g);
+      s_d1[t][jj] = s_h1[t][jj] > 0.f ? g : 0.f;
+    }
+    s_x[t] = x;
+    s_dout[t] = dout;
+    s_r2[t] = valid ? r * r * inv_s : 0.f;
+    __syncthreads();
+    // sums over the chunk's samples; W2 entry t + e * TG_BLOCK is
+    // (row t / 32 + 2e, column t % 32)
+    {
+      const int jj = t % MLP_H, i0 = t / MLP_H;
+#pragma unroll 1
+      for (int q = 0; q < TG_BLOCK; ++q) {
+        const float h = s_h1[q][jj];
+#pragma unroll
+        for (int e = 0; e < NW2; ++e) acc_w2[e] = fmaf(s_d2[q][i0 + 2 * e], h, acc_w2[e]);
+      }
+    }
+    if (t < MLP_H) {
+      float gb2 = 0.f, gw1 = 0.f, gb1 = 0.f;
+      for (int q = 0; q < TG_BLOCK; ++q) {
+        gb2 += s_d2[q][t];
+        gw1 = fmaf(s_d1[q][t], s_x[q], gw1);
+        gb1 += s_d1[q][t];
+      }
+      acc_a0 += gb2;
+      acc_b0 += gw1;
+      acc_b1 += gb1;
+    } else {
+      const int i = t - MLP_H;  // w3[i]; lane MLP_H also sums b3 and the loss
+      float gw3 = 0.f, gb3 = 0.f, l = 0.f;
+      for (int q = 0; q < TG_BLOCK; ++q) {
+        gw3 = fmaf(s_dout[q], s_h2[q][i], gw3);
+        gb3 += s_dout[q];
+        l += s_r2[q];
+      }
+      acc_a0 += gw3;
+      acc_b0 += gb3;
+      acc_b1 += l;
+    }
+    __syncthreads();
+  }
+  float* out_p = part + size_t(blockIdx.x) * (MLP_P + 1);
+#pragma unroll
+  for (int e = 0; e < NW2; ++e) out_p[MLP_W2 + t + e * TG_BLOCK] = acc_w2[e];
+  if (t < MLP_H) {
+    out_p[MLP_B2 + t] = acc_a0;
+    out_p[MLP_W1 + t] = acc_b0;
+    out_p[MLP_B1 + t] = acc_b1;
+  } else {
+    out_p[MLP_W3 + t - MLP_H] = acc_a0;
+    if (t == MLP_H) {
+      out_p[MLP_B3] = acc_b0;
+      out_p[MLP_P] = acc_b1;
+    }
+  }
+}
+
+// Sum of the per-workgroup partials in workgroup order; with lr > 0 also
+// one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on P with backtracking:
+// best_p / best_loss hold the lowest-loss parameters seen (the loss in
+// `part` is that of P before this step); when P's loss is below it the step
+// scale lr_scale grows by 1.2, otherwise P returns to best_p, lr_scale
+// halves and no update is made.
+// grad_out / loss_out (optional) receive the summed gradient and loss.
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ P, float* __restrict__ m1,
+                                                  float* __restrict__ m2,
+                                                  const float* __restrict__ part, uint32_t nb,
+                                                  float lr, float bc1, float bc2,
+                                                  float* __restrict__ grad_out,
+                                                  float* __restrict__ loss_out,
+                                                  float* __restrict__ best_p,
+                                                  float* __restrict__ best_loss,
+                                                  float* __restrict__ lr_scale) {
+  __shared__ float s_loss;
+  __shared__ int s_state;  // 0 plain, 1 better (save), 2 worse (revert)
+  if (threadIdx.x == 0) {
+    float l = 0.f;
+    for (uint32_t b = 0; b < nb; ++b) l += part[size_t(b) * (MLP_P + 1) + MLP_P];
+    s_loss = l;
+    s_state = 0;
+    if (best_loss != nullptr) s_state = (l < *best_loss) ? 1 : 2;  // NaN counts as worse
+    if (loss_out) *loss_out = l;
+  }
+  __syncthreads();
+  const int state = s_state;
+  if (state == 1) {
+    for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) best_p[q] = P[q];
+    if (threadIdx.x == 0) {
+      *best_loss = s_loss;
+      if (lr_scale) *lr_scale *= 1.2f;
+    }
+    __syncthreads();
+  } else if (state == 2) {
+    for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) P[q] = best_p[q];
+    if (threadIdx.x == 0 && lr_scale) *lr_scale *= 0.5f;
+    return;
+  }
+  const float step = lr * (lr_scale ? *lr_scale : 1.f);
+  for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) {
+    float g = 0.f;
+    for (uint32_t b = 0; b < nb; ++b) g += part[size_t(b) * (MLP_P + 1) + q];
+    if (grad_out) grad_out[q] = g;
+    if (step > 0.f) {
+      const float a = 0.9f * m1[q] + 0.1f * g;
+      const float v = 0.999f * m2[q] + 0.001f * g * g;
+      m1[q] = a;
+      m2[q] = v;
+      P[q] -= step * (a / bc1) / (sqrtf(v / bc2) + 1e-8f);
+    }
+  }
+}
+
+// Tabulate the model into integer knot codes, non-decreasing.
+__global__ __launch_bounds__(1024) void k_cdf_knots(const float* __restrict__ P, float scale,
+                                                    uint32_t* __restrict__ knots) {
+  __shared__ float Ps[MLP_P];
+  __shared__ uint32_t s_k[KNOTS + 1];
+  __shared__ uint32_t s_tmp[16];
+  for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) Ps[q] = P[q];
+  __syncthreads();
+  for (int k = threadIdx.x; k <= KNOTS; k += blockDim.x) {
+    float f = mlp_eval(Ps, float(k) / float(KNOTS));
+    f = f > 0.f ? (f < 1.f ? f : 1.f) : 0.f;  // NaN -> 0
+    s_k[k] = uint32_t(f * scale + 0.5f);
+  }
+  __syncthreads();
+  // running max: per-thread chunk max, block scan of maxima, apply
+  constexpr int PER = (KNOTS + 1 + 1023) / 1024;
+  const int t = threadIdx.x;
+  uint32_t mx = 0;
+  for (int k = t * PER; k < (t + 1) * PER && k <= KNOTS; ++k) mx = s_k[k] > mx ? s_k[k] : mx;
+  // inclusive max-scan over threads (wave scan + wave carries)
+  uint32_t v = mx;
+  const int lane = lane_id();
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const uint32_t o = __shfl_up(v, off);
+    if (lane >= off) v = o > v ? o : v;
+  }
+  if (lane == WAVE - 1) s_tmp[wave_id()] = v;
+  __syncthreads();
+  uint32_t carry = 0;
+  for (int w = 0; w < wave_id(); ++w) carry = s_tmp[w] > carry ? s_tmp[w] : carry;
+  uint32_t run = __shfl_up(v, 1);
+  run = lane == 0 ? 0u : run;
+  run = run > carry ? run : carry;  // max of everything before this thread
+  for (int k = t * PER; k < (t + 1) * PER && k <= KNOTS; ++k) {
+    run = s_k[k] > run ? s_k[k] : run;
+    knots[k] = run;
+  }
+}
+
+// ------------------------------------------------------------ sort kernels
+// Histograms of the code's low and high byte.  A wave reads CH_RUN
+// consecutive 1 KiB stretches (coalesced quads, lane-contiguous); each lane
+// adds runs of equal codes at once, so nearly sorted keys (long runs of one
+// code) do not serialise on one LDS counter.
+constexpr int CH_RUN = 16;
+__global__ __launch_bounds__(256) void k_code_hist(const int32_t* __restrict__ keys, uint32_t n,
+                                                   LearnedMap m, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_kn[KNOTS + 1];
+  __shared__ uint32_t s_h[512];
+  for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) s_kn[i] = m.knots[i];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  auto flush = [&](uint32_t c, uint32_t cnt) {
+    if (cnt == 0) return;
+    atomicAdd(&s_h[c & 255u], cnt);
+    atomicAdd(&s_h[256 + ((c >> 8) & 255u)], cnt);
+  };
+  uint32_t cur = 0, cnt = 0;
+  auto add = [&](int32_t key) {
+    const uint32_t code = lcode(uint32_t(key) ^ 0x80000000u, s_kn, m);
+    if (code == cur) {
+      ++cnt;
+    } else {
+      flush(cur, cnt);
+      cur = code;
+      cnt = 1;
+    }
+  };
+  const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
+  const uint32_t nq = vec ? n / 4 : 0;
+  const uint32_t wchunk = WAVE * CH_RUN;  // quads per wave stretch
+  const uint32_t gw = blockIdx.x * (blockDim.x / WAVE) + wave_id();
+  const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
+  for (uint32_t base = gw * wchunk; base < nq; base += nw * wchunk) {
+#pragma unroll 4
+    for (int r = 0; r < CH_RUN; ++r) {
+      const uint32_t q = base + uint32_t(r * WAVE + lane_id());
+      if (q >= nq) break;
+      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
+      add(k4.x);
+      add(k4.y);
+      add(k4.z);
+      add(k4.w);
+    }
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = nq * 4 + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    add(keys[i]);
+  flush(cur, cnt);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// Exclusive scans of the two 256-bin histograms.
+__global__ __launch_bounds__(256) void k_bins2(const uint32_t* __restrict__ hist,
+                                               uint32_t* __restrict__ bins) {
+  __shared__ uint32_t s_tmp[4];
+  const int p = blockIdx.x;
+  uint32_t total;
+  const uint32_t ex = block_excl_scan<4>(hist[p * 256 + threadIdx.x], s_tmp, &total);
+  bins[p * 256 + threadIdx.x] = ex;
+}
+
+// start[b] = first position whose code >= b (b in [0, B]); start[B] = n.
+__global__ __launch_bounds__(256) void k_bucket_bounds(const uint32_t* __restrict__ W, uint32_t n,
+                                                       LearnedMap m, uint32_t B,
+                                                       uint32_t* __restrict__ start) {
+  __shared__ uint32_t s_kn[KNOTS + 1];
+  for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) s_kn[i] = m.knots[i];
+  __syncthreads();
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > B) return;
+  uint32_t lo = 0, hi = n;
+  if (b == B) {
+    lo = n;
+  } else if (b > 0) {
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (lcode(W[mid], s_kn, m) < b)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+  }
+  start[b] = lo;
+}
+
+// One workgroup per bucket: stable LSD sort in LDS of the bucket-relative
+// key w - min (9-bit digits, only as many passes as max - min needs); the
+// items carry their 16-bit position in the bucket, and the rows are
+// gathered by it at the end (6 bytes of LDS per key: three workgroups per
+// CU).  Buckets over LS_CAP are listed for the large-bucket path.  Items
+// are wave-contiguous (wave w holds positions [w * 64 * LS_IPT,
+// (w + 1) * 64 * LS_IPT)) and ranked with lane-ordered LDS atomics.
+__global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __restrict__ W,
+                                                             const uint32_t* __restrict__ I,
+                                                             const uint32_t* __restrict__ start,
+                                                             int32_t* __restrict__ perm,
+                                                             uint32_t* __restrict__ large,
+                                                             uint32_t* __restrict__ nlarge) {
+  __shared__ uint32_t s_w[LS_CAP];
+  __shared__ uint16_t s_l[LS_CAP];
+  __shared__ uint32_t s_cnt[LS_WAVES][LS_DIG];
+  __shared__ uint32_t s_dst[LS_DIG];
+  __shared__ uint32_t s_tmp[2 * LS_WAVES];
+  const uint32_t b = blockIdx.x;
+  const uint32_t lo = start[b], hi = start[b + 1];
+  const uint32_t size = hi - lo;
+  if (size == 0) return;
+  const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+  if (size > uint32_t(LS_CAP)) {
+    if (tid == 0) large[atomicAdd(nlarge, 1u)] = b;
+    return;
+  }
+  const uint32_t wb = uint32_t(wave) * (WAVE * LS_IPT);
+  uint32_t kw[LS_IPT], lp[LS_IPT];
+  uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+  for (int j = 0; j < LS_IPT; ++j) {
+    const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+    lp[j] = pos;
+    if (pos < size) {
+      kw[j] = W[lo + pos];
+      mn = kw[j] < mn ? kw[j] : mn;
+      mx = kw[j] > mx ? kw[j] : mx;
+    } else {
+      kw[j] = 0;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t a = __shfl_xor(mn, off), c = __shfl_xor(mx, off);
+    mn = a < mn ? a : mn;
+    mx = c > mx ? c : mx;
+  }
+  if (lane == 0) {
+    s_tmp[wave] = mn;
+    s_tmp[LS_WAVES + wave] = mx;
+  }
+  __syncthreads();
+  mn = 0xFFFFFFFFu;
+  mx = 0;
+#pragma unroll
+  for (int w = 0; w < LS_WAVES; ++w) {
+    mn = s_tmp[w] < mn ? s_tmp[w] : mn;
+    mx = s_tmp[LS_WAVES + w] > mx ? s_tmp[LS_WAVES + w] : mx;
+  }
+  const uint32_t range = mx - mn;
+  const int bits = range == 0 ? 0 : 32 - __clz(range);
+  __syncthreads();
+  for (int shift = 0; shift < bits; shift += LS_DBITS) {  // uniform over the workgroup
+    for (int i = tid; i < LS_WAVES * LS_DIG; i += LS_BLOCK) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    uint32_t rk[LS_IPT];
+#pragma unroll
+    for (int j = 0; j < LS_IPT; ++j) {
+      const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+      const uint32_t d = ((kw[j] - mn) >> shift) & uint32_t(LS_DIG - 1);
+      const uint32_t r = pos < size ? atomicAdd(&s_cnt[wave][d], 1u) : 0u;
+      rk[j] = (r << LS_DBITS) | d;
+    }
+    __syncthreads();
+    {
+      // thread == digit: offsets of each wave's run, digit starts
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < LS_WAVES; ++w) {
+        const uint32_t x = s_cnt[w][tid];
+        s_cnt[w][tid] = c;
+        c += x;
+      }
+      uint32_t tot;
+      s_dst[tid] = block_excl_scan<LS_WAVES>(c, s_tmp, &tot);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LS_IPT; ++j) {
+      const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+      if (pos < size) {
+        const uint32_t d = rk[j] & uint32_t(LS_DIG - 1);
+        const uint32_t p = s_dst[d] + s_cnt[wave][d] + (rk[j] >> LS_DBITS);
+        s_w[p] = kw[j];
+        s_l[p] = uint16_t(lp[j]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LS_IPT; ++j) {
+      const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+      if (pos < size) {
+        kw[j] = s_w[pos];
+        lp[j] = s_l[pos];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < LS_IPT; ++j) {
+    const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+    if (pos < size) perm[lo + pos] = int32_t(I[lo + lp[j]]);
+  }
+}
+
+// Large buckets, LS_CHUNK rows per workgroup: rows copied in their current
+// (stable) order and a flag raised when a bucket holds two different keys.
+// seg: per large bucket {lo, hi}; cstart: chunk prefix over the buckets.
+__global__ __launch_bounds__(256) void k_large_copy(const uint32_t* __restrict__ W,
+                                                    const uint32_t* __restrict__ I,
+                                                    const uint2* __restrict__ seg,
+                                                    const uint32_t* __restrict__ cstart,
+                                                    uint32_t nseg, int32_t* __restrict__ perm,
+                                                    uint32_t* __restrict__ unequal) {
+  __shared__ uint32_t s_seg, s_any;
+  const uint32_t c = blockIdx.x;
+  if (threadIdx.x == 0) {
+    uint32_t lo = 0, hi = nseg;  // last segment with cstart <= c
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (cstart[mid] <= c)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    s_seg = lo;
+    s_any = 0;
+  }
+  __syncthreads();
+  const uint32_t k = s_seg;
+  const uint2 sg = seg[k];
+  const uint32_t first = sg.x + (c - cstart[k]) * uint32_t(LS_CHUNK);
+  const uint32_t last = first + uint32_t(LS_CHUNK) < sg.y ? first + uint32_t(LS_CHUNK) : sg.y;
+  const uint32_t w0 = W[sg.x];
+  bool diff = false;
+  for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
+    perm[i] = int32_t(I[i]);
+    diff |= W[i] != w0;
+  }
+  if (__ballot(diff) != 0 && lane_id() == 0) s_any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_any) unequal[k] = 1;
+}
+
+__global__ void k_seg_keys(const uint32_t* __restrict__ W, uint32_t lo, uint32_t m,
+                           int32_t* __restrict__ out) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+    out[i] = int32_t(W[lo + i] ^ 0x80000000u);
+}
+
+// perm[lo + i] = I[lo + sp[i] - off]: sp is this segment's slice of the
+// concatenation's permutation (indices into the concatenation; the slice
+// of a segment holds exactly its own indices, offset by off).
+__global__ void k_seg_gather(const int32_t* __restrict__ sp, const uint32_t* __restrict__ I,
+                             uint32_t lo, uint32_t m, uint32_t off, int32_t* __restrict__ perm) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+    perm[lo + i] = int32_t(I[lo + (uint32_t(sp[i]) - off)]);
+}
+
+// ------------------------------------------------------------ host: sort
+static uint32_t code_bits_for(uint32_t n) {
+  int lg = 0;
+  while ((uint64_t(1) << lg) < n) ++lg;
+  return uint32_t(std::min(16, std::max(1, lg - 12)));
+}
+
+void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model, int32_t* perm,
+                         hipStream_t s, uint64_t* stats) {
+  CdfModel hm;
+  if (is_device_ptr(model)) {
+    NFH_CHECK(hipMemcpyAsync(&hm, model, sizeof(hm), hipMemcpyDeviceToHost, s));
+    NFH_CHECK(hipStreamSynchronize(s));
+  } else {
+    hm = *model;
+  }
+  if (stats)
+    for (int i = 0; i < 4; ++i) stats[i] = 0;
+  if (n < (1u << 14) || hm.span == 0) {
+    // too small to bucket, or a degenerate model: the plain sort
+    uint32_t* t0 = scratch_t<uint32_t>(S_LSEG1, n);
+    uint32_t* t1 = scratch_t<uint32_t>(S_LSEG2, n);
+    int32_t* tp = scratch_t<int32_t>(S_LSEG3, n);
+    radix_argsort<int32_t>(keys, n, false, perm, t0, t1, tp, nullptr, s);
+    return;
+  }
+  const uint32_t bits = code_bits_for(n);
+  const uint32_t B = 1u << bits;
+  // model -> knots
+  float* dP = scratch_t<float>(S_LMODEL, MLP_P + KNOTS + 8);
+  uint32_t* knots = reinterpret_cast<uint32_t*>(dP + MLP_P + 3);
+  NFH_CHECK(hipMemcpyAsync(dP, hm.p, sizeof(float) * MLP_P, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_cdf_knots, dim3(1), dim3(1024), 0, s, dP, float(B) * float(1 << KN_UNIT),
+                     knots);
+  NFH_LAUNCH_CHECK();
+  LearnedMap map;
+  map.knots = knots;
+  map.wmin = hm.wmin;
+  map.span = hm.span;
+  static_assert(KNOTS << KN_FRAC == 1 << 21, "knot position has 21 bits");
+  map.shl = uint32_t(__builtin_clz(hm.span));
+  map.mul = uint32_t((uint64_t(1) << 53) / ((uint64_t(hm.span) + 1u) << map.shl));
+  map.maxcode = B - 1;
+  // code histograms and bin starts
+  uint32_t* meta = scratch_t<uint32_t>(S_LBUCKET, 1024 + 4 + (B + 1) + 2 * 4096);
+  uint32_t* hist = meta;
+  uint32_t* bins = meta + 512;
+  uint32_t* ctr = meta + 1024;  // 2 tile counters, nlarge
+  uint32_t* bstart = ctr + 4;
+  uint32_t* large = bstart + (B + 1);
+  fill_u32(meta, 0, 1024 + 4, s);
+  {
+    ProfScope prof("learned_code_hist", s, double(n) * 4.0);
+    hipLaunchKernelGGL(k_code_hist, dim3(grid_for(n, 256, 4 * num_cus())), dim3(256), 0, s, keys,
+                       n, map, hist);
+    NFH_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_bins2, dim3(2), dim3(256), 0, s, hist, bins);
+  NFH_LAUNCH_CHECK();
+  // stable code passes: low byte (reads the raw keys), then high byte
+  uint32_t* w0 = scratch_t<uint32_t>(S_L0, n);
+  uint32_t* i0 = scratch_t<uint32_t>(S_L1, n);
+  uint32_t* w1 = scratch_t<uint32_t>(S_L2, n);
+  uint32_t* i1 = scratch_t<uint32_t>(S_L3, n);
+  const size_t tiles = (n + SortGeom<int32_t>::TILE - 1) / SortGeom<int32_t>::TILE;
+  uint64_t* lb = lookback_words(tiles * 256, s);
+  onesweep_pass<int32_t, true, false>(keys, nullptr, nullptr, w0, reinterpret_cast<int32_t*>(i0),
+                                      nullptr, bins, lb, ctr, n, LearnedDigit{map, 0}, false, s,
+                                      "learned_pass");
+  const uint32_t* W = w0;
+  const uint32_t* I = i0;
+  if (bits > 8) {
+    onesweep_pass<int32_t, false, false>(nullptr, w0, reinterpret_cast<const int32_t*>(i0), w1,
+                                         reinterpret_cast<int32_t*>(i1), nullptr, bins + 256, lb,
+                                         ctr + 1, n, LearnedDigit{map, 8}, false, s,
+                                         "learned_pass");
+    W = w1;
+    I = i1;
+  }
+  hipLaunchKernelGGL(k_bucket_bounds, dim3((B + 1 + 255) / 256), dim3(256), 0, s, W, n, map, B,
+                     bstart);
+  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("learned_bucket_sort", s, double(n) * 12.0);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(B), dim3(LS_BLOCK), 0, s, W, I, bstart, perm, large,
+                       ctr + 2);
+    NFH_LAUNCH_CHECK();
+  }
+  const uint32_t nl = read_scalar(ctr + 2, s);
+  if (stats) {
+    std::vector<uint32_t> st(B + 1);
+    NFH_CHECK(hipMemcpyAsync(st.data(), bstart, (B + 1) * 4, hipMemcpyDeviceToHost, s));
+    NFH_CHECK(hipStreamSynchronize(s));
+    uint64_t nonempty = 0, mx = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      const uint64_t sz = st[b + 1] - st[b];
+      nonempty += sz > 0;
+      mx = std::max(mx, sz);
+    }
+    stats[0] = nonempty;
+    stats[1] = mx;
+    stats[2] = nl;
+  }
+  if (nl == 0) return;
+  // large buckets: copy rows, flag the ones holding different keys
+  std::vector<uint32_t> lst(nl), st(B + 1);
+  NFH_CHECK(hipMemcpyAsync(lst.data(), large, nl * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipMemcpyAsync(st.data(), bstart, (B + 1) * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  std::sort(lst.begin(), lst.end());
+  std::vector<uint32_t> segs(2 * nl), cst(nl + 1);
+  uint32_t chunks = 0;
+  for (uint32_t k = 0; k < nl; ++k) {
+    segs[2 * k] = st[lst[k]];
+    segs[2 * k + 1] = st[lst[k] + 1];
+    cst[k] = chunks;
+    chunks += (segs[2 * k + 1] - segs[2 * k] + LS_CHUNK - 1) / LS_CHUNK;
+  }
+  cst[nl] = chunks;
+  uint32_t* dseg = scratch_t<uint32_t>(S_LSEG0, 2 * nl + (nl + 1) + nl);
+  uint32_t* dcst = dseg + 2 * nl;
+  uint32_t* dflag = dcst + (nl + 1);
+  NFH_CHECK(hipMemcpyAsync(dseg, segs.data(), 2 * nl * 4, hipMemcpyHostToDevice, s));
+  NFH_CHECK(hipMemcpyAsync(dcst, cst.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
+  fill_u32(dflag, 0, nl, s);
+  {
+    ProfScope prof("learned_large_copy", s);
+    hipLaunchKernelGGL(k_large_copy, dim3(chunks), dim3(256), 0, s, W, I,
+                       reinterpret_cast<const uint2*>(dseg), dcst, nl, perm, dflag);
+    NFH_LAUNCH_CHECK();
+  }
+  std::vector<uint32_t> flag(nl);
+  NFH_CHECK(hipMemcpyAsync(flag.data(), dflag, nl * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  // overflow fix-up: the buckets holding different keys are concatenated
+  // and sorted by one plain radix argsort.  Buckets cover disjoint,
+  // increasing key ranges and keep their rows in input order, so the
+  // stable order of the concatenation is each bucket's stable order.
+  uint64_t fixed = 0;
+  uint32_t total = 0;
+  std::vector<uint32_t> fseg;  // {lo, m, offset in the concatenation}
+  for (uint32_t k = 0; k < nl; ++k) {
+    if (!flag[k]) continue;
+    const uint32_t lo = segs[2 * k], m = segs[2 * k + 1] - lo;
+    fseg.insert(fseg.end(), {lo, m, total});
+    total += m;
+    ++fixed;
+  }
+  if (total > 0) {
+    int32_t* sk = scratch_t<int32_t>(S_LSEG1, total);
+    int32_t* sp = scratch_t<int32_t>(S_LSEG2, total);
+    uint32_t* t0 = scratch_t<uint32_t>(S_LSEG3, total);
+    uint32_t* t1 = scratch_t<uint32_t>(S_LSEG4, total);
+    int32_t* tp = scratch_t<int32_t>(S_LTRAIN, total);
+    for (size_t q = 0; q < fseg.size(); q += 3) {
+      hipLaunchKernelGGL(k_seg_keys, dim3(grid_for(fseg[q + 1], 256, 4096)), dim3(256), 0, s, W,
+                         fseg[q], fseg[q + 1], sk + fseg[q + 2]);
+      NFH_LAUNCH_CHECK();
+    }
+    radix_argsort<int32_t>(sk, total, false, sp, t0, t1, tp, nullptr, s);
+    for (size_t q = 0; q < fseg.size(); q += 3) {
+      hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(fseg[q + 1], 256, 4096)), dim3(256), 0, s,
+                         sp + fseg[q + 2], I, fseg[q], fseg[q + 1], fseg[q + 2], perm);
+      NFH_LAUNCH_CHECK();
+    }
+  }
+  if (stats) stats[3] = fixed;
+}
+
+// ------------------------------------------------------------ host: training
+__global__ void k_train_sample(const int32_t* __restrict__ keys, uint32_t n, uint32_t S,
+                               int32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S) out[i] = keys[uint32_t((uint64_t(i) * n) / S)];
+}
+
+// Training pairs from the sorted sample: x = normalised word, y = midpoint
+// of the key's empirical CDF step (equal keys share one target).
+__global__ void k_train_pairs(const int32_t* __restrict__ sorted, uint32_t S, uint32_t wmin,
+                              float inv_span, float* __restrict__ x, float* __restrict__ y) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  const int32_t k = sorted[i];
+  uint32_t lo = 0, hi = i;  // first index of k
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (sorted[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  uint32_t a = i + 1, b = S;  // one past the last index of k
+  while (a < b) {
+    const uint32_t mid = (a + b) / 2;
+    if (sorted[mid] <= k)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  const uint32_t w = uint32_t(k) ^ 0x80000000u;
+  x[i] = float(double(w - wmin) * double(inv_span));
+  y[i] = float((double(lo) + double(a)) * 0.5 / double(S));
+}
+
+// Initial network: the piecewise-linear interpolant of the sample's
+// empirical CDF through 33 knots chosen greedily (each new knot at the
+// sample point farthest, vertically, from the current interpolant), written
+// as layer 1 = ReLU ramps at the knots, layer 2 = identity, layer 3 = slope
+// changes.  Gaps between clusters get knots at their edges.
+static void init_params(const std::vector<int32_t>& sorted, uint32_t wmin, uint32_t span,
+                        float* P) {
+  for (int q = 0; q < MLP_P; ++q) P[q] = 0.f;
+  const size_t S = sorted.size();
+  std::vector<double> x(S), y(S);
+  for (size_t i = 0; i < S; ++i) {
+    x[i] = double((uint32_t(sorted[i]) ^ 0x80000000u) - wmin) / double(span);
+    y[i] = (double(i) + 0.5) / double(S);
+  }
+  std::vector<size_t> kn = {0, S - 1};
+  auto worst = [&](size_t a, size_t b, double* err) {
+    size_t arg = a;
+    *err = -1.0;
+    const double dx = x[b] - x[a];
+    for (size_t i = a + 1; i < b; ++i) {
+      const double t = dx > 0 ? (x[i] - x[a]) / dx : 0.0;
+      const double e = std::fabs(y[i] - (y[a] + t * (y[b] - y[a])));
+      if (e > *err) {
+        *err = e;
+        arg = i;
+      }
+    }
+    return arg;
+  };
+  while (int(kn.size()) < MLP_H + 1) {
+    double best = -1.0;
+    size_t bi = 0, pos = 0;
+    for (size_t s = 0; s + 1 < kn.size(); ++s) {
+      double e;
+      const size_t a = worst(kn[s], kn[s + 1], &e);
+      if (e > best) {
+        best = e;
+        bi = a;
+        pos = s + 1;
+      }
+    }
+    if (best <= 0.0) break;
+    kn.insert(kn.begin() + long(pos), bi);
+  }
+  float prev = 0.f;
+  for (int k = 0; k < MLP_H; ++k) {
+    P[MLP_W2 + k * MLP_H + k] = 1.f;  // identity second layer
+    if (k + 1 >= int(kn.size())) {
+      P[MLP_B1 + k] = -1.f;
+      continue;
+    }
+    const size_t a = kn[size_t(k)], b = kn[size_t(k) + 1];
+    P[MLP_W1 + k] = 1.f;
+    P[MLP_B1 + k] = -float(x[a]);
+    const double dx = std::max(x[b] - x[a], 1e-7);
+    const float slope = float(std::min((y[b] - y[a]) / dx, 1e6));
+    P[MLP_W3 + k] = slope - prev;
+    prev = slope;
+  }
+  P[MLP_B3] = float(y[0]);
+}
+
+float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, CdfModel* model,
+                    hipStream_t s) {
+  const uint32_t S = std::min<uint32_t>(n, LS_SAMPLE);
+  // strided sample, sorted by the plain radix argsort
+  int32_t* smp = scratch_t<int32_t>(S_LTRAIN, size_t(S) * 6 + 4096);
+  int32_t* sorted = smp + S;
+  int32_t* sperm = sorted + S;
+  uint32_t* t0 = reinterpret_cast<uint32_t*>(sperm + S);
+  uint32_t* t1 = t0 + S;
+  int32_t* tp = reinterpret_cast<int32_t*>(t1 + S);
+  hipLaunchKernelGGL(k_train_sample, dim3((S + 255) / 256), dim3(256), 0, s, keys, n, S, smp);
+  NFH_LAUNCH_CHECK();
+  radix_argsort<int32_t>(smp, S, false, sperm, t0, t1, tp, sorted, s);
+  // the plain kernel's digit histograms over the whole column
+  uint32_t* hist = scratch_t<uint32_t>(S_LBUCKET, 4 * 256);
+  radix_histograms_i32(keys, n, hist, s);
+  std::vector<int32_t> hs(S);
+  std::vector<uint32_t> hh(4 * 256);
+  NFH_CHECK(hipMemcpyAsync(hs.data(), sorted, size_t(S) * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipMemcpyAsync(hh.data(), hist, 4 * 256 * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  const uint32_t wmin = uint32_t(hs.front()) ^ 0x80000000u;
+  const uint32_t wmax = uint32_t(hs.back()) ^ 0x80000000u;
+  model->wmin = wmin;
+  model->span = wmax - wmin;
+  model->pad = 0;
+  if (model->span == 0) {
+    for (int q = 0; q < MLP_P; ++q) model->p[q] = 0.f;
+    return 0.f;
+  }
+  const float inv_span = float(1.0 / double(model->span));
+  // exact CDF points at the top-byte boundaries inside [wmin, wmax]
+  std::vector<float> hx, hy;
+  uint64_t cum = 0;
+  for (int b = 0; b < 255; ++b) {
+    cum += hh[3 * 256 + b];
+    const uint32_t wb = uint32_t(b + 1) << 24;
+    if (wb <= wmin || wb > wmax) continue;
+    hx.push_back(float(double(wb - wmin) / double(model->span)));
+    hy.push_back(float(double(cum) / double(n)));
+  }
+  const uint32_t T = S + uint32_t(hx.size());
+  float* xs = scratch_t<float>(S_LSEG0, size_t(T) * 2 + MLP_P * 5 + 64);
+  float* ys = xs + T;
+  float* P = ys + T;
+  float* m1 = P + MLP_P;
+  float* m2 = m1 + MLP_P;
+  float* best = m2 + MLP_P;
+  float* loss = best + MLP_P;  // [0] last loss, [1] best loss, [2] step scale
+  hipLaunchKernelGGL(k_train_pairs, dim3((S + 255) / 256), dim3(256), 0, s, sorted, S, wmin,
+                     inv_span, xs, ys);
+  NFH_LAUNCH_CHECK();
+  if (!hx.empty()) {
+    NFH_CHECK(hipMemcpyAsync(xs + S, hx.data(), hx.size() * 4, hipMemcpyHostToDevice, s));
+    NFH_CHECK(hipMemcpyAsync(ys + S, hy.data(), hy.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  std::vector<float> p0(MLP_P);
+  init_params(hs, wmin, model->span, p0.data());
+  NFH_CHECK(hipMemcpyAsync(P, p0.data(), MLP_P * 4, hipMemcpyHostToDevice, s));
+  NFH_CHECK(hipMemcpyAsync(best, p0.data(), MLP_P * 4, hipMemcpyHostToDevice, s));
+  fill_u32(m1, 0, 2 * MLP_P, s);
+  fill_u32(loss + 1, 0x7F800000u, 1, s);  // best loss = +inf
+  fill_u32(loss + 2, 0x3F800000u, 1, s);  // step scale = 1.0f
+  const uint32_t nb = std::min<uint32_t>(TG_GRID, (T + TG_BLOCK - 1) / TG_BLOCK);
+  float* part = scratch_t<float>(S_LSEG1, size_t(nb) * (MLP_P + 1));
+  const float lr = 1e-5f;  // initial step; backtracking adapts it
+  double b1 = 1.0, b2 = 1.0;
+  for (uint32_t it = 0; it < steps; ++it) {
+    b1 *= 0.9;
+    b2 *= 0.999;
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TG_BLOCK), 0, s, P, xs, ys, T, 1.f / float(T),
+                       part, nullptr);
+    NFH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, s, P, m1, m2, part, nb, lr,
+                       float(1.0 - b1), float(1.0 - b2), nullptr, loss, best, loss + 1,
+                       loss + 2);
+    NFH_LAUNCH_CHECK();
+  }
+  // final loss
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TG_BLOCK), 0, s, P, xs, ys, T, 1.f / float(T),
+                     part, nullptr);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, s, P, m1, m2, part, nb, 0.f, 1.f, 1.f,
+                     nullptr, loss, best, loss + 1, loss + 2);
+  NFH_LAUNCH_CHECK();
+  float hl = 0.f;
+  NFH_CHECK(hipMemcpyAsync(model->p, best, MLP_P * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipMemcpyAsync(&hl, loss + 1, 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  return hl;
+}
+
+// Gradient of the mean squared error at params P over (x, y): the same
+// kernels as training, for checking against the oracle.
+void mlp_grad_device(const float* P, const float* x, const float* y, uint32_t S, float* grad,
+                     float* loss, float* pred, hipStream_t s) {
+  const uint32_t nb = std::min<uint32_t>(TG_GRID, (S + TG_BLOCK - 1) / TG_BLOCK);
+  float* part = scratch_t<float>(S_LSEG1, size_t(nb) * (MLP_P + 1));
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TG_BLOCK), 0, s, P, x, y, S, 1.f / float(S), part,
+                     pred);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, s, nullptr, nullptr, nullptr, part, nb,
+                     0.f, 1.f, 1.f, grad, loss, nullptr, nullptr, nullptr);
+  NFH_LAUNCH_CHECK();
+}
+
+}  // namespace nfh
+
+using namespace nfh;
+
+// Additive (no reference counterpart): train the CDF model on a column.
+// Returns 0, -1 null, -3 n > i32::MAX, -6 device error.  *loss_out (may be
+// null): final mean squared CDF error on the training points.
+NF_EXPORT int32_t nfh_cdf_train_i32(const int32_t* keys, size_t n, uint32_t steps,
+                                    CdfModel* model_out, float* loss_out) {
+  if (!keys || !model_out) return NULL_OR_ABORT;
+  if (n == 0) return MALFORMED;
+  if (n > size_t(INT32_MAX)) return MALFORMED;
+  NFH_GUARD_BEGIN
+  Stage st;
+  const int32_t* k = st.in(keys, n, S_IN0);
+  const float l = cdf_train_i32(k, uint32_t(n), steps, model_out, st.s);
+  if (loss_out) *loss_out = l;
+  st.finish();
+  return OK;
+  NFH_GUARD_END(int32_t)
+}
+
+// Additive: stable ascending argsort with a trained model; the permutation
+// equals nf_sort_perm_i32(descending = 0).  stats (optional, 4 words):
+// non-empty buckets, largest bucket, buckets over the LDS capacity, buckets
+// re-sorted by the overflow fix-up.
+NF_EXPORT int32_t nfh_sort_perm_learned_i32(const int32_t* keys, size_t n,
+                                            const CdfModel* model, int32_t* perm_out,
+                                            uint64_t* stats) {
+  if (n == 0) return OK;
+  if (!keys || !model || !perm_out) return NULL_OR_ABORT;
+  if (n > size_t(INT32_MAX)) return MALFORMED;
+  NFH_GUARD_BEGIN
+  Stage st;
+  const int32_t* k = st.in(keys, n, S_IN0);
+  int32_t* p = st.out(perm_out, n, S_OUT0);
+  learned_argsort_i32(k, uint32_t(n), model, p, st.s, stats);
+  st.finish();
+  return OK;
+  NFH_GUARD_END(int32_t)
+}
+
+// Additive: mean-squared-error gradient of the CDF network (training
+// kernels) at params[1153] over (x, y); for checking against the oracle.
+NF_EXPORT int32_t nfh_mlp_grad(const float* params, const float* x, const float* y, size_t n,
+                               float* grad_out, float* loss_out, float* pred_out) {
+  if (n == 0) return OK;
+  if (!params || !x || !y || !grad_out) return NULL_OR_ABORT;
+  if (n > size_t(INT32_MAX)) return MALFORMED;
+  NFH_GUARD_BEGIN
+  Stage st;
+  const float* P = st.in(params, MLP_P, S_IN0);
+  const float* dx = st.in(x, n, S_IN1);
+  const float* dy = st.in(y, n, S_IN2);
+  float* g = st.out(grad_out, MLP_P, S_OUT0);
+  float* l = loss_out ? st.out(loss_out, 1, S_OUT1) : nullptr;
+  float* pr = pred_out ? st.out(pred_out, n, S_OUT2) : nullptr;
+  mlp_grad_device(P, dx, dy, uint32_t(n), g, l, pr, st.s);
+  st.finish();
+  return OK;
+  NFH_GUARD_END(int32_t)
+}

@@ -50,6 +50,20 @@ inline void check(hipError_t e, const char* what) {
 #define NFH_LAUNCH_CHECK() ::nfh::check(hipGetLastError(), "kernel launch")
 
 // Per-thread device context: stream, scratch pool, look-back epoch.
+// Slot ids.  Slots 0..15 hold staged user buffers, 16.. hold internal
+// scratch, so staging and scratch never alias within one call.
+enum Slot : int {
+  S_IN0 = 0, S_IN1, S_IN2, S_IN3, S_IN4, S_IN5,
+  S_OUT0 = 6, S_OUT1, S_OUT2, S_OUT3, S_OUT4, S_OUT5, S_OUT6,
+  S_T0 = 16, S_T1, S_T2, S_T3, S_T4, S_T5,
+  S_LOOKBACK = 24, S_META, S_META2, S_WORK0, S_WORK1, S_WORK2, S_WORK3, S_MISC,
+  // learned-CDF sort: code-ordered (word, row) pairs, model / knots, buckets,
+  // and the overflow segment sort
+  S_L0 = 32, S_L1, S_L2, S_L3, S_LMODEL, S_LBUCKET, S_LSEG0, S_LSEG1, S_LSEG2, S_LSEG3,
+  S_LSEG4, S_LTRAIN,
+  S_COUNT
+};
+
 struct Ctx {
   hipStream_t stream = nullptr;
   bool stream_set_by_user = false;
@@ -58,7 +72,7 @@ struct Ctx {
   void* pinned = nullptr;  // small pinned buffer for scalar readbacks
   size_t pinned_cap = 0;
   uint32_t epoch = 0;  // 30-bit look-back epoch (see sort.hip)
-  Ctx() : slot(32, nullptr), cap(32, 0) {}
+  Ctx() : slot(S_COUNT, nullptr), cap(S_COUNT, 0) {}
   ~Ctx();
 };
 
@@ -78,14 +92,6 @@ void* pinned(size_t bytes);
 
 bool is_device_ptr(const void* p);
 
-// Slot ids.  Slots 0..15 hold staged user buffers, 16.. hold internal
-// scratch, so staging and scratch never alias within one call.
-enum Slot : int {
-  S_IN0 = 0, S_IN1, S_IN2, S_IN3, S_IN4, S_IN5,
-  S_OUT0 = 6, S_OUT1, S_OUT2, S_OUT3, S_OUT4, S_OUT5, S_OUT6,
-  S_T0 = 16, S_T1, S_T2, S_T3, S_T4, S_T5,
-  S_LOOKBACK = 24, S_META, S_META2, S_WORK0, S_WORK1, S_WORK2, S_WORK3, S_MISC,
-};
 
 // Staging of one call's buffers.  in() returns a device pointer holding
 // the caller's input; out() returns a device pointer whose contents are

@@ -22,6 +22,10 @@ template <class K>
 void radix_argsort(const K* keys, size_t n, bool desc, int32_t* perm, word_t<K>* t0,
                    word_t<K>* t1, int32_t* tp, K* sorted_out, hipStream_t s);
 
+// The digit histograms the radix argsort's first kernel computes (4 x 256
+// bins of the ascending key word, bin p*256+d = keys whose byte p is d).
+void radix_histograms_i32(const int32_t* keys, uint32_t n, uint32_t* hist, hipStream_t s);
+
 // Look-back scratch of `words` 64-bit words (zeroed whenever it grows) and
 // a fresh epoch for one decoupled look-back pass.
 uint64_t* lookback_words(size_t words, hipStream_t s);

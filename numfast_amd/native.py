@@ -74,6 +74,27 @@ GROUPS = {
         ("nf_sort_perm_i32", [_vp, _sz, _u8] + [_vp] * 4, _i32),
         ("nf_sort_perm_i64", [_vp, _sz, _u8] + [_vp] * 4, _i32),
     ],
+    # native_cpu.py:164-200 (_req_select).  The reference declares the
+    # scatter restype as c_int32 although the symbols return i64
+    # (lib.rs:654-659); the binding keeps that quirk (exact for m < 2**31).
+    "select": [
+        ("nf_select_count", [_vp, _sz], _i64),
+        ("nf_select_scatter_i32", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_select_scatter_f32", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_select_scatter_f64", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_select_scatter_i64", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_select_scatter_u8", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_mask_and", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_mask_or", [_vp, _vp, _sz, _vp], _i32),
+        ("nf_mask_not", [_vp, _sz, _vp], _i32),
+    ],
+    # Additive: learned-CDF sort, its training, the sort pipeline.
+    "learned": [
+        ("nfh_cdf_train_i32", [_vp, _sz, _u32, _vp, _vp], _i32),
+        ("nfh_sort_perm_learned_i32", [_vp, _sz, _vp, _vp, _vp], _i32),
+        ("nfh_mlp_grad", [_vp, _vp, _vp, _sz, _vp, _vp, _vp], _i32),
+        ("nfh_filter_derive_sort_limit_i32", [_vp, _sz, _i32, _sz, _vp, _vp, _vp], _i64),
+    ],
     # Additive surface of this library (no reference counterpart).
     "hip": [
         ("nfh_set_stream", [_vp], _i32),
@@ -565,3 +586,113 @@ def unique_valid(keys, valid=None):
     if ng < 0:
         raise RuntimeError("nfh_unique_valid_i32 rc=%d" % ng)
     return uniq[:ng], inv
+
+
+# ------------------------------------------------------------ select / mask
+_SCATTER = {np.dtype(np.int32): "nf_select_scatter_i32", np.dtype(np.int64): "nf_select_scatter_i64",
+            np.dtype(np.float32): "nf_select_scatter_f32",
+            np.dtype(np.float64): "nf_select_scatter_f64", np.dtype(np.uint8): "nf_select_scatter_u8"}
+
+
+def select_count(mask):
+    """native_cpu.py select lane -> nf_select_count: rows with mask != 0."""
+    m = _c(mask, np.uint8)
+    r = int(lib("select").nf_select_count(_p(m), m.size))
+    if r < 0:
+        raise RuntimeError("nf_select_count rc=%d" % r)
+    return r
+
+
+def select_scatter(src, mask):
+    """Order-preserving compaction src[mask != 0] (nf_select_scatter_*)."""
+    src = np.ascontiguousarray(src)
+    m = _c(mask, np.uint8)
+    fn = getattr(lib("select"), _SCATTER[src.dtype])
+    out = np.zeros(src.size, src.dtype)
+    r = int(fn(_p(src), _p(m), src.size, _p(out)))
+    if r < 0:
+        raise RuntimeError("select scatter rc=%d" % r)
+    return out[:r]
+
+
+def mask_op(op, a, b=None):
+    """nf_mask_and / nf_mask_or / nf_mask_not; output normalised 0/1."""
+    a = _c(a, np.uint8)
+    out = np.zeros(a.size, np.uint8)
+    L = lib("select")
+    if op == "not":
+        rc = L.nf_mask_not(_p(a), a.size, _p(out))
+    else:
+        b = _c(b, np.uint8)
+        rc = getattr(L, "nf_mask_" + op)(_p(a), _p(b), a.size, _p(out))
+    if rc != 0:
+        raise RuntimeError("nf_mask_%s rc=%d" % (op, rc))
+    return out
+
+
+# ------------------------------------------------------------ learned sort
+MLP_P = 1153
+
+
+class CdfModel(ctypes.Structure):
+    """struct nfh_cdf_model (include/numfast_hip.h)."""
+    _fields_ = [("p", ctypes.c_float * MLP_P), ("wmin", ctypes.c_uint32),
+                ("span", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+    def params(self):
+        return np.ctypeslib.as_array(self.p).copy()
+
+
+def cdf_train(keys, steps=100):
+    """Train the CDF model on a column (host array or device tensor).
+    Returns (model, final loss)."""
+    L = lib("learned")
+    n = keys.numel() if hasattr(keys, "numel") else np.asarray(keys).size
+    k = keys if hasattr(keys, "data_ptr") else _c(keys, np.int32)
+    model = CdfModel()
+    loss = ctypes.c_float()
+    rc = L.nfh_cdf_train_i32(_p(k), n, steps, ctypes.byref(model), ctypes.byref(loss))
+    if rc != 0:
+        raise RuntimeError("nfh_cdf_train_i32 rc=%d" % rc)
+    return model, float(loss.value)
+
+
+def sort_perm_learned(keys, model):
+    """Stable ascending argsort via the learned-CDF bucket sort; returns
+    (perm, stats) with stats = (buckets, largest, over capacity, fixed)."""
+    keys = _c(keys, np.int32)
+    perm = np.zeros(keys.size, np.int32)
+    stats = np.zeros(4, np.uint64)
+    rc = lib("learned").nfh_sort_perm_learned_i32(_p(keys), keys.size, ctypes.byref(model),
+                                                  _p(perm), _p(stats))
+    if rc != 0:
+        raise RuntimeError("nfh_sort_perm_learned_i32 rc=%d" % rc)
+    return perm, stats
+
+
+def mlp_grad(params, x, y):
+    """(grad[1153], loss, pred) of the CDF network's mean squared error,
+    from the training kernels."""
+    P = _c(params, np.float32)
+    x, y = _c(x, np.float32), _c(y, np.float32)
+    g = np.zeros(MLP_P, np.float32)
+    loss = np.zeros(1, np.float32)
+    pred = np.zeros(x.size, np.float32)
+    rc = lib("learned").nfh_mlp_grad(_p(P), _p(x), _p(y), x.size, _p(g), _p(loss), _p(pred))
+    if rc != 0:
+        raise RuntimeError("nfh_mlp_grad rc=%d" % rc)
+    return g, float(loss[0]), pred
+
+
+def filter_derive_sort_limit(v, limit=1000, mul=3, model=None):
+    """filter(v > 0) -> derive(w = v * mul, int32 wrap) -> sort(w) -> limit:
+    (w, source rows) of the first min(limit, m) rows."""
+    v = _c(v, np.int32)
+    w = np.zeros(limit, np.int32)
+    rows = np.zeros(limit, np.int32)
+    r = int(lib("learned").nfh_filter_derive_sort_limit_i32(
+        _p(v), v.size, mul, limit, None if model is None else ctypes.byref(model), _p(w),
+        _p(rows)))
+    if r < 0:
+        raise RuntimeError("nfh_filter_derive_sort_limit_i32 rc=%d" % r)
+    return w[:r], rows[:r]

@@ -569,3 +569,48 @@ EXPORT int32_t nf_pattern_encode(const uint8_t *data, size_t total, const int32_
     if (offs[n] < 0 || (size_t)offs[n] > total) return NF_MALFORMED;
     return NF_OK;
 }
+
+/* ---- select / mask: numfast-native/src/series/select.rs:13-67 behind
+ * lib.rs:616-1062.  Count and scatter return m, -2 on a null pointer; the
+ * mask ops return 0 / -1. */
+EXPORT int64_t nf_select_count(const uint8_t *mask, size_t n) {
+    if (n == 0) return 0;
+    if (!mask) return NF_BAD_RANGE;
+    int64_t m = 0;
+    for (size_t i = 0; i < n; i++) m += mask[i] != 0;
+    return m;
+}
+
+#define SELECT_SCATTER(name, T)                                                   \
+    EXPORT int64_t name(const T *src, const uint8_t *mask, size_t n, T *out) {    \
+        if (n == 0) return 0;                                                     \
+        if (!src || !mask || !out) return NF_BAD_RANGE;                           \
+        size_t m = 0;                                                             \
+        for (size_t i = 0; i < n; i++)                                            \
+            if (mask[i] != 0) out[m++] = src[i];                                  \
+        return (int64_t)m;                                                        \
+    }
+SELECT_SCATTER(nf_select_scatter_i32, int32_t)
+SELECT_SCATTER(nf_select_scatter_i64, int64_t)
+SELECT_SCATTER(nf_select_scatter_f32, float)
+SELECT_SCATTER(nf_select_scatter_f64, double)
+SELECT_SCATTER(nf_select_scatter_u8, uint8_t)
+
+EXPORT int32_t nf_mask_and(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
+    if (n == 0) return NF_OK;
+    if (!a || !b || !out) return NF_NULL_OR_ABORT;
+    for (size_t i = 0; i < n; i++) out[i] = (uint8_t)((a[i] != 0) & (b[i] != 0));
+    return NF_OK;
+}
+EXPORT int32_t nf_mask_or(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
+    if (n == 0) return NF_OK;
+    if (!a || !b || !out) return NF_NULL_OR_ABORT;
+    for (size_t i = 0; i < n; i++) out[i] = (uint8_t)((a[i] != 0) | (b[i] != 0));
+    return NF_OK;
+}
+EXPORT int32_t nf_mask_not(const uint8_t *a, size_t n, uint8_t *out) {
+    if (n == 0) return NF_OK;
+    if (!a || !out) return NF_NULL_OR_ABORT;
+    for (size_t i = 0; i < n; i++) out[i] = (uint8_t)(a[i] == 0);
+    return NF_OK;
+}

@@ -26,6 +26,9 @@ def hip():
     """The product library, loaded through the same binding as NumFast's
     loader.  Only GPU tests request this fixture."""
     os.environ.pop("NUMFAST_NATIVE_DISABLE", None)
+    # torch first: loading the library before torch initialises the HIP
+    # runtime in a way that leaves torch without a visible GPU
+    import torch  # noqa: F401
     from numfast_amd import native
     lib = native.lib()
     if not lib.nfh_device_ok():

@@ -137,10 +137,23 @@ UNIQUE_CASES = {"src": "tests/fast/test_ops_m7b_native_lanes.py:209-293",
                 "gate": 100000, "cards": [2, 8, 1000, 100000],
                 "hash_mult": 2654435761}
 
+# tests/fast/test_ops_filter.py: filter keeps order; empty / all-true /
+# all-false masks; and / or / not of v > 1, v < 4 over [1, 2, 3, 4].
+FILTER = [
+    {"src": "tests/fast/test_ops_filter.py:40-48", "vals": [10, 20, 30, 40],
+     "mask": [0, 0, 1, 1], "out": [30, 40]},
+    {"src": "tests/fast/test_ops_filter.py:51-56", "vals": [1, 2, 3], "mask": [1, 1, 1],
+     "out": [1, 2, 3]},
+    {"src": "tests/fast/test_ops_filter.py:51-58", "vals": [1, 2, 3], "mask": [0, 0, 0],
+     "out": []},
+]
+MASKS = {"src": "tests/fast/test_ops_filter.py:95-108", "hi": [0, 1, 1, 1], "lo": [1, 1, 1, 0],
+         "and": [0, 1, 1, 0], "or": [1, 1, 1, 1], "not_hi": [1, 0, 0, 0]}
+
 
 def main():
     out = {"sort": SORT, "topk": TOPK, "group": GROUP, "abi": ABI, "mixed": MIXED,
-           "pattern": PATTERN, "unique": UNIQUE_CASES}
+           "pattern": PATTERN, "unique": UNIQUE_CASES, "filter": FILTER, "masks": MASKS}
     p = Path(__file__).with_name("reference_vectors.json")
     p.write_text(json.dumps(out, indent=1) + "\n")
     print("wrote", p)

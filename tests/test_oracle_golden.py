@@ -302,3 +302,75 @@ def test_pipeline_restatement_closed_form():
     # v*3 with int32 wrap: 2^30*3 = 3*2^30 - 2^32 = -2^30
     assert w.tolist() == [-(2 ** 30), 9, 9, 15]
     assert rows.tolist() == [4, 2, 6, 0]
+
+
+# ------------------------------------------------------------------ select / mask
+def _c_select(lib, vals, mask, dt):
+    v = np.ascontiguousarray(vals, dt)
+    m = np.ascontiguousarray(mask, np.uint8)
+    out = np.zeros(max(v.size, 1), dt)
+    fn = {np.int32: lib.nf_select_scatter_i32, np.int64: lib.nf_select_scatter_i64,
+          np.float32: lib.nf_select_scatter_f32, np.float64: lib.nf_select_scatter_f64,
+          np.uint8: lib.nf_select_scatter_u8}[dt]
+    r = fn(v.ctypes.data, m.ctypes.data, v.size, out.ctypes.data)
+    return out[:r]
+
+
+@pytest.mark.parametrize("case", GOLD["filter"], ids=lambda c: c["src"])
+def test_c_select_golden(c_oracle, case):
+    got = _c_select(c_oracle, case["vals"], case["mask"], np.int32)
+    assert got.tolist() == case["out"]
+    m = np.array(case["mask"], np.uint8)
+    assert c_oracle.nf_select_count(m.ctypes.data, m.size) == len(case["out"])
+
+
+def test_c_masks_golden(c_oracle):
+    g = GOLD["masks"]
+    hi, lo = np.array(g["hi"], np.uint8), np.array(g["lo"], np.uint8)
+    out = np.zeros(4, np.uint8)
+    assert c_oracle.nf_mask_and(hi.ctypes.data, lo.ctypes.data, 4, out.ctypes.data) == 0
+    assert out.tolist() == g["and"]
+    c_oracle.nf_mask_or(hi.ctypes.data, lo.ctypes.data, 4, out.ctypes.data)
+    assert out.tolist() == g["or"]
+    c_oracle.nf_mask_not(hi.ctypes.data, 4, out.ctypes.data)
+    assert out.tolist() == g["not_hi"]
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.float32, np.float64, np.uint8])
+def test_c_select_vs_boolean_index(c_oracle, dt):
+    """tests/fast/test_ops_filter.py:155-186: select_scatter == v[m]."""
+    rng = np.random.default_rng(42)
+    v = rng.integers(0, 100, size=1000).astype(dt)
+    m = rng.integers(0, 3, size=1000).astype(np.uint8)  # any non-zero byte keeps
+    assert np.array_equal(_c_select(c_oracle, v, m, dt), v[m != 0])
+
+
+# ------------------------------------------------------------------ CDF network
+def test_mlp_oracle_gradient_vs_finite_differences():
+    from oracle import mlp
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.5, mlp.P)
+    x = rng.random(300)
+    y = np.sort(rng.random(300))
+    g = mlp.grad(p, x, y)
+    for idx in list(range(0, mlp.P, 37)) + [mlp.B3, mlp.W3 + 5, mlp.W1 + 31]:
+        fd = mlp.finite_diff(p, x, y, idx)
+        assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), idx
+
+
+def test_mlp_fd_check_rejects_missing_relu_mask():
+    """Planted mistake: backward without the layer-2 ReLU mask."""
+    from oracle import mlp
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.5, mlp.P)
+    x, y = rng.random(300), np.sort(rng.random(300))
+    w1, b1, w2, b2, w3, b3 = mlp.unpack(p)
+    pre1 = x[:, None] * w1 + b1
+    h1 = np.maximum(pre1, 0)
+    a2 = h1 @ w2.T + b2
+    out = np.maximum(a2, 0) @ w3 + b3
+    dout = 2 * (out - y) / x.size
+    bad_b2 = (dout[:, None] * w3).sum(0)  # mask dropped
+    fd = np.array([mlp.finite_diff(p, x, y, mlp.B2 + i) for i in range(32)])
+    assert np.allclose(mlp.grad(p, x, y)[mlp.B2:mlp.W3], fd, rtol=1e-5, atol=1e-9)
+    assert not np.allclose(bad_b2, fd, rtol=1e-5, atol=1e-9)
