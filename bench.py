@@ -172,7 +172,7 @@ def sort_pipeline_leg(torch, L, W, native, n, steps):
         model = native.CdfModel()
         loss = ctypes.c_float()
         t = time.perf_counter()
-        rc = L.nfh_cdf_train_i32(k.data_ptr(), n, 100, ctypes.byref(model), ctypes.byref(loss))
+        rc = L.nfh_cdf_train_i32(k.data_ptr(), n, 100, 0, ctypes.byref(model), ctypes.byref(loss))
         train_s = time.perf_counter() - t
         if rc != 0:
             raise RuntimeError("train rc %d" % rc)
@@ -197,7 +197,7 @@ def sort_pipeline_leg(torch, L, W, native, n, steps):
         # pipeline: model for the derived column w = v * 3 (int32 wrap)
         wcol = (k[k > 0].to(torch.int64) * 3).to(torch.int32)
         pmodel = native.CdfModel()
-        if L.nfh_cdf_train_i32(wcol.data_ptr(), wcol.numel(), 100, ctypes.byref(pmodel),
+        if L.nfh_cdf_train_i32(wcol.data_ptr(), wcol.numel(), 100, 0, ctypes.byref(pmodel),
                                None) != 0:
             raise RuntimeError("train rc")
         del wcol
@@ -223,6 +223,35 @@ def sort_pipeline_leg(torch, L, W, native, n, steps):
             "pipeline_speedup": round(tpp / tpl, 3),
             "pipeline_identical": bool(torch.equal(w0, w1) and torch.equal(r0, r1))}
         del k
+    return out
+
+
+def batch_leg(torch, W, native, n, steps):
+    """4096 independent sum/count group-bys over n rows in total (config
+    shape: row counts log-uniform in [1e3, 1e7] rescaled, cardinality
+    log-uniform in [10, 1e6], Zipf s in [0, 1.5]): one hipGraph replay of
+    the persistent batch vs one launch set per query.  Algorithmic bytes:
+    sum_q n_q * 8 + G_q * 24 (G_q present groups)."""
+    keys, vals, row_off, card, scale = W.batch_config(nq=4096, total=n)
+    N = keys.numel()
+    t = time.perf_counter()
+    b = native.GroupBatch(torch, keys, vals, row_off, card)
+    plan_s = time.perf_counter() - t
+    ng0, uk0, ct0, sm0, go0 = b.run(0)
+    ref = [x.clone() for x in (uk0, ct0, sm0, go0)]
+    ng1, uk1, ct1, sm1, go1 = b.run(1)
+    same = (ng0 == ng1 and all(torch.equal(a, c) for a, c in zip(ref, (uk1, ct1, sm1, go1))))
+    tg = timed(torch, lambda: b.run(0), steps, 1)
+    tq = timed(torch, lambda: b.run(1), max(2, steps // 2), 1)
+    alg = 8.0 * N + 24.0 * ng0
+    out = {"queries": 4096, "rows": N, "row_scale": scale, "groups": ng0,
+           "graph_ms": round(tg * 1e3, 3), "per_query_ms": round(tq * 1e3, 3),
+           "speedup": round(tq / tg, 3), "identical": bool(same),
+           "graph_rows_per_s": N / tg, "alg_GBps": round(alg / tg / 1e9, 1),
+           "frac_of_peak": round(alg / tg / 1e9 / HBM_PEAK_GBS, 4),
+           "plan_ms": round(plan_s * 1e3, 1), "autotuned": b.info()}
+    b.close()
+    del keys, vals
     return out
 
 
@@ -317,6 +346,9 @@ def main():
         # filter(v>0) -> derive(v*3) -> sort -> limit(1000) pipeline
         extra["sort_pipeline_2^28"] = sort_pipeline_leg(torch, L, W, native, n,
                                                         max(3, args.steps // 2))
+        # batched group-by: 4096 queries, one graph-captured persistent launch
+        # vs the same kernels launched query by query
+        extra["batched_groupby_4096q"] = batch_leg(torch, W, native, n, max(3, args.steps // 2))
         # uniform-2^24 group-by variant
         keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")
         tu = timed(torch, step_on(keys, vals, valid, kvalid), max(3, args.steps // 2), 1)

@@ -163,9 +163,11 @@ typedef struct nfh_cdf_model {
 
 /* Train a model on a column (2^16-key sample sorted by the radix argsort,
  * plus the exact CDF at top-byte boundaries from the radix histogram
- * kernel; Adam on the device for `steps` full-batch steps).  *loss_out
- * (may be NULL): final mean squared CDF error.  0, -1, -3 or -6. */
-int32_t nfh_cdf_train_i32(const int32_t *keys, size_t n, uint32_t steps,
+ * kernel): the network starts from a greedy 32-piece fit of the sample CDF
+ * (flags bit 0: from the linear CDF) and takes `steps` full-batch gradient
+ * steps on the device (normalised, with backtracking).  *loss_out (may be
+ * NULL): final mean squared CDF error.  0, -1, -3 or -6. */
+int32_t nfh_cdf_train_i32(const int32_t *keys, size_t n, uint32_t steps, uint32_t flags,
                           nfh_cdf_model *model_out, float *loss_out);
 /* Stable ascending argsort through the model's bucket code; the result
  * equals nf_sort_perm_i32(keys, n, 0, ...) for any model.  stats (may be
@@ -187,6 +189,25 @@ int32_t nfh_mlp_grad(const float *params, const float *x, const float *y, size_t
 int64_t nfh_filter_derive_sort_limit_i32(const int32_t *v, size_t n, int32_t mul, size_t limit,
                                          const nfh_cdf_model *model, int32_t *w_out,
                                          int32_t *rows_out);
+
+/* Batched group-by (no reference counterpart): nq independent dense
+ * sum / count queries over concatenated device rows; query q owns rows
+ * [row_off[q], row_off[q+1]) (< 2^24 rows) and keys [0, card[q]).  Output
+ * (device, cap lanes): present groups query by query, keys ascending, as
+ * ukeys / counts / sums (int64), and group_off[nq + 1].  The plan autotunes
+ * rows per work item and the persistent grid on these buffers and captures
+ * the batched run in a hipGraph; it keeps the pointers (the buffers must
+ * outlive it).  NULL on failure with *rc_out = -1, -3 or -6. */
+void *nfh_group_batch_plan(const int32_t *keys, const int32_t *vals, const int64_t *row_off,
+                           const int32_t *card, size_t nq, int64_t *ukeys, int64_t *counts,
+                           int64_t *sums, int64_t *group_off, size_t cap, int32_t *rc_out);
+/* mode 0: replay the captured graph (one persistent launch of the batch);
+ * mode 1: the plain form, kernels launched query by query.  Returns the
+ * group count, -2 (a key outside its query's range) or -7 (cap too small). */
+int64_t nfh_group_batch_run(void *plan, int32_t mode);
+/* rows per work item, workgroups, work items, key slots. */
+int32_t nfh_group_batch_info(void *plan, uint64_t *out4);
+void nfh_group_batch_free(void *plan);
 
 #ifdef __cplusplus
 }

@@ -9,13 +9,13 @@
 // exact CDF at every top-byte boundary from that histogram.  The network
 // starts as a 32-piece piecewise-linear interpolant of the sample CDF
 // (layer 1: ReLU ramps at greedily placed knots of the empirical CDF,
-// layer 2: identity, layer 3: slope changes) and is refined by full-batch
-// Adam on the device: k_mlp_grad (forward + backward, one sample per
-// thread, per-workgroup partial gradients) and k_mlp_adam (fixed-order
-// reduction + update) with backtracking: a step that raises the training
-// loss is undone and the step size halved, so the result is never worse
-// than the initial fit.  Nothing
-// about correctness depends on the fit.
+// layer 2: identity, layer 3: slope changes) and is refined on the device
+// by full-batch gradient descent: k_mlp_grad (forward + backward, one
+// sample per thread, per-workgroup partial gradients) and k_mlp_step
+// (fixed-order reduction, normalised step with backtracking: a step that
+// raises the training loss is undone and the step length halved, so the
+// result is never worse than the initial fit).  Nothing about correctness
+// depends on the fit.
 //
 // Sort (learned_argsort_i32):
 //   1. the model is tabulated at 2049 knots into integer codes scaled to
@@ -228,22 +228,23 @@ __global__ __launch_bounds__(TG_BLOCK) void k_mlp_grad(const float* __restrict__
   }
 }
 
-// Sum of the per-workgroup partials in workgroup order; with lr > 0 also
-// one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on P with backtracking:
-// best_p / best_loss hold the lowest-loss parameters seen (the loss in
-// `part` is that of P before this step); when P's loss is below it the step
-// scale lr_scale grows by 1.2, otherwise P returns to best_p, lr_scale
-// halves and no update is made.
-// grad_out / loss_out (optional) receive the summed gradient and loss.
-__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ P, float* __restrict__ m1,
-                                                  float* __restrict__ m2,
+// Sum of the per-workgroup partials in workgroup order (gradient and
+// loss).  With step > 0 also one training step: normalised gradient
+// descent, P -= step_len * g / |g|, with backtracking.  best_p / best_loss
+// hold the lowest-loss parameters seen (the loss in `part` is that of P
+// before this step); when P's loss is below it the step length grows by
+// 1.2, otherwise P returns to best_p, the step length halves and no update
+// is made.  grad_out / loss_out (optional) receive the summed gradient and
+// loss.
+__global__ __launch_bounds__(256) void k_mlp_step(float* __restrict__ P,
                                                   const float* __restrict__ part, uint32_t nb,
-                                                  float lr, float bc1, float bc2,
-                                                  float* __restrict__ grad_out,
+                                                  float step, float* __restrict__ grad_out,
                                                   float* __restrict__ loss_out,
                                                   float* __restrict__ best_p,
                                                   float* __restrict__ best_loss,
-                                                  float* __restrict__ lr_scale) {
+                                                  float* __restrict__ step_scale) {
+  __shared__ float s_g[MLP_P];
+  __shared__ float s_red[4];
   __shared__ float s_loss;
   __shared__ int s_state;  // 0 plain, 1 better (save), 2 worse (revert)
   if (threadIdx.x == 0) {
@@ -254,33 +255,35 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ P, float* 
     if (best_loss != nullptr) s_state = (l < *best_loss) ? 1 : 2;  // NaN counts as worse
     if (loss_out) *loss_out = l;
   }
+  float sq = 0.f;
+  for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) {
+    float g = 0.f;
+    for (uint32_t b = 0; b < nb; ++b) g += part[size_t(b) * (MLP_P + 1) + q];
+    s_g[q] = g;
+    sq = fmaf(g, g, sq);
+    if (grad_out) grad_out[q] = g;
+  }
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+  if (lane_id() == 0) s_red[wave_id()] = sq;
   __syncthreads();
   const int state = s_state;
   if (state == 1) {
     for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) best_p[q] = P[q];
     if (threadIdx.x == 0) {
       *best_loss = s_loss;
-      if (lr_scale) *lr_scale *= 1.2f;
+      if (step_scale) *step_scale *= 1.2f;
     }
-    __syncthreads();
   } else if (state == 2) {
     for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) P[q] = best_p[q];
-    if (threadIdx.x == 0 && lr_scale) *lr_scale *= 0.5f;
+    if (threadIdx.x == 0 && step_scale) *step_scale *= 0.5f;
     return;
   }
-  const float step = lr * (lr_scale ? *lr_scale : 1.f);
-  for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) {
-    float g = 0.f;
-    for (uint32_t b = 0; b < nb; ++b) g += part[size_t(b) * (MLP_P + 1) + q];
-    if (grad_out) grad_out[q] = g;
-    if (step > 0.f) {
-      const float a = 0.9f * m1[q] + 0.1f * g;
-      const float v = 0.999f * m2[q] + 0.001f * g * g;
-      m1[q] = a;
-      m2[q] = v;
-      P[q] -= step * (a / bc1) / (sqrtf(v / bc2) + 1e-8f);
-    }
-  }
+  if (!(step > 0.f)) return;
+  __syncthreads();
+  const float norm = sqrtf(s_red[0] + s_red[1] + s_red[2] + s_red[3]);
+  if (!(norm > 0.f)) return;
+  const float len = step * (step_scale ? *step_scale : 1.f) / norm;
+  for (int q = threadIdx.x; q < MLP_P; q += blockDim.x) P[q] -= len * s_g[q];
 }
 
 // Tabulate the model into integer knot codes, non-decreasing.
@@ -791,6 +794,19 @@ __global__ void k_train_pairs(const int32_t* __restrict__ sorted, uint32_t S, ui
 // sample point farthest, vertically, from the current interpolant), written
 // as layer 1 = ReLU ramps at the knots, layer 2 = identity, layer 3 = slope
 // changes.  Gaps between clusters get knots at their edges.
+// The alternative start (training flag 1): 32 ramps at uniformly spaced
+// knots with only the first one weighted, i.e. the linear CDF f(x) = x;
+// training then has to place every slope change itself.
+static void init_uniform(float* P) {
+  for (int q = 0; q < MLP_P; ++q) P[q] = 0.f;
+  for (int k = 0; k < MLP_H; ++k) {
+    P[MLP_W1 + k] = 1.f;
+    P[MLP_B1 + k] = -float(k) / MLP_H;
+    P[MLP_W2 + k * MLP_H + k] = 1.f;
+  }
+  P[MLP_W3] = 1.f;
+}
+
 static void init_params(const std::vector<int32_t>& sorted, uint32_t wmin, uint32_t span,
                         float* P) {
   for (int q = 0; q < MLP_P; ++q) P[q] = 0.f;
@@ -848,8 +864,8 @@ static void init_params(const std::vector<int32_t>& sorted, uint32_t wmin, uint3
   P[MLP_B3] = float(y[0]);
 }
 
-float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, CdfModel* model,
-                    hipStream_t s) {
+float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, uint32_t flags,
+                    CdfModel* model, hipStream_t s) {
   const uint32_t S = std::min<uint32_t>(n, LS_SAMPLE);
   // strided sample, sorted by the plain radix argsort
   int32_t* smp = scratch_t<int32_t>(S_LTRAIN, size_t(S) * 6 + 4096);
@@ -893,9 +909,7 @@ float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, CdfModel* m
   float* xs = scratch_t<float>(S_LSEG0, size_t(T) * 2 + MLP_P * 5 + 64);
   float* ys = xs + T;
   float* P = ys + T;
-  float* m1 = P + MLP_P;
-  float* m2 = m1 + MLP_P;
-  float* best = m2 + MLP_P;
+  float* best = P + MLP_P;
   float* loss = best + MLP_P;  // [0] last loss, [1] best loss, [2] step scale
   hipLaunchKernelGGL(k_train_pairs, dim3((S + 255) / 256), dim3(256), 0, s, sorted, S, wmin,
                      inv_span, xs, ys);
@@ -905,34 +919,26 @@ float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, CdfModel* m
     NFH_CHECK(hipMemcpyAsync(ys + S, hy.data(), hy.size() * 4, hipMemcpyHostToDevice, s));
   }
   std::vector<float> p0(MLP_P);
-  init_params(hs, wmin, model->span, p0.data());
+  if (flags & 1u)
+    init_uniform(p0.data());
+  else
+    init_params(hs, wmin, model->span, p0.data());
   NFH_CHECK(hipMemcpyAsync(P, p0.data(), MLP_P * 4, hipMemcpyHostToDevice, s));
   NFH_CHECK(hipMemcpyAsync(best, p0.data(), MLP_P * 4, hipMemcpyHostToDevice, s));
-  fill_u32(m1, 0, 2 * MLP_P, s);
   fill_u32(loss + 1, 0x7F800000u, 1, s);  // best loss = +inf
   fill_u32(loss + 2, 0x3F800000u, 1, s);  // step scale = 1.0f
   const uint32_t nb = std::min<uint32_t>(TG_GRID, (T + TG_BLOCK - 1) / TG_BLOCK);
   float* part = scratch_t<float>(S_LSEG1, size_t(nb) * (MLP_P + 1));
-  const float lr = 1e-5f;  // initial step; backtracking adapts it
-  double b1 = 1.0, b2 = 1.0;
-  for (uint32_t it = 0; it < steps; ++it) {
-    b1 *= 0.9;
-    b2 *= 0.999;
+  const float step = 1e-3f;  // initial step length; backtracking adapts it
+  for (uint32_t it = 0; it <= steps; ++it) {
+    // the last round only evaluates (and keeps the best parameters)
     hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TG_BLOCK), 0, s, P, xs, ys, T, 1.f / float(T),
                        part, nullptr);
     NFH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, s, P, m1, m2, part, nb, lr,
-                       float(1.0 - b1), float(1.0 - b2), nullptr, loss, best, loss + 1,
-                       loss + 2);
+    hipLaunchKernelGGL(k_mlp_step, dim3(1), dim3(256), 0, s, P, part, nb,
+                       it < steps ? step : 0.f, nullptr, loss, best, loss + 1, loss + 2);
     NFH_LAUNCH_CHECK();
   }
-  // final loss
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TG_BLOCK), 0, s, P, xs, ys, T, 1.f / float(T),
-                     part, nullptr);
-  NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, s, P, m1, m2, part, nb, 0.f, 1.f, 1.f,
-                     nullptr, loss, best, loss + 1, loss + 2);
-  NFH_LAUNCH_CHECK();
   float hl = 0.f;
   NFH_CHECK(hipMemcpyAsync(model->p, best, MLP_P * 4, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipMemcpyAsync(&hl, loss + 1, 4, hipMemcpyDeviceToHost, s));
@@ -949,8 +955,8 @@ void mlp_grad_device(const float* P, const float* x, const float* y, uint32_t S,
   hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TG_BLOCK), 0, s, P, x, y, S, 1.f / float(S), part,
                      pred);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, s, nullptr, nullptr, nullptr, part, nb,
-                     0.f, 1.f, 1.f, grad, loss, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_mlp_step, dim3(1), dim3(256), 0, s, nullptr, part, nb, 0.f, grad, loss,
+                     nullptr, nullptr, nullptr);
   NFH_LAUNCH_CHECK();
 }
 
@@ -959,17 +965,19 @@ void mlp_grad_device(const float* P, const float* x, const float* y, uint32_t S,
 using namespace nfh;
 
 // Additive (no reference counterpart): train the CDF model on a column.
-// Returns 0, -1 null, -3 n > i32::MAX, -6 device error.  *loss_out (may be
-// null): final mean squared CDF error on the training points.
+// flags bit 0: start from the linear CDF instead of the greedy
+// piecewise-linear fit.  Returns 0, -1 null, -3 n > i32::MAX, -6 device
+// error.  *loss_out (may be null): final mean squared CDF error on the
+// training points.
 NF_EXPORT int32_t nfh_cdf_train_i32(const int32_t* keys, size_t n, uint32_t steps,
-                                    CdfModel* model_out, float* loss_out) {
+                                    uint32_t flags, CdfModel* model_out, float* loss_out) {
   if (!keys || !model_out) return NULL_OR_ABORT;
   if (n == 0) return MALFORMED;
   if (n > size_t(INT32_MAX)) return MALFORMED;
   NFH_GUARD_BEGIN
   Stage st;
   const int32_t* k = st.in(keys, n, S_IN0);
-  const float l = cdf_train_i32(k, uint32_t(n), steps, model_out, st.s);
+  const float l = cdf_train_i32(k, uint32_t(n), steps, flags, model_out, st.s);
   if (loss_out) *loss_out = l;
   st.finish();
   return OK;

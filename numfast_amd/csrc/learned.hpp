@@ -31,9 +31,11 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
                          hipStream_t s, uint64_t* stats = nullptr);
 
 // Train on a strided sample of the device keys (plain radix-sorted on the
-// device; its digit histograms add CDF points).  Writes *model (host).
+// device; its digit histograms add CDF points), by normalised gradient
+// descent with backtracking.  Writes *model (host).
 // Returns the final training loss (mean squared CDF error).
-float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, CdfModel* model,
-                    hipStream_t s);
+// flags bit 0: start from the linear CDF instead of the greedy fit.
+float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, uint32_t flags,
+                    CdfModel* model, hipStream_t s);
 
 }  // namespace nfh

@@ -90,10 +90,17 @@ GROUPS = {
     ],
     # Additive: learned-CDF sort, its training, the sort pipeline.
     "learned": [
-        ("nfh_cdf_train_i32", [_vp, _sz, _u32, _vp, _vp], _i32),
+        ("nfh_cdf_train_i32", [_vp, _sz, _u32, _u32, _vp, _vp], _i32),
         ("nfh_sort_perm_learned_i32", [_vp, _sz, _vp, _vp, _vp], _i32),
         ("nfh_mlp_grad", [_vp, _vp, _vp, _sz, _vp, _vp, _vp], _i32),
         ("nfh_filter_derive_sort_limit_i32", [_vp, _sz, _i32, _sz, _vp, _vp, _vp], _i64),
+    ],
+    # Additive: batched group-by plans (hipGraph).
+    "batch": [
+        ("nfh_group_batch_plan", [_vp] * 4 + [_sz] + [_vp] * 4 + [_sz, _vp], _vp),
+        ("nfh_group_batch_run", [_vp, _i32], _i64),
+        ("nfh_group_batch_info", [_vp, _vp], _i32),
+        ("nfh_group_batch_free", [_vp], None),
     ],
     # Additive surface of this library (no reference counterpart).
     "hip": [
@@ -643,15 +650,17 @@ class CdfModel(ctypes.Structure):
         return np.ctypeslib.as_array(self.p).copy()
 
 
-def cdf_train(keys, steps=100):
-    """Train the CDF model on a column (host array or device tensor).
+def cdf_train(keys, steps=100, linear_start=False):
+    """Train the CDF model on a column (host array or device tensor);
+    linear_start: begin from the linear CDF instead of the greedy fit.
     Returns (model, final loss)."""
     L = lib("learned")
     n = keys.numel() if hasattr(keys, "numel") else np.asarray(keys).size
     k = keys if hasattr(keys, "data_ptr") else _c(keys, np.int32)
     model = CdfModel()
     loss = ctypes.c_float()
-    rc = L.nfh_cdf_train_i32(_p(k), n, steps, ctypes.byref(model), ctypes.byref(loss))
+    rc = L.nfh_cdf_train_i32(_p(k), n, steps, 1 if linear_start else 0, ctypes.byref(model),
+                             ctypes.byref(loss))
     if rc != 0:
         raise RuntimeError("nfh_cdf_train_i32 rc=%d" % rc)
     return model, float(loss.value)
@@ -696,3 +705,60 @@ def filter_derive_sort_limit(v, limit=1000, mul=3, model=None):
     if r < 0:
         raise RuntimeError("nfh_filter_derive_sort_limit_i32 rc=%d" % r)
     return w[:r], rows[:r]
+
+
+# ------------------------------------------------------------ batched group-by
+class GroupBatch:
+    """A batched group-by plan over device tensors (nfh_group_batch_plan).
+
+    keys / vals: int32 CUDA tensors of all queries' rows, concatenated;
+    row_off: int64 (nq + 1); card: int32 (nq).  run(mode) returns
+    (ng, ukeys, counts, sums, group_off) views of the device outputs;
+    mode 0 replays the captured graph, mode 1 launches query by query.
+    """
+
+    def __init__(self, torch, keys, vals, row_off, card, cap=None):
+        L = lib("batch")
+        self._L, self._torch = L, torch
+        nq = int(card.numel() if hasattr(card, "numel") else len(card))
+        n = int(keys.numel())
+        cap = n if cap is None else int(cap)
+        dev = keys.device
+        self.keys, self.vals = keys, vals
+        self.row_off = torch.as_tensor(row_off, dtype=torch.int64, device=dev)
+        self.card = torch.as_tensor(card, dtype=torch.int32, device=dev)
+        self.ukeys = torch.empty(max(cap, 1), dtype=torch.int64, device=dev)
+        self.counts = torch.empty_like(self.ukeys)
+        self.sums = torch.empty_like(self.ukeys)
+        self.group_off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+        rc = ctypes.c_int32(0)
+        torch.cuda.synchronize()
+        self.plan = L.nfh_group_batch_plan(
+            keys.data_ptr(), vals.data_ptr(), self.row_off.data_ptr(), self.card.data_ptr(), nq,
+            self.ukeys.data_ptr(), self.counts.data_ptr(), self.sums.data_ptr(),
+            self.group_off.data_ptr(), cap, ctypes.byref(rc))
+        if not self.plan:
+            raise RuntimeError("nfh_group_batch_plan rc=%d" % rc.value)
+
+    def run(self, mode=0):
+        ng = int(self._L.nfh_group_batch_run(self.plan, mode))
+        if ng < 0:
+            return ng, None, None, None, None
+        return ng, self.ukeys[:ng], self.counts[:ng], self.sums[:ng], self.group_off
+
+    def info(self):
+        out = np.zeros(4, np.uint64)
+        self._L.nfh_group_batch_info(self.plan, out.ctypes.data)
+        return {"rows_per_item": int(out[0]), "workgroups": int(out[1]),
+                "work_items": int(out[2]), "slots": int(out[3])}
+
+    def close(self):
+        if self.plan:
+            self._L.nfh_group_batch_free(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -6,6 +6,8 @@ with torch on ``cuda:0`` so 2**28-row inputs are ready in milliseconds and
 stay resident in HBM.  Seed 0 unless stated.
 """
 
+import math
+
 import torch
 
 
@@ -63,3 +65,40 @@ def sort_config(n=1 << 28, dist="uniform", seed=0, device="cuda"):
         pool = full(1024)
         return pool[torch.randint(0, 1024, (n,), generator=g, device=device)]
     raise ValueError(dist)
+
+
+def batch_config(nq=4096, total=1 << 28, seed=0, device="cuda"):
+    """Batched group-by inputs on the device: nq queries, row counts
+    log-uniform in [1e3, 1e7] rescaled to `total` (factor returned, counts
+    floored at 1e3), key cardinality log-uniform in [10, 1e6], keys drawn
+    from a bounded Zipf(s) by the continuous inverse CDF, s uniform in
+    [0, 1.5] (s = 0: uniform), ranks scattered over [0, card) by an affine
+    map with a prime multiplier above 1e6; int32 values uniform in
+    [-2^15, 2^15).  Returns keys, vals, row_off (int64, nq + 1), card
+    (int32), scale."""
+    g = _gen(seed, device)
+    u = lambda m: torch.rand(m, generator=g, device=device, dtype=torch.float64)
+    lo, hi = math.log(1e3), math.log(1e7)
+    n = torch.exp(lo + (hi - lo) * u(nq))
+    scale = float(total / n.sum().item())
+    n = torch.clamp(torch.floor(n * scale), min=1e3).to(torch.int64)
+    card = torch.exp(math.log(10) + (math.log(1e6) - math.log(10)) * u(nq)).to(torch.int64)
+    s = 1.5 * u(nq)
+    row_off = torch.zeros(nq + 1, dtype=torch.int64, device=device)
+    row_off[1:] = torch.cumsum(n, 0)
+    N = int(row_off[-1].item())
+    qid = torch.repeat_interleave(torch.arange(nq, device=device), n)
+    c = card[qid].to(torch.float64)
+    sq = s[qid]
+    x = u(N)
+    one = (sq - 1.0).abs() < 1e-9
+    e = torch.where(one, torch.ones_like(sq), 1.0 - sq)
+    # inverse CDF of the density r^-s on [1, c + 1)
+    r = torch.where(one, torch.exp(x * torch.log(c + 1.0)),
+                    torch.pow(1.0 + x * (torch.pow(c + 1.0, e) - 1.0), 1.0 / e))
+    rank = torch.clamp(torch.floor(r) - 1.0, min=0.0)
+    rank = torch.minimum(rank, c - 1.0).to(torch.int64)
+    keys = ((rank * 1000003 + qid * 7919) % card[qid]).to(torch.int32)
+    vals = torch.randint(-(1 << 15), 1 << 15, (N,), generator=g, device=device,
+                         dtype=torch.int32)
+    return keys, vals, row_off, card.to(torch.int32), scale

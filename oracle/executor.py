@@ -172,3 +172,53 @@ def filter_derive_sort_limit(v, limit=1000, mul=3):
     w = (v[rows].astype(np.int64) * mul).astype(np.int32)
     perm = np.argsort(w, kind="stable")[:limit]
     return w[perm], rows[perm].astype(np.int64)
+
+
+# ----------------------------------------------------------------- batched group-by
+def group_batch(keys, vals, row_off, card):
+    """Independent dense group-bys (the dense lane, groupby/dense.rs:18-37,
+    with exact int64 sums of int32 values), compacted per query like the
+    carry build (core/compact.rs:16-37).  Returns (ukeys, counts, sums,
+    group_off) concatenated query by query, or -2 when a key is outside
+    its query's range [0, card)."""
+    keys = np.asarray(keys, np.int64)
+    vals = np.asarray(vals, np.int64)
+    uk, ct, sm, go = [], [], [], [0]
+    for q in range(len(card)):
+        k = keys[row_off[q]:row_off[q + 1]]
+        v = vals[row_off[q]:row_off[q + 1]]
+        if k.size and (k.min() < 0 or k.max() >= card[q]):
+            return -2
+        c = np.bincount(k, minlength=card[q]).astype(np.int64)
+        s = np.zeros(card[q], np.int64)
+        np.add.at(s, k, v)
+        present = np.flatnonzero(c)
+        uk.append(present.astype(np.int64))
+        ct.append(c[present])
+        sm.append(s[present])
+        go.append(go[-1] + present.size)
+    cat = (lambda a: np.concatenate(a) if a else np.zeros(0, np.int64))
+    return cat(uk), cat(ct), cat(sm), np.array(go, np.int64)
+
+
+def batch_workload(rng, nq, total_rows, card_lo=10, card_hi=10 ** 6, rows_lo=1e3,
+                   rows_hi=1e7, s_hi=1.5, vlo=-(2 ** 15), vhi=2 ** 15):
+    """Seeded shape of the batched configuration: row counts log-uniform in
+    [rows_lo, rows_hi] rescaled to total_rows (floored at rows_lo), key
+    cardinality log-uniform in [card_lo, card_hi], keys Zipf(s), s uniform
+    in [0, s_hi] (s = 0 uniform), values uniform in [vlo, vhi)."""
+    n = np.exp(rng.uniform(np.log(rows_lo), np.log(rows_hi), nq))
+    n = np.maximum(np.floor(n * (total_rows / n.sum())), rows_lo).astype(np.int64)
+    card = np.exp(rng.uniform(np.log(card_lo), np.log(card_hi), nq)).astype(np.int64)
+    svals = rng.uniform(0, s_hi, nq)
+    ks, vs = [], []
+    for q in range(nq):
+        c = int(card[q])
+        w = np.arange(1, c + 1, dtype=np.float64) ** -svals[q]
+        cdf = np.cumsum(w)
+        cdf /= cdf[-1]
+        r = np.minimum(np.searchsorted(cdf, rng.random(n[q]), side="right"), c - 1)
+        ks.append(rng.permutation(c)[r].astype(np.int32))
+        vs.append(rng.integers(vlo, vhi, n[q]).astype(np.int32))
+    row_off = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    return np.concatenate(ks), np.concatenate(vs), row_off, card.astype(np.int32)

@@ -374,3 +374,26 @@ def test_mlp_fd_check_rejects_missing_relu_mask():
     fd = np.array([mlp.finite_diff(p, x, y, mlp.B2 + i) for i in range(32)])
     assert np.allclose(mlp.grad(p, x, y)[mlp.B2:mlp.W3], fd, rtol=1e-5, atol=1e-9)
     assert not np.allclose(bad_b2, fd, rtol=1e-5, atol=1e-9)
+
+
+# ------------------------------------------------------------------ batched group-by
+def test_group_batch_oracle_vs_dict():
+    """Per query, a plain dict over the rows (sorted keys, counts, sums)."""
+    rng = np.random.default_rng(5)
+    k, v, row_off, card = ox.batch_workload(rng, 12, 4000, card_hi=300, rows_lo=5, rows_hi=1e3)
+    v[::7] = rng.integers(-(2 ** 31), 2 ** 31 - 1, v[::7].size, dtype=np.int64).astype(np.int32)
+    uk, ct, sm, go = ox.group_batch(k, v, row_off, card)
+    for q in range(len(card)):
+        acc = {}
+        for kk, vv in zip(k[row_off[q]:row_off[q + 1]].tolist(),
+                          v[row_off[q]:row_off[q + 1]].tolist()):
+            e = acc.setdefault(kk, [0, 0])
+            e[0] += 1
+            e[1] += vv
+        a, b = go[q], go[q + 1]
+        assert uk[a:b].tolist() == sorted(acc)
+        assert ct[a:b].tolist() == [acc[x][0] for x in sorted(acc)]
+        assert sm[a:b].tolist() == [acc[x][1] for x in sorted(acc)]
+    bad = k.copy()
+    bad[row_off[3]] = card[3]
+    assert ox.group_batch(bad, v, row_off, card) == -2
