@@ -9,7 +9,9 @@
 // ascending; counts int64; sums int64 (exact); min/max int32; mean float64
 // = double(sum) / double(count).
 //
-// Strategies (identical results):
+// Strategies (identical results; the rule picks partitioned, then dense,
+// then sort-based, and nfh_set_groupby_strategy can force one):
+//  * partitioned (span <= 2^24, no key mask): gbpart.hip;
 //  * dense (valid key span moderate): one accumulate pass into span-indexed
 //    32-byte slots {sum, count, min, max}.  Hot keys -- found by sampling
 //    2^16 rows, sorting the sample and keeping the keys seen most often --
@@ -489,6 +491,23 @@ static bool build_hot_table(const int32_t* keys, size_t n, int2** table, int32_t
   return true;
 }
 
+// Strategy of the next group-by on this thread (0: the rule below) and the
+// one the last group-by used: 1 dense (LDS hot keys + global spill), 2
+// partitioned (bucket passes, no per-row global atomics), 3 sort-based
+// (radix unique + aggregation by code).  A forced strategy that cannot run
+// on the input (span too large, key mask for the partitioned path) falls
+// back to the rule.
+namespace {
+thread_local int t_strategy = 0;
+thread_local int t_last_strategy = 0;
+}  // namespace
+int set_groupby_strategy(int s) {
+  const int old = t_strategy;
+  t_strategy = (s >= 0 && s <= 3) ? s : 0;
+  return old;
+}
+int last_groupby_strategy() { return t_last_strategy; }
+
 int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t* v,
                          const uint8_t* kv, size_t n, const GroupAggOut& o, hipStream_t s,
                          const int32_t* known_range) {
@@ -502,8 +521,13 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
     return 0;
   }
   const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
-  int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
-  if (span <= dense_span_limit(n) && gp_supported(span) && kv == nullptr) {
+  const bool dense_ok = span <= dense_span_limit(n);
+  const bool part_ok = dense_ok && gp_supported(span) && kv == nullptr;
+  int strat = t_strategy;
+  if ((strat == 1 && !dense_ok) || (strat == 2 && !part_ok)) strat = 0;
+  if (strat == 0) strat = part_ok ? 2 : dense_ok ? 1 : 3;
+  t_last_strategy = strat;
+  if (strat == 2) {
     // Partitioned path: no per-row global atomics.
     GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
     int2* table = nullptr;
@@ -513,7 +537,7 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
     gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, slots, s);
     return compact_slots(slots, span, nullptr, kmin, o, s);
   }
-  if (span <= dense_span_limit(n)) {
+  if (strat == 1) {
     GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
     hipLaunchKernelGGL(k_slot_init, dim3(grid_for(span, 256, 16 * num_cus())), dim3(256), 0, s,
                        slots, span);
@@ -758,6 +782,14 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
 }  // namespace nfh
 
 using namespace nfh;
+
+// Strategy override for this thread's following group-bys (0 = rule,
+// 1 dense, 2 partitioned, 3 sort-based); returns the previous setting.
+NF_EXPORT int32_t nfh_set_groupby_strategy(int32_t strategy) {
+  return set_groupby_strategy(strategy);
+}
+// The strategy this thread's last group-by ran (0 before any).
+NF_EXPORT int32_t nfh_groupby_last_strategy(void) { return last_groupby_strategy(); }
 
 // Group-by over raw int32 keys with NULL masks (valid / key_valid may be
 // null = all valid).  Outputs hold `cap` lanes; any output pointer may be

@@ -95,6 +95,15 @@ GROUPS = {
         ("nfh_mlp_grad", [_vp, _vp, _vp, _sz, _vp, _vp, _vp], _i32),
         ("nfh_filter_derive_sort_limit_i32", [_vp, _sz, _i32, _sz, _vp, _vp, _vp], _i64),
     ],
+    # Additive: learned group-by strategy selector.
+    "selector": [
+        ("nfh_groupby_features_i32", [_vp, _sz, _vp], _i32),
+        ("nfh_selector_grad", [_vp, _vp, _vp, _sz, _vp, _vp], _i32),
+        ("nfh_selector_train", [_vp, _vp, _sz, _u32, ctypes.c_float, _vp, _vp], _i32),
+        ("nfh_selector_predict", [_vp, _vp, _sz, _vp], _i32),
+        ("nfh_set_groupby_strategy", [_i32], _i32),
+        ("nfh_groupby_last_strategy", [], _i32),
+    ],
     # Additive: batched group-by plans (hipGraph).
     "batch": [
         ("nfh_group_batch_plan", [_vp] * 4 + [_sz] + [_vp] * 4 + [_sz, _vp], _vp),
@@ -762,3 +771,57 @@ class GroupBatch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ strategy selector
+SEL_P = 64 * 64 + 64 + 3 * 64 + 3
+
+
+def groupby_features(keys):
+    """The selector's 64 features of a key column (host array or tensor)."""
+    out = np.zeros(64, np.float32)
+    k = keys if hasattr(keys, "data_ptr") else _c(keys, np.int32)
+    n = k.numel() if hasattr(k, "numel") else k.size
+    rc = lib("selector").nfh_groupby_features_i32(_p(k), n, _p(out))
+    if rc != 0:
+        raise RuntimeError("nfh_groupby_features_i32 rc=%d" % rc)
+    return out
+
+
+def selector_init(seed=0):
+    """He-style random start of the 64 -> 64 -> 3 classifier."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(SEL_P, np.float32)
+    p[:64 * 64] = rng.normal(0, np.sqrt(2 / 64), 64 * 64)
+    p[64 * 64 + 64:64 * 64 + 64 + 192] = rng.normal(0, np.sqrt(1 / 64), 192)
+    return p
+
+
+def selector_grad(params, X, y):
+    g = np.zeros(SEL_P, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib("selector").nfh_selector_grad(_p(_c(params, np.float32)), _p(_c(X, np.float32)),
+                                           _p(_c(y, np.int32)), len(y), _p(g), _p(loss))
+    if rc != 0:
+        raise RuntimeError("nfh_selector_grad rc=%d" % rc)
+    return g, float(loss[0])
+
+
+def selector_train(X, y, steps=300, lr=1e-2, params=None):
+    p = selector_init() if params is None else _c(params, np.float32).copy()
+    loss = np.zeros(1, np.float32)
+    rc = lib("selector").nfh_selector_train(_p(_c(X, np.float32)), _p(_c(y, np.int32)), len(y),
+                                            steps, lr, _p(p), _p(loss))
+    if rc != 0:
+        raise RuntimeError("nfh_selector_train rc=%d" % rc)
+    return p, float(loss[0])
+
+
+def selector_predict(params, X):
+    X = _c(X, np.float32)
+    out = np.zeros(X.shape[0], np.int32)
+    rc = lib("selector").nfh_selector_predict(_p(_c(params, np.float32)), _p(X), X.shape[0],
+                                              _p(out))
+    if rc != 0:
+        raise RuntimeError("nfh_selector_predict rc=%d" % rc)
+    return out

@@ -1,0 +1,170 @@
+"""Learned group-by strategy selector: data set, training, evaluation.
+
+Key columns are generated on the GPU from a seeded mixture -- uniform keys,
+Zipf(s) with s in [0.5, 2], clustered runs (run length 1..4096) and
+presorted keys -- with 2^lo .. 2^hi rows (log-uniform) and a log-uniform
+key cardinality.  Each column is labelled by timing the three plain
+group-by strategies (nfh_group_agg_i32 forced to dense / partitioned /
+sort-based; a strategy that cannot run on the column counts at the time
+of the one that runs instead).  The selector (64 features -> 64 -> 3 MLP)
+is trained on the device on 80% of the columns; on the held-out 20% the
+report compares the total time of the selected strategies with the best
+fixed plan (one strategy for every column, chosen on the training split),
+with the built-in rule and with the per-column best.
+
+    python -m numfast_amd.selector --columns 10000 --out report.json
+"""
+
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+KINDS = ("uniform", "zipf", "runs", "presorted")
+
+
+def gen_column(torch, g, n, kind, card, s=1.0, device="cuda"):
+    """One int32 key column of n rows over [0, card)."""
+    u = lambda m: torch.rand(m, generator=g, device=device, dtype=torch.float64)
+    if kind in ("uniform", "presorted"):
+        k = torch.randint(0, card, (n,), generator=g, device=device, dtype=torch.int64)
+        if kind == "presorted":
+            k = torch.sort(k).values
+        return k.to(torch.int32)
+    if kind == "zipf":
+        x = u(n)
+        c = float(card)
+        if abs(s - 1.0) < 1e-9:
+            r = torch.exp(x * math.log(c + 1.0))
+        else:
+            e = 1.0 - s
+            r = torch.pow(1.0 + x * ((c + 1.0) ** e - 1.0), 1.0 / e)
+        rank = torch.clamp(torch.floor(r) - 1.0, 0, c - 1).to(torch.int64)
+        return ((rank * 1000003) % card).to(torch.int32)
+    if kind == "runs":
+        # expected total 2n (mean run 2048.5); a short draw is topped up
+        lens = torch.randint(1, 4097, (n // 1024 + 8,), generator=g, device=device)
+        vals = torch.randint(0, card, (lens.numel(),), generator=g, device=device,
+                             dtype=torch.int64)
+        k = torch.repeat_interleave(vals, lens)[:n]
+        if k.numel() < n:
+            k = torch.cat([k, torch.randint(0, card, (n - k.numel(),), generator=g,
+                                             device=device, dtype=torch.int64)])
+        return k.to(torch.int32)
+    raise ValueError(kind)
+
+
+def label_columns(torch, native, columns, lo=20, hi=26, seed=0, log=None):
+    """Features and per-strategy times (ms) for `columns` seeded columns."""
+    L = native.lib("hip")
+    S = native.lib("selector")
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    nmax = 1 << hi
+    vals = torch.randint(-(1 << 15), 1 << 15, (nmax,), generator=g, device="cuda",
+                         dtype=torch.int32)
+    outs = [torch.empty(nmax, dtype=dt, device="cuda") for dt in
+            (torch.int64, torch.int64, torch.int64, torch.int32, torch.int32, torch.float64)]
+    X = np.zeros((columns, 64), np.float32)
+    T = np.zeros((columns, 3), np.float64)
+    ran = np.zeros((columns, 3), np.int32)
+    meta = []
+    t_start = time.time()
+    for c in range(columns):
+        n = int(2 ** rng.uniform(lo, hi))
+        kind = KINDS[rng.integers(0, len(KINDS))]
+        card = int(2 ** rng.uniform(4, 26))
+        s = float(rng.uniform(0.5, 2.0))
+        k = gen_column(torch, g, n, kind, card, s)
+        X[c] = native.groupby_features(k)
+        for st in (1, 2, 3):
+            S.nfh_set_groupby_strategy(st)
+            args = (k.data_ptr(), vals.data_ptr(), None, None, n,
+                    *[o.data_ptr() for o in outs], nmax)
+            L.nfh_group_agg_i32(*args)  # warm (scratch sized)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ng = L.nfh_group_agg_i32(*args)
+            torch.cuda.synchronize()
+            T[c, st - 1] = (time.perf_counter() - t) * 1e3
+            ran[c, st - 1] = S.nfh_groupby_last_strategy()
+            if ng < 0:
+                raise RuntimeError("group-by rc %d" % ng)
+        S.nfh_set_groupby_strategy(0)
+        meta.append({"n": n, "kind": kind, "card": card, "s": round(s, 3)})
+        del k
+        if log and (c + 1) % 500 == 0:
+            log("labelled %d / %d columns (%.0f s)" % (c + 1, columns, time.time() - t_start))
+    # a strategy that could not run costs the time of the one that ran
+    for c in range(columns):
+        for st in range(3):
+            if ran[c, st] != st + 1:
+                T[c, st] = T[c, ran[c, st] - 1]
+    return X, T, ran, meta
+
+
+def rule_choice(ran):
+    """The built-in rule's strategy (what strategy 0 would run): the
+    partitioned path when it can run, else dense, else sort-based."""
+    return np.where(ran[:, 1] == 2, 1, np.where(ran[:, 0] == 1, 0, 2))
+
+
+def evaluate(native, X, T, ran, train_frac=0.8, steps=400, lr=1e-2, seed=0):
+    n = X.shape[0]
+    order = np.random.default_rng(seed + 1).permutation(n)
+    ntr = int(train_frac * n)
+    tr, te = order[:ntr], order[ntr:]
+    y = np.argmin(T, axis=1).astype(np.int32)
+    t0 = time.perf_counter()
+    params, loss = native.selector_train(X[tr], y[tr], steps=steps, lr=lr)
+    train_ms = (time.perf_counter() - t0) * 1e3
+    pred = native.selector_predict(params, X[te])
+    fixed = int(np.argmin(T[tr].sum(0)))
+    rule = rule_choice(ran)
+    t_sel = float(T[te, pred].sum())
+    t_fixed = float(T[te, fixed].sum())
+    t_rule = float(T[te, rule[te]].sum())
+    t_best = float(T[te].min(1).sum())
+    return {
+        "columns": n, "train": int(ntr), "held_out": int(n - ntr),
+        "train_steps": steps, "train_loss": loss, "train_ms": round(train_ms, 1),
+        "label_share": [round(float(np.mean(y == c)), 3) for c in range(3)],
+        "held_out_accuracy": round(float(np.mean(pred == y[te])), 4),
+        "fixed_plan": ["dense", "partitioned", "sort"][fixed],
+        "held_out_ms": {"selected": round(t_sel, 2), "fixed_plan": round(t_fixed, 2),
+                        "rule": round(t_rule, 2), "per_column_best": round(t_best, 2)},
+        "speedup_vs_fixed_plan": round(t_fixed / t_sel, 4),
+        "speedup_vs_rule": round(t_rule / t_sel, 4),
+        "selected_over_best": round(t_sel / t_best, 4),
+    }, params
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--columns", type=int, default=10000)
+    ap.add_argument("--lo", type=int, default=20, help="log2 of the fewest rows")
+    ap.add_argument("--hi", type=int, default=26, help="log2 of the most rows")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    import torch  # first: the library shares torch's HIP runtime
+    torch.cuda.init()
+    from numfast_amd import native
+    t = time.time()
+    X, T, ran, meta = label_columns(torch, native, args.columns, args.lo, args.hi, args.seed,
+                                    log=lambda m: print(m, flush=True))
+    rep, params = evaluate(native, X, T, ran, steps=args.steps, seed=args.seed)
+    rep["rows_log2"] = [args.lo, args.hi]
+    rep["label_s"] = round(time.time() - t, 1)
+    print(json.dumps(rep), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump({"report": rep, "params": params.tolist()}, f)
+
+
+if __name__ == "__main__":
+    main()

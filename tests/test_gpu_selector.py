@@ -1,0 +1,94 @@
+"""GPU: group-by strategies agree with the oracle, the selector's features
+and classifier gradients match their CPU restatement, and its device
+training learns."""
+
+import numpy as np
+import pytest
+
+from numfast_amd import native
+from oracle import executor as ox
+from oracle import selector as osel
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(got, ref):
+    assert got["ukeys"].tolist() == ref["ukeys"].tolist()
+    for k in ("counts", "sums", "mins", "maxs"):
+        assert got[k].tobytes() == ref[k].astype(got[k].dtype).tobytes(), k
+    assert got["means"].tobytes() == ref["means"].tobytes()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "zipf", "runs", "wide"])
+def test_every_strategy_gives_the_same_groups(hip, kind):
+    S = native.lib("selector")
+    rng = np.random.default_rng(3)
+    n = 1 << 18
+    if kind == "uniform":
+        keys = rng.integers(0, 50000, n).astype(np.int32)
+    elif kind == "zipf":
+        keys = (rng.zipf(1.3, n) % 100000).astype(np.int32)
+    elif kind == "runs":
+        keys = np.repeat(rng.integers(-5000, 5000, n // 100 + 1), 100)[:n].astype(np.int32)
+    else:
+        keys = rng.integers(-(2 ** 31), 2 ** 31 - 1, n, dtype=np.int64).astype(np.int32)
+    vals = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
+    valid = rng.random(n) > 0.05
+    ref = ox.group_agg(keys, vals, valid)
+    try:
+        for st in (1, 2, 3):
+            S.nfh_set_groupby_strategy(st)
+            _check(native.group_agg(keys, vals, valid), ref)
+            ran = S.nfh_groupby_last_strategy()
+            assert ran == st or (kind == "wide" and st in (1, 2) and ran == 3)
+    finally:
+        S.nfh_set_groupby_strategy(0)
+
+
+@pytest.mark.parametrize("kind", ["ramp", "equal", "random", "small"])
+def test_features_match_restatement(hip, kind):
+    rng = np.random.default_rng(4)
+    keys = {"ramp": np.arange(300000, dtype=np.int32),
+            "equal": np.full(70000, -3, np.int32),
+            "random": rng.integers(-(2 ** 31), 2 ** 31 - 1, 1 << 20, dtype=np.int64).astype(
+                np.int32),
+            "small": rng.integers(0, 9, 37).astype(np.int32)}[kind]
+    got = native.groupby_features(keys)
+    assert np.allclose(got, osel.features(keys), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("N", [1, 63, 1000, 5000])
+def test_selector_gradient_matches_oracle(hip, N):
+    rng = np.random.default_rng(N)
+    p = rng.normal(0, 0.3, osel.P).astype(np.float32)
+    X = rng.random((N, 64)).astype(np.float32)
+    y = rng.integers(0, 3, N).astype(np.int32)
+    g, loss = native.selector_grad(p, X, y)
+    ref = osel.grad(p, X, y)
+    scale = max(1e-12, np.abs(ref).max())
+    assert np.max(np.abs(g - ref)) <= 2e-4 * scale
+    assert abs(loss - osel.loss(p, X, y)) <= 1e-4 * max(1.0, osel.loss(p, X, y))
+
+
+def test_selector_training_learns_a_rule():
+    """Labels from a rule on two features; device training fits it."""
+    rng = np.random.default_rng(5)
+    X = rng.random((4000, 64)).astype(np.float32)
+    y = np.where(X[:, 0] > 0.6, 2, np.where(X[:, 1] > 0.5, 1, 0)).astype(np.int32)
+    p0 = native.selector_init(1)
+    _, l0 = native.selector_grad(p0, X, y)
+    p, _ = native.selector_train(X[:3000], y[:3000], steps=600, lr=1e-2, params=p0)
+    _, l1 = native.selector_grad(p, X, y)
+    acc = np.mean(native.selector_predict(p, X[3000:]) == y[3000:])
+    print("loss %.3f -> %.3f, held-out accuracy %.3f" % (l0, l1, acc))
+    assert l1 < 0.5 * l0 and acc > 0.85
+
+
+def test_selector_end_to_end_small():
+    import torch
+    from numfast_amd import selector
+    X, T, ran, meta = selector.label_columns(torch, native, 60, lo=14, hi=17, seed=2)
+    assert np.all(T > 0) and X.shape == (60, 64)
+    rep, _ = selector.evaluate(native, X, T, ran, steps=100)
+    print(rep)
+    assert rep["held_out"] == 12 and rep["speedup_vs_fixed_plan"] > 0

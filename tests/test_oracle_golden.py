@@ -397,3 +397,27 @@ def test_group_batch_oracle_vs_dict():
     bad = k.copy()
     bad[row_off[3]] = card[3]
     assert ox.group_batch(bad, v, row_off, card) == -2
+
+
+# ------------------------------------------------------------------ strategy selector
+def test_selector_oracle_gradient_vs_finite_differences():
+    from oracle import selector as sel
+    rng = np.random.default_rng(8)
+    p = rng.normal(0, 0.3, sel.P)
+    X = rng.random((200, 64))
+    y = rng.integers(0, 3, 200)
+    g = sel.grad(p, X, y)
+    for idx in list(range(0, sel.P, 97)) + [sel.B2, sel.B2 + 2, sel.W2 + 5, sel.B1 + 7]:
+        fd = sel.finite_diff(p, X, y, idx)
+        assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), idx
+
+
+def test_selector_features_restatement():
+    """Closed forms on small columns: sorted keys are all ascending pairs;
+    a single repeated key has one distinct value holding every row."""
+    from oracle import selector as sel
+    f = sel.features(np.arange(10000, dtype=np.int32))
+    assert f[sel.HBINS + 5] == 1.0 and f[sel.HBINS + 2] == 1.0
+    assert abs(f[:sel.HBINS].sum() - 1.0) < 1e-12
+    f = sel.features(np.full(5000, 7, np.int32))
+    assert f[sel.HBINS + 2] == 1 / 4096 and f[sel.HBINS + 3] == 1.0 and f[0] == 1.0
