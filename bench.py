@@ -35,7 +35,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows-log2", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip argsort / uniform legs")
+    ap.add_argument("--no-extra", action="store_true", help="only the group-by + unique step")
+    ap.add_argument("--legs", default="step,argsort,learned,batch,selector,uniform",
+                    help="comma list of legs to run (for profiling one workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling)")
@@ -297,27 +299,31 @@ def main():
             state["ng"], state["nu"] = ng, nu
         return step
 
+    legs = {"step"} if args.no_extra else set(args.legs.split(","))
+    extra = {}
+    t, roofline = None, None
     torch.cuda.synchronize()
-    L.nfh_profile_enable(0)
-    step = step_on(keys, vals, valid, kvalid)
-    for _ in range(args.warmup):
-        step()
-    L.nfh_profile_reset()
-    L.nfh_profile_enable(1)
-    t = timed(torch, step, args.steps, 0)
-    L.nfh_profile_enable(0)
-    kernels = kernel_table(L)
-    ng, nu = state["ng"], state["nu"]
-    roofline = roofline_of(kernels, args.steps)
-    # whole-step algorithmic bytes: group-by 9 B/row + 40 B/group, unique
-    # 5 B/row read + 4 B/row inverse + 4 B per distinct key
-    step_bytes = 9.0 * n + 40.0 * ng + 9.0 * n + 4.0 * nu
-    extra = {"groupby_ng": ng, "unique_ng": nu, "kernels": kernels,
-             "step_alg_GBps": round(step_bytes / t / 1e9, 1),
-             "step_alg_frac": round(step_bytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+    if "step" in legs:
+        L.nfh_profile_enable(0)
+        step = step_on(keys, vals, valid, kvalid)
+        for _ in range(args.warmup):
+            step()
+        L.nfh_profile_reset()
+        L.nfh_profile_enable(1)
+        t = timed(torch, step, args.steps, 0)
+        L.nfh_profile_enable(0)
+        kernels = kernel_table(L)
+        ng, nu = state["ng"], state["nu"]
+        roofline = roofline_of(kernels, args.steps)
+        # whole-step algorithmic bytes: group-by 9 B/row + 40 B/group,
+        # unique 5 B/row read + 4 B/row inverse + 4 B per distinct key
+        step_bytes = 9.0 * n + 40.0 * ng + 9.0 * n + 4.0 * nu
+        extra.update({"groupby_ng": ng, "unique_ng": nu, "kernels": kernels,
+                      "step_alg_GBps": round(step_bytes / t / 1e9, 1),
+                      "step_alg_frac": round(step_bytes / t / 1e9 / HBM_PEAK_GBS, 4)})
     del keys, vals, valid, kvalid
 
-    if not args.no_extra:
+    if "argsort" in legs:
         # stable argsort, 2^28 uniform int32 (pass model 60 B/row)
         k = W.sort_config(n, "uniform")
         perm = torch.empty(n, dtype=torch.int32, device="cuda")
@@ -342,13 +348,22 @@ def main():
             "compulsory_frac_8B": round(8.0 * n / ts / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": sk, "roofline": roofline_of(sk, sort_steps)}
         del k, perm, t0b, t1b, tpb
+    if "learned" in legs:
         # config-3 shapes: plain LSD vs learned-CDF sort, and the
         # filter(v>0) -> derive(v*3) -> sort -> limit(1000) pipeline
         extra["sort_pipeline_2^28"] = sort_pipeline_leg(torch, L, W, native, n,
                                                         max(3, args.steps // 2))
+    if "batch" in legs:
         # batched group-by: 4096 queries, one graph-captured persistent launch
         # vs the same kernels launched query by query
         extra["batched_groupby_4096q"] = batch_leg(torch, W, native, n, max(3, args.steps // 2))
+    if "selector" in legs:
+        # learned strategy selector on a reduced column set (the full
+        # 10,000-column run: python -m numfast_amd.selector)
+        from numfast_amd import selector as SEL
+        X, T, ran, _ = SEL.label_columns(torch, native, 400, lo=18, hi=22, seed=7)
+        extra["strategy_selector_400col"], _ = SEL.evaluate(native, X, T, ran, steps=400)
+    if "uniform" in legs:
         # uniform-2^24 group-by variant
         keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")
         tu = timed(torch, step_on(keys, vals, valid, kvalid), max(3, args.steps // 2), 1)
@@ -358,12 +373,12 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": n / t,
+        "value": None if t is None else n / t,
         "unit": "rows/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t * 1e3, 4),
+        "ms_per_step": None if t is None else round(t * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
