@@ -161,4 +161,22 @@ __device__ __forceinline__ uint32_t lb_lookback(const uint64_t* lb, uint32_t til
   return excl;
 }
 
+// Streaming (non-temporal) quad loads and stores: rows read or written
+// once bypass the L2 so that randomly accessed tables stay resident.
+typedef int32_t nf_v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int4 ld_stream4(const int32_t* p, uint32_t q) {
+  const nf_v4i x = __builtin_nontemporal_load(reinterpret_cast<const nf_v4i*>(p) + q);
+  return make_int4(x.x, x.y, x.z, x.w);
+}
+
+__device__ __forceinline__ void st_stream4(int32_t* p, uint32_t q, int4 v) {
+  nf_v4i x;
+  x.x = v.x;
+  x.y = v.y;
+  x.z = v.z;
+  x.w = v.w;
+  __builtin_nontemporal_store(x, reinterpret_cast<nf_v4i*>(p) + q);
+}
+
 }  // namespace nfh

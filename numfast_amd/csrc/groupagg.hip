@@ -586,36 +586,85 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
 }
 
 // ---------------------------------------------------------------- unique
+// Dense unique.  k_mark sets one flag byte per present key offset
+// o = key - kmin with plain byte stores (set-once: most rows only read, and
+// a stale read only repeats a store of 1).  The scan packs the flags into
+// one 64-bit word per 32 offsets: presence bits low, and high the number of
+// present offsets in the words below, so the dense code of a present key is
+// hi(w) + popc(lo(w) & below(o)) -- one random 8-byte lookup per row over
+// span / 4 bytes (4 MiB at 2^24 ids).  Keys of the sampled hot-key table
+// are resolved in LDS in both row passes; they stay off the global tables.
+
+// Open-addressing lookup in the LDS copy of the hot-key table: the entry
+// index of k, or -1.
+__device__ __forceinline__ int32_t hot_find(const int32_t* t_key, const int32_t* t_idx,
+                                            int32_t k) {
+  uint32_t h = hot_hash(k);
+  for (;;) {
+    const int32_t idx = t_idx[h];
+    if (idx < 0 || t_key[h] == k) return idx;
+    h = (h + 1) & (HOT_TBL - 1);
+  }
+}
+
+__device__ __forceinline__ uint32_t load_hot_table(const int2* __restrict__ table,
+                                                   const uint32_t* __restrict__ hot_n,
+                                                   int32_t* t_key, int32_t* t_idx) {
+  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) {
+    const int2 e = table ? table[b] : make_int2(0, -1);
+    t_key[b] = e.x;
+    t_idx[b] = e.y;
+  }
+  return table ? *hot_n : 0u;
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
                                               const uint8_t* __restrict__ v, uint32_t n,
-                                              int32_t kmin, uint32_t* __restrict__ flag) {
-  // Set-once flags: most rows only read.  In the quad path all 4*RQ flag
-  // reads of a thread are issued before any store (rows with equal keys
-  // may both store 1).
+                                              int32_t kmin, uint8_t* __restrict__ flag,
+                                              const int2* __restrict__ table,
+                                              const int32_t* __restrict__ hot_keys,
+                                              const uint32_t* __restrict__ hot_n) {
+  __shared__ int32_t t_key[HOT_TBL];
+  __shared__ int32_t t_idx[HOT_TBL];
+  __shared__ uint32_t s_seen[HOT_MAX];
+  const uint32_t nh = load_hot_table(table, hot_n, t_key, t_idx);
+  for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) s_seen[b] = 0;
+  __syncthreads();
+  // true when the row still needs the global flag
+  auto cold = [&](int32_t k) -> bool {
+    if (nh == 0) return true;
+    const int32_t idx = hot_find(t_key, t_idx, k);
+    if (idx < 0) return true;
+    s_seen[idx] = 1u;
+    return false;
+  };
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t tail = 0;
   if constexpr (VEC) {
+    // all 4*RQ flag reads of a thread are issued before any store (rows
+    // with equal keys may both store 1)
     const uint32_t nq = n / 4;
     uint32_t q = tid;
     for (; q + (RQ - 1) * stride < nq; q += RQ * stride) {
       uint32_t off[4 * RQ], f[4 * RQ];
-      uint32_t ok = 0;
+      uint32_t need = 0;
 #pragma unroll
       for (int u = 0; u < RQ; ++u) {
         const uint32_t qq = q + uint32_t(u) * stride;
-        const int4 k4 = reinterpret_cast<const int4*>(keys)[qq];
-        const uint32_t m = v ? reinterpret_cast<const uint32_t*>(v)[qq] : 0x01010101u;
-        off[4 * u + 0] = uint32_t(k4.x - kmin);
-        off[4 * u + 1] = uint32_t(k4.y - kmin);
-        off[4 * u + 2] = uint32_t(k4.z - kmin);
-        off[4 * u + 3] = uint32_t(k4.w - kmin);
+        const int4 k4 = ld_stream4(keys, qq);
+        const uint32_t m =
+            v ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(v) + qq) : 0x01010101u;
+        const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ok |= uint32_t((m >> (8 * c)) & 0xFFu ? 1u : 0u) << (4 * u + c);
+        for (int c = 0; c < 4; ++c) {
+          off[4 * u + c] = uint32_t(kk[c] - kmin);
+          if (((m >> (8 * c)) & 0xFFu) && cold(kk[c])) need |= 1u << (4 * u + c);
+        }
       }
 #pragma unroll
-      for (int j = 0; j < 4 * RQ; ++j) f[j] = ((ok >> j) & 1u) ? flag[off[j]] : 1u;
+      for (int j = 0; j < 4 * RQ; ++j) f[j] = ((need >> j) & 1u) ? flag[off[j]] : 1u;
 #pragma unroll
       for (int j = 0; j < 4 * RQ; ++j)
         if (f[j] == 0u) flag[off[j]] = 1u;
@@ -626,79 +675,139 @@ __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
       const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        if (!((m >> (8 * c)) & 0xFFu)) continue;
-        uint32_t* p = flag + uint32_t(kk[c] - kmin);
+        if (!((m >> (8 * c)) & 0xFFu) || !cold(kk[c])) continue;
+        uint8_t* p = flag + uint32_t(kk[c] - kmin);
         if (*p == 0u) *p = 1u;
       }
     }
     tail = nq * 4;
   }
   for (uint32_t i = tail + tid; i < n; i += stride) {
-    if (v != nullptr && v[i] == 0) continue;
-    uint32_t* p = flag + uint32_t(keys[i] - kmin);
+    if ((v != nullptr && v[i] == 0) || !cold(keys[i])) continue;
+    uint8_t* p = flag + uint32_t(keys[i] - kmin);
     if (*p == 0u) *p = 1u;
   }
+  if (nh == 0) return;
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x)
+    if (s_seen[b]) flag[uint32_t(hot_keys[b] - kmin)] = 1u;
 }
 
-struct RankOp {
-  uint32_t* flag;  // in: presence, out: rank (dense code)
+// The 32 flag bytes of word w (each 0 or 1) as a bit mask.
+__device__ __forceinline__ uint32_t flag_word(const uint8_t* __restrict__ flag, uint32_t w) {
+  const uint4* p = reinterpret_cast<const uint4*>(flag) + 2 * size_t(w);
+  const uint4 a = p[0], b = p[1];
+  const uint32_t c[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  uint32_t bits = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t x = c[i];
+    const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+    bits |= nib << (4 * i);
+  }
+  return bits;
+}
+
+// Exclusive scan of the per-word flag counts; each word writes its packed
+// (bits, base) word and its present keys (ascending) to uniq.
+struct BitRankOp {
+  const uint8_t* flag;
+  u64* words;
   int32_t kmin;
   int32_t* uniq;
   int64_t* ng;
-  __device__ uint32_t value(uint32_t s) const { return flag[s]; }
-  __device__ void emit(uint32_t s, uint32_t ex) const {
-    if (flag[s]) uniq[ex] = kmin + int32_t(s);
-    flag[s] = ex;
+  __device__ uint32_t value(uint32_t w) const { return uint32_t(__popc(flag_word(flag, w))); }
+  __device__ void emit(uint32_t w, uint32_t ex) const {
+    uint32_t b = flag_word(flag, w);
+    words[w] = (u64(ex) << 32) | b;
+    const int32_t k0 = kmin + int32_t(w << 5);
+    while (b) {
+      uniq[ex++] = k0 + (__ffs(b) - 1);
+      b &= b - 1u;
+    }
   }
   __device__ void total(uint32_t t) const { *ng = int64_t(t); }
 };
 
+__device__ __forceinline__ int32_t bit_code(const u64* __restrict__ words, uint32_t o) {
+  const u64 w = words[o >> 5];
+  return int32_t(uint32_t(w >> 32) + uint32_t(__popc(uint32_t(w) & ((1u << (o & 31u)) - 1u))));
+}
+
+// Dense codes of the hot keys (read by k_inv from LDS).  A hot key with no
+// valid row gets the code of its insertion point, never read.
+__global__ void k_hot_codes(const int32_t* __restrict__ hot_keys,
+                            const uint32_t* __restrict__ hot_n, int32_t kmin,
+                            const u64* __restrict__ words, int32_t* __restrict__ codes) {
+  const uint32_t nh = *hot_n;
+  for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x)
+    codes[b] = bit_code(words, uint32_t(hot_keys[b] - kmin));
+}
+
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_inv_dense(const int32_t* __restrict__ keys,
-                                                   const uint8_t* __restrict__ v, uint32_t n,
-                                                   int32_t kmin, const uint32_t* __restrict__ rank,
-                                                   int32_t* __restrict__ inv) {
+__global__ __launch_bounds__(256) void k_inv(const int32_t* __restrict__ keys,
+                                             const uint8_t* __restrict__ v, uint32_t n,
+                                             int32_t kmin, const u64* __restrict__ words,
+                                             const int2* __restrict__ table,
+                                             const uint32_t* __restrict__ hot_n,
+                                             const int32_t* __restrict__ hot_code,
+                                             int32_t* __restrict__ inv) {
+  __shared__ int32_t t_key[HOT_TBL];
+  __shared__ int32_t t_idx[HOT_TBL];
+  __shared__ int32_t s_code[HOT_MAX];
+  const uint32_t nh = load_hot_table(table, hot_n, t_key, t_idx);
+  for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x) s_code[b] = hot_code[b];
+  __syncthreads();
+  auto code = [&](int32_t k, bool ok) -> int32_t {
+    if (!ok) return -1;
+    if (nh != 0) {
+      const int32_t idx = hot_find(t_key, t_idx, k);
+      if (idx >= 0) return s_code[idx];
+    }
+    return bit_code(words, uint32_t(k - kmin));
+  };
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t tail = 0;
   if constexpr (VEC) {
     const uint32_t nq = n / 4;
     uint32_t q = tid;
-    // RQ quads per thread: key loads, then all rank gathers, then stores
+    // RQ quads per thread: key loads, then all code lookups, then stores
     for (; q + (RQ - 1) * stride < nq; q += RQ * stride) {
       int4 k4[RQ];
       uint32_t m[RQ];
 #pragma unroll
       for (int u = 0; u < RQ; ++u) {
         const uint32_t qq = q + uint32_t(u) * stride;
-        k4[u] = reinterpret_cast<const int4*>(keys)[qq];
-        m[u] = v ? reinterpret_cast<const uint32_t*>(v)[qq] : 0x01010101u;
+        k4[u] = ld_stream4(keys, qq);
+        m[u] = v ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(v) + qq)
+                 : 0x01010101u;
       }
       int4 o[RQ];
 #pragma unroll
       for (int u = 0; u < RQ; ++u) {
-        o[u].x = (m[u] & 0xFFu) ? int32_t(rank[uint32_t(k4[u].x - kmin)]) : -1;
-        o[u].y = (m[u] & 0xFF00u) ? int32_t(rank[uint32_t(k4[u].y - kmin)]) : -1;
-        o[u].z = (m[u] & 0xFF0000u) ? int32_t(rank[uint32_t(k4[u].z - kmin)]) : -1;
-        o[u].w = (m[u] & 0xFF000000u) ? int32_t(rank[uint32_t(k4[u].w - kmin)]) : -1;
+        o[u].x = code(k4[u].x, m[u] & 0xFFu);
+        o[u].y = code(k4[u].y, m[u] & 0xFF00u);
+        o[u].z = code(k4[u].z, m[u] & 0xFF0000u);
+        o[u].w = code(k4[u].w, m[u] & 0xFF000000u);
       }
 #pragma unroll
-      for (int u = 0; u < RQ; ++u) reinterpret_cast<int4*>(inv)[q + uint32_t(u) * stride] = o[u];
+      for (int u = 0; u < RQ; ++u) st_stream4(inv, q + uint32_t(u) * stride, o[u]);
     }
     for (; q < nq; q += stride) {
       const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
       const uint32_t m = v ? reinterpret_cast<const uint32_t*>(v)[q] : 0x01010101u;
       int4 o;
-      o.x = (m & 0xFFu) ? int32_t(rank[uint32_t(k4.x - kmin)]) : -1;
-      o.y = (m & 0xFF00u) ? int32_t(rank[uint32_t(k4.y - kmin)]) : -1;
-      o.z = (m & 0xFF0000u) ? int32_t(rank[uint32_t(k4.z - kmin)]) : -1;
-      o.w = (m & 0xFF000000u) ? int32_t(rank[uint32_t(k4.w - kmin)]) : -1;
+      o.x = code(k4.x, m & 0xFFu);
+      o.y = code(k4.y, m & 0xFF00u);
+      o.z = code(k4.z, m & 0xFF0000u);
+      o.w = code(k4.w, m & 0xFF000000u);
       reinterpret_cast<int4*>(inv)[q] = o;
     }
     tail = nq * 4;
   }
   for (uint32_t i = tail + tid; i < n; i += stride)
-    inv[i] = (v == nullptr || v[i]) ? int32_t(rank[uint32_t(keys[i] - kmin)]) : -1;
+    inv[i] = code(keys[i], v == nullptr || v[i]);
 }
 
 __global__ void k_inv_scatter(const uint8_t* __restrict__ v, uint32_t n,
@@ -734,29 +843,42 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
   const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
   int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
   if (span <= dense_span_limit(n)) {
-    uint32_t* flag = scratch_t<uint32_t>(S_WORK0, span);
-    fill_u32(flag, 0, span, s);
+    const size_t nw = size_t((span + 31) >> 5);
+    uint8_t* flag = scratch_t<uint8_t>(S_WORK0, 32 * nw);
+    u64* words = scratch_t<u64>(S_WORK1, nw);
+    int32_t* hot_code = scratch_t<int32_t>(S_T0, HOT_MAX);
+    fill_u32(flag, 0, 8 * nw, s);
+    int2* table = nullptr;
+    int32_t* hot_keys = nullptr;
+    uint32_t* hot_n = nullptr;
+    if (n < (size_t(1) << 22) || !build_hot_table(keys, n, &table, &hot_keys, &hot_n, s))
+      table = nullptr;
     const bool vec = vec_ok(keys, nullptr, v, nullptr) && aligned(inv, 16);
     const unsigned grid = grid_for(n / 4 + 1, 256, 8 * num_cus());
     {
       ProfScope prof("unique_mark", s, double(n) * (4.0 + (v ? 1.0 : 0.0)));
       if (vec)
         hipLaunchKernelGGL(k_mark<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
-                           flag);
+                           flag, table, hot_keys, hot_n);
       else
         hipLaunchKernelGGL(k_mark<false>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n),
-                           kmin, flag);
+                           kmin, flag, table, hot_keys, hot_n);
       NFH_LAUNCH_CHECK();
     }
-    scan_tiles(RankOp{flag, kmin, uniq, ng_d}, span, s);
+    scan_tiles(BitRankOp{flag, words, kmin, uniq, ng_d}, nw, s);
+    if (table != nullptr) {
+      hipLaunchKernelGGL(k_hot_codes, dim3(1), dim3(HOT_MAX), 0, s, hot_keys, hot_n, kmin, words,
+                         hot_code);
+      NFH_LAUNCH_CHECK();
+    }
     {
       ProfScope prof("unique_inv_dense", s, double(n) * (8.0 + (v ? 1.0 : 0.0)));
       if (vec)
-        hipLaunchKernelGGL(k_inv_dense<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n),
-                           kmin, flag, inv);
+        hipLaunchKernelGGL(k_inv<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
+                           words, table, hot_n, hot_code, inv);
       else
-        hipLaunchKernelGGL(k_inv_dense<false>, dim3(grid), dim3(256), 0, s, keys, v,
-                           uint32_t(n), kmin, flag, inv);
+        hipLaunchKernelGGL(k_inv<false>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
+                           words, table, hot_n, hot_code, inv);
       NFH_LAUNCH_CHECK();
     }
     return read_scalar(ng_d, s);

@@ -173,6 +173,13 @@ def _probe():
         _lib, _groups, _why = None, set(), "disabled-by-env"
         return
     try:
+        # torch ships its own libamdhip64 with the same soname; whichever is
+        # loaded first serves the process.  Loading torch's first keeps one
+        # HIP runtime for both (the other order leaves torch without a GPU).
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    try:
         lib = ctypes.CDLL(key[1])
         _groups = bind(lib)
         _lib, _why = lib, "loaded:" + key[1]

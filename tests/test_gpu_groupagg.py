@@ -109,3 +109,73 @@ def test_unique_valid_parity(n, card, wide):
         u, inv = native.unique_valid(keys, valid)
         ru, ri = ox.unique_inverse(keys, valid)
         assert np.array_equal(u, ru) and np.array_equal(inv, ri)
+
+
+@pytest.mark.parametrize("case", ["top_edge", "bottom_edge", "full_words", "zipf", "unaligned"])
+def test_unique_valid_bitmap_edges(case):
+    """Dense-span unique: spans that end mid-word, keys at the int32 limits,
+    fully set bitmap words, skewed keys, and the unaligned (row-at-a-time)
+    path."""
+    rng = np.random.default_rng(7)
+    n = 100_003
+    if case == "top_edge":
+        keys = (I32MAX - rng.integers(0, 1001, n)).astype(np.int32)
+    elif case == "bottom_edge":
+        keys = (I32MIN + rng.integers(0, 77, n)).astype(np.int32)
+    elif case == "full_words":
+        keys = rng.permutation(np.repeat(np.arange(-4096, 4096, dtype=np.int32), 13))[:n]
+    elif case == "zipf":
+        keys = ((rng.zipf(1.1, n) % (1 << 20)) * 7919 % (1 << 20) - 5).astype(np.int32)
+    else:
+        keys = rng.integers(-50_000, 50_000, n).astype(np.int32)
+    for valid in (None, rng.random(keys.size) > 0.3):
+        ru, ri = ox.unique_inverse(keys, valid)
+        if case == "unaligned":
+            u, inv = _unique_device_offset(keys, valid)
+        else:
+            u, inv = native.unique_valid(keys, valid)
+        assert np.array_equal(u, ru) and np.array_equal(inv, ri), case
+
+
+@pytest.mark.parametrize("vec", [True, False])
+def test_unique_valid_hot_keys(vec):
+    """Skewed keys at a size where the hot keys are resolved in LDS: one hot
+    key has only NULL rows (in the hot table, absent from the result), one
+    has a single valid row, and the rest mix NULL and valid rows."""
+    rng = np.random.default_rng(11)
+    n = (1 << 22) + 5
+    keys = zipf_keys(rng, n, 1 << 20, 1.2) - 3
+    hot = np.bincount(keys + 3).argsort()[::-1][:3] - 3
+    valid = rng.random(n) > 0.1
+    valid[keys == hot[0]] = False
+    rows1 = np.flatnonzero(keys == hot[1])
+    valid[rows1] = False
+    valid[rows1[-1]] = True
+    ru, ri = ox.unique_inverse(keys, valid)
+    if vec:
+        u, inv = native.unique_valid(keys, valid)
+    else:
+        u, inv = _unique_device_offset(keys, valid)
+    assert hot[0] not in u and hot[1] in u
+    assert np.array_equal(u, ru) and np.array_equal(inv, ri)
+
+
+def _unique_device_offset(keys, valid):
+    """nfh_unique_valid_i32 on device buffers that start one element past a
+    16-byte boundary, so the row-at-a-time kernels run."""
+    import torch
+    n = keys.size
+    k = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    k[1:] = torch.from_numpy(keys).cuda()
+    m = None
+    if valid is not None:
+        m = torch.empty(n + 1, dtype=torch.uint8, device="cuda")
+        m[1:] = torch.from_numpy(valid.astype(np.uint8)).cuda()
+    uniq = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    inv = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    ng = native.lib("hip").nfh_unique_valid_i32(
+        k[1:].data_ptr(), None if m is None else m[1:].data_ptr(), n, uniq[1:].data_ptr(),
+        inv[1:].data_ptr())
+    torch.cuda.synchronize()
+    assert ng >= 0
+    return uniq[1:1 + ng].cpu().numpy(), inv[1:].cpu().numpy()
