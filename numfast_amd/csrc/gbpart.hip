@@ -19,6 +19,7 @@
 
 #include "device.hpp"
 #include "gbpart.hpp"
+#include "hot.hpp"
 #include "runtime.hpp"
 #include "sort.hpp"
 
@@ -37,32 +38,7 @@ constexpr int GP_IPT = 16;
 constexpr int GP_TILE = GP_PBLOCK * GP_IPT;
 constexpr size_t GP_PART_LDS = size_t(GP_TILE) * 8 + GP_WAVES * 128 * 4 + 128 * 4 * 2 +
                                GP_WAVES * 4 + 16;
-constexpr int GP_HOT_MAX = 512;
-constexpr int GP_HOT_TBL = 1024;
-constexpr int GP_HOT_BITS = 10;
-
-__device__ __forceinline__ uint32_t gp_hash(int32_t k) {
-  return (uint32_t(k) * 0x9E3779B1u) >> (32 - GP_HOT_BITS);
-}
-
-// LDS hot table: returns the hot index of k or -1.
-__device__ __forceinline__ int gp_hot_find(const int32_t* t_key, const int32_t* t_idx, int32_t k) {
-  uint32_t h = gp_hash(k);
-  for (;;) {
-    const int32_t idx = t_idx[h];
-    if (idx < 0) return -1;
-    if (t_key[h] == k) return idx;
-    h = (h + 1) & (GP_HOT_TBL - 1);
-  }
-}
-
-__device__ __forceinline__ void gp_load_table(const int2* table, int32_t* t_key, int32_t* t_idx) {
-  for (int b = threadIdx.x; b < GP_HOT_TBL; b += blockDim.x) {
-    const int2 e = table ? table[b] : make_int2(0, -1);
-    t_key[b] = e.x;
-    t_idx[b] = e.y;
-  }
-}
+static_assert(HOT_MAX == 512 && HOT_TBL == 1024, "LDS budget of the passes below");
 
 // ------------------------------------------------------------ histogram
 struct HotAcc {
@@ -82,13 +58,13 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
     const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc,
     bool vec) {
   extern __shared__ uint32_t s_h[];  // nb counters
-  __shared__ int32_t t_key[GP_HOT_TBL], t_idx[GP_HOT_TBL];
-  __shared__ u64 h_sum[GP_HOT_MAX];
-  __shared__ uint32_t h_cnt[GP_HOT_MAX];
-  __shared__ int32_t h_mn[GP_HOT_MAX], h_mx[GP_HOT_MAX];
+  __shared__ int32_t t_key[HOT_TBL], t_idx[HOT_TBL];
+  __shared__ u64 h_sum[HOT_MAX];
+  __shared__ uint32_t h_cnt[HOT_MAX];
+  __shared__ int32_t h_mn[HOT_MAX], h_mx[HOT_MAX];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) s_h[b] = 0;
-  gp_load_table(table, t_key, t_idx);
-  for (int b = threadIdx.x; b < GP_HOT_MAX; b += blockDim.x) {
+  hot_load(table, t_key, t_idx);
+  for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
     h_sum[b] = 0;
     h_cnt[b] = 0;
     h_mn[b] = INT32_MAX;
@@ -97,7 +73,7 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
   __syncthreads();
   auto row = [&](int32_t k, int32_t x, bool ok) {
     if (!ok) return;
-    const int hi = table ? gp_hot_find(t_key, t_idx, k) : -1;
+    const int hi = table ? hot_index(t_key, t_idx, k) : -1;
     if (hi >= 0) {
       atomicAdd(&h_sum[hi], (u64)(long long)x);
       atomicAdd(&h_cnt[hi], 1u);
@@ -147,7 +123,7 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
   if (table) {
-    for (int b = threadIdx.x; b < GP_HOT_MAX; b += blockDim.x) {
+    for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
       const uint32_t c = h_cnt[b];
       if (c == 0) continue;
       atomicAdd(&hot_acc[b].sum, h_sum[b]);
@@ -217,12 +193,12 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
   uint32_t* s_gbase = s_tstart + 128;
   uint32_t* s_tmp = s_gbase + 128;
   uint32_t* s_tile = s_tmp + GP_WAVES;
-  __shared__ int32_t t_key[FIRST ? GP_HOT_TBL : 1], t_idx[FIRST ? GP_HOT_TBL : 1];
+  __shared__ int32_t t_key[FIRST ? HOT_TBL : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
   if (tid == 0) *s_tile = atomicAdd(tile_ctr, 1u);
   for (int i = tid; i < GP_WAVES * 128; i += GP_PBLOCK) s_wh[i >> 7][i & 127] = 0;
-  if constexpr (FIRST) gp_load_table(table, t_key, t_idx);
+  if constexpr (FIRST) hot_load(table, t_key, nullptr);
   __syncthreads();
   const uint32_t tile = *s_tile;
   const uint32_t n = FIRST ? n_rows : n_rec;
@@ -234,7 +210,8 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
   uint32_t live = 0;  // bit j: item j is a record of this pass
   if constexpr (FIRST) {
     // key and mask loads first (independent, coalesced per item), then the
-    // filter; values are loaded at the LDS scatter, for records only
+    // filter, then the value loads of the records only -- all issued before
+    // the ranking so their latency hides behind it
     uint32_t okm = 0;
 #pragma unroll
     for (int j = 0; j < GP_IPT; ++j) {
@@ -247,12 +224,15 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     for (int j = 0; j < GP_IPT; ++j) {
       if ((okm >> j) & 1u) {
         const int32_t k = int32_t(ov[j]);
-        if (!table || gp_hot_find(t_key, t_idx, k) < 0) {
+        if (!table || hot_slot_of(t_key, k) < 0) {
           ov[j] = uint32_t(k - kmin);
           live |= 1u << j;
         }
       }
     }
+#pragma unroll
+    for (int j = 0; j < GP_IPT; ++j)
+      vv[j] = ((live >> j) & 1u) ? vals[wbase + uint32_t(j * WAVE + lane)] : 0;
   }
   if constexpr (!FIRST) {
 #pragma unroll
@@ -312,10 +292,7 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
       const uint32_t d = (ov[j] >> shift) & 127u;
       const uint32_t p = s_tstart[d] + s_wh[wave][d] + rk[j];
       s_o[p] = ov[j];
-      if constexpr (FIRST)
-        s_v[p] = vals[wbase + uint32_t(j * WAVE + lane)];
-      else
-        s_v[p] = vv[j];
+      s_v[p] = vv[j];
     }
   }
   if (tid < 128) {
@@ -416,14 +393,14 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   const int passes = nb <= 1 ? 0 : (nb <= 128 ? 1 : 2);
   // scratch: hist[nb] | bstart[nb+1] | dstart[256] | ctr[2] | hot_acc
   const size_t meta_words = size_t(nb) * 2 + 1 + 256 + 4;
-  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + GP_HOT_MAX * 6);
+  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + HOT_MAX * 6);
   uint32_t* hist = meta;
   uint32_t* bstart = hist + nb;
   uint32_t* dstart = bstart + nb + 1;
   uint32_t* ctr = dstart + 256;
   HotAcc* hot_acc = reinterpret_cast<HotAcc*>(ctr + 4 + (4 - ((meta_words) & 3)));
   fill_u32(meta, 0, meta_words, s);
-  hipLaunchKernelGGL(k_gp_hotinit, dim3(2), dim3(256), 0, s, hot_acc, uint32_t(GP_HOT_MAX));
+  hipLaunchKernelGGL(k_gp_hotinit, dim3(2), dim3(256), 0, s, hot_acc, uint32_t(HOT_MAX));
   NFH_LAUNCH_CHECK();
   const double mask_b = vm ? 1.0 : 0.0;
   {

@@ -26,6 +26,7 @@
 
 #include "device.hpp"
 #include "gbpart.hpp"
+#include "hot.hpp"
 #include "groupagg.hpp"
 #include "runtime.hpp"
 #include "scan.hpp"
@@ -38,18 +39,11 @@ typedef unsigned long long u64;
 
 using GroupSlot = GpSlot;
 
-// The hot-key table layout is shared with gbpart.hip (same hash, sizes).
-constexpr int HOT_MAX = 512;     // hot keys privatised per workgroup
-constexpr int HOT_TBL = 1024;    // LDS open-addressing table (load <= 0.5)
-constexpr int HOT_TBL_BITS = 10;
 constexpr int HOT_SAMPLE = 1 << 16;
 constexpr uint32_t HOT_MIN_COUNT = 4;  // sampled occurrences to qualify
 constexpr int HOT_LEN_BUCKETS = 1024;
 constexpr int ACC_BLOCK = 256;
 
-__device__ __forceinline__ uint32_t hot_hash(int32_t k) {
-  return (uint32_t(k) * 0x9E3779B1u) >> (32 - HOT_TBL_BITS);
-}
 
 // Rows are visited in quads (128-bit key/value loads, 32-bit mask loads)
 // when every pointer is suitably aligned, else one by one.  Each thread
@@ -184,14 +178,12 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int32_t* __restrict__ s
                                                    int2* __restrict__ table,
                                                    uint32_t* __restrict__ hot_n) {
   __shared__ uint32_t hist[HOT_LEN_BUCKETS];
+  __shared__ int32_t s_hot[HOT_MAX];
+  __shared__ unsigned long long cell[HOT_TBL];
   __shared__ int32_t t_idx[HOT_TBL];
   __shared__ int32_t t_key[HOT_TBL];
   __shared__ uint32_t s_thr, s_cnt;
   for (int b = threadIdx.x; b < HOT_LEN_BUCKETS; b += blockDim.x) hist[b] = 0;
-  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) {
-    t_idx[b] = -1;
-    t_key[b] = 0;
-  }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
     const uint32_t L = len[i];
@@ -214,17 +206,16 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int32_t* __restrict__ s
     const uint32_t L = len[i];
     const uint32_t Lb = L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1;
     if (L >= HOT_MIN_COUNT && Lb >= thr) {
-      const int32_t idx = int32_t(atomicAdd(&s_cnt, 1u));
-      const int32_t key = sk[i];
-      hot_keys[idx] = key;
-      uint32_t h = hot_hash(key);
-      while (atomicCAS(&t_idx[h], -1, idx) != -1) h = (h + 1) & (HOT_TBL - 1);
-      t_key[h] = key;
+      const uint32_t idx = atomicAdd(&s_cnt, 1u);
+      hot_keys[idx] = sk[i];
+      s_hot[idx] = sk[i];
     }
   }
   __syncthreads();
+  const uint32_t nh = s_cnt;
+  hot_build(s_hot, nh, cell, t_key, t_idx);
   for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) table[b] = make_int2(t_key[b], t_idx[b]);
-  if (threadIdx.x == 0) *hot_n = s_cnt;
+  if (threadIdx.x == 0) *hot_n = nh;
 }
 
 __device__ __forceinline__ void slot_add(GroupSlot* p, int32_t x) {
@@ -250,11 +241,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void k_group_acc(
   __shared__ int32_t a_mn[HOT_MAX];
   __shared__ int32_t a_mx[HOT_MAX];
   const uint32_t nh = table ? *hot_n : 0u;
-  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) {
-    const int2 e = table ? table[b] : make_int2(0, -1);
-    t_key[b] = e.x;
-    t_idx[b] = e.y;
-  }
+  hot_load(table, t_key, t_idx);
   for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
     a_sum[b] = 0;
     a_cnt[b] = 0;
@@ -265,18 +252,13 @@ __global__ __launch_bounds__(ACC_BLOCK) void k_group_acc(
   auto row = [&](int32_t k, int32_t x, bool ok, uint32_t s) {
     if (!ok) return;
     if (nh != 0) {
-      uint32_t h = hot_hash(k);
-      for (;;) {
-        const int32_t idx = t_idx[h];
-        if (idx < 0) break;
-        if (t_key[h] == k) {
-          atomicAdd(&a_sum[idx], (u64)(long long)x);
-          atomicAdd(&a_cnt[idx], 1u);
-          atomicMin(&a_mn[idx], x);
-          atomicMax(&a_mx[idx], x);
-          return;
-        }
-        h = (h + 1) & (HOT_TBL - 1);
+      const int idx = hot_index(t_key, t_idx, k);
+      if (idx >= 0) {
+        atomicAdd(&a_sum[idx], (u64)(long long)x);
+        atomicAdd(&a_cnt[idx], 1u);
+        atomicMin(&a_mn[idx], x);
+        atomicMax(&a_mx[idx], x);
+        return;
       }
     }
     slot_add(slots + s, x);
@@ -595,29 +577,6 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
 // span / 4 bytes (4 MiB at 2^24 ids).  Keys of the sampled hot-key table
 // are resolved in LDS in both row passes; they stay off the global tables.
 
-// Open-addressing lookup in the LDS copy of the hot-key table: the entry
-// index of k, or -1.
-__device__ __forceinline__ int32_t hot_find(const int32_t* t_key, const int32_t* t_idx,
-                                            int32_t k) {
-  uint32_t h = hot_hash(k);
-  for (;;) {
-    const int32_t idx = t_idx[h];
-    if (idx < 0 || t_key[h] == k) return idx;
-    h = (h + 1) & (HOT_TBL - 1);
-  }
-}
-
-__device__ __forceinline__ uint32_t load_hot_table(const int2* __restrict__ table,
-                                                   const uint32_t* __restrict__ hot_n,
-                                                   int32_t* t_key, int32_t* t_idx) {
-  for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) {
-    const int2 e = table ? table[b] : make_int2(0, -1);
-    t_key[b] = e.x;
-    t_idx[b] = e.y;
-  }
-  return table ? *hot_n : 0u;
-}
-
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
                                               const uint8_t* __restrict__ v, uint32_t n,
@@ -628,13 +587,14 @@ __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
   __shared__ int32_t t_key[HOT_TBL];
   __shared__ int32_t t_idx[HOT_TBL];
   __shared__ uint32_t s_seen[HOT_MAX];
-  const uint32_t nh = load_hot_table(table, hot_n, t_key, t_idx);
+  hot_load(table, t_key, t_idx);
+  const uint32_t nh = table ? *hot_n : 0u;
   for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) s_seen[b] = 0;
   __syncthreads();
   // true when the row still needs the global flag
   auto cold = [&](int32_t k) -> bool {
     if (nh == 0) return true;
-    const int32_t idx = hot_find(t_key, t_idx, k);
+    const int32_t idx = hot_index(t_key, t_idx, k);
     if (idx < 0) return true;
     s_seen[idx] = 1u;
     return false;
@@ -735,13 +695,16 @@ __device__ __forceinline__ int32_t bit_code(const u64* __restrict__ words, uint3
 }
 
 // Dense codes of the hot keys (read by k_inv from LDS).  A hot key with no
-// valid row gets the code of its insertion point, never read.
+// valid row is never looked up; it may even lie outside [kmin, kmin + span)
+// (the sample ignores validity), so only in-range keys read the words.
 __global__ void k_hot_codes(const int32_t* __restrict__ hot_keys,
-                            const uint32_t* __restrict__ hot_n, int32_t kmin,
+                            const uint32_t* __restrict__ hot_n, int32_t kmin, uint32_t span,
                             const u64* __restrict__ words, int32_t* __restrict__ codes) {
   const uint32_t nh = *hot_n;
-  for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x)
-    codes[b] = bit_code(words, uint32_t(hot_keys[b] - kmin));
+  for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x) {
+    const uint32_t o = uint32_t(hot_keys[b]) - uint32_t(kmin);
+    codes[b] = o < span ? bit_code(words, o) : -1;
+  }
 }
 
 template <bool VEC>
@@ -755,13 +718,14 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t* __restrict__ keys,
   __shared__ int32_t t_key[HOT_TBL];
   __shared__ int32_t t_idx[HOT_TBL];
   __shared__ int32_t s_code[HOT_MAX];
-  const uint32_t nh = load_hot_table(table, hot_n, t_key, t_idx);
+  hot_load(table, t_key, t_idx);
+  const uint32_t nh = table ? *hot_n : 0u;
   for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x) s_code[b] = hot_code[b];
   __syncthreads();
   auto code = [&](int32_t k, bool ok) -> int32_t {
     if (!ok) return -1;
     if (nh != 0) {
-      const int32_t idx = hot_find(t_key, t_idx, k);
+      const int32_t idx = hot_index(t_key, t_idx, k);
       if (idx >= 0) return s_code[idx];
     }
     return bit_code(words, uint32_t(k - kmin));
@@ -867,8 +831,8 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
     }
     scan_tiles(BitRankOp{flag, words, kmin, uniq, ng_d}, nw, s);
     if (table != nullptr) {
-      hipLaunchKernelGGL(k_hot_codes, dim3(1), dim3(HOT_MAX), 0, s, hot_keys, hot_n, kmin, words,
-                         hot_code);
+      hipLaunchKernelGGL(k_hot_codes, dim3(1), dim3(HOT_MAX), 0, s, hot_keys, hot_n, kmin,
+                         uint32_t(span), words, hot_code);
       NFH_LAUNCH_CHECK();
     }
     {

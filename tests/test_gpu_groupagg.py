@@ -137,15 +137,19 @@ def test_unique_valid_bitmap_edges(case):
         assert np.array_equal(u, ru) and np.array_equal(inv, ri), case
 
 
-@pytest.mark.parametrize("vec", [True, False])
-def test_unique_valid_hot_keys(vec):
+@pytest.mark.parametrize("vec,outside", [(True, False), (False, False), (True, True)])
+def test_unique_valid_hot_keys(vec, outside):
     """Skewed keys at a size where the hot keys are resolved in LDS: one hot
-    key has only NULL rows (in the hot table, absent from the result), one
-    has a single valid row, and the rest mix NULL and valid rows."""
+    key has only NULL rows (in the hot table, absent from the result; with
+    `outside` it also lies below every valid key), one has a single valid
+    row, and the rest mix NULL and valid rows."""
     rng = np.random.default_rng(11)
     n = (1 << 22) + 5
     keys = zipf_keys(rng, n, 1 << 20, 1.2) - 3
     hot = np.bincount(keys + 3).argsort()[::-1][:3] - 3
+    if outside:
+        keys[keys == hot[0]] = keys.min() - 1000
+        hot[0] = keys.min()
     valid = rng.random(n) > 0.1
     valid[keys == hot[0]] = False
     rows1 = np.flatnonzero(keys == hot[1])
