@@ -47,11 +47,12 @@ constexpr int ACC_BLOCK = 256;
 
 // Rows are visited in quads (128-bit key/value loads, 32-bit mask loads)
 // when every pointer is suitably aligned, else one by one.  Each thread
-// issues the loads of RQ quads before using any of them, so enough bytes
-// are in flight per CU to cover HBM latency.
+// issues the loads of R quads (RQ unless the caller asks for more) before
+// using any of them, so enough bytes are in flight per CU to cover HBM
+// latency.
 constexpr int RQ = 4;
 
-template <bool VEC, class F>
+template <bool VEC, int R = RQ, class F>
 __device__ __forceinline__ void for_each_row(const int32_t* __restrict__ keys,
                                              const int32_t* __restrict__ vals,
                                              const uint8_t* __restrict__ vm,
@@ -62,11 +63,11 @@ __device__ __forceinline__ void for_each_row(const int32_t* __restrict__ keys,
   if constexpr (VEC) {
     const uint32_t nq = n / 4;
     uint32_t q = tid;
-    for (; q + (RQ - 1) * stride < nq; q += RQ * stride) {
-      int4 k4[RQ], v4[RQ];
-      uint32_t a[RQ], b[RQ];
+    for (; q + (R - 1) * stride < nq; q += R * stride) {
+      int4 k4[R], v4[R];
+      uint32_t a[R], b[R];
 #pragma unroll
-      for (int u = 0; u < RQ; ++u) {
+      for (int u = 0; u < R; ++u) {
         const uint32_t qq = q + uint32_t(u) * stride;
         k4[u] = reinterpret_cast<const int4*>(keys)[qq];
         v4[u] = vals ? reinterpret_cast<const int4*>(vals)[qq] : make_int4(0, 0, 0, 0);
@@ -74,7 +75,7 @@ __device__ __forceinline__ void for_each_row(const int32_t* __restrict__ keys,
         b[u] = km ? reinterpret_cast<const uint32_t*>(km)[qq] : 0x01010101u;
       }
 #pragma unroll
-      for (int u = 0; u < RQ; ++u) {
+      for (int u = 0; u < R; ++u) {
         f(k4[u].x, v4[u].x, (a[u] & 0xFFu) && (b[u] & 0xFFu));
         f(k4[u].y, v4[u].y, (a[u] & 0xFF00u) && (b[u] & 0xFF00u));
         f(k4[u].z, v4[u].z, (a[u] & 0xFF0000u) && (b[u] & 0xFF0000u));
@@ -112,7 +113,8 @@ __global__ __launch_bounds__(256) void k_key_range(const int32_t* __restrict__ k
                                                    const uint8_t* __restrict__ kv, uint32_t n,
                                                    int* __restrict__ out /* min, max */) {
   int mn = INT32_MAX, mx = INT32_MIN;
-  for_each_row<VEC>(keys, nullptr, v, kv, n, [&](int32_t k, int32_t, bool ok) {
+  // key-only pass: eight quads in flight per thread
+  for_each_row<VEC, 8>(keys, nullptr, v, kv, n, [&](int32_t k, int32_t, bool ok) {
     if (ok) {
       mn = k < mn ? k : mn;
       mx = k > mx ? k : mx;
