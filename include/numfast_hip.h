@@ -172,7 +172,9 @@ int32_t nfh_cdf_train_i32(const int32_t *keys, size_t n, uint32_t steps, uint32_
 /* Stable ascending argsort through the model's bucket code; the result
  * equals nf_sort_perm_i32(keys, n, 0, ...) for any model.  stats (may be
  * NULL, 4 words): non-empty buckets, largest bucket, buckets over the LDS
- * capacity, buckets re-sorted by the overflow fix-up. */
+ * capacity (each sorted by one workgroup through global memory, or copied
+ * when its keys are all equal), and of those the ones over 2^17 keys with
+ * unequal keys, re-sorted by the plain-sort fix-up. */
 int32_t nfh_sort_perm_learned_i32(const int32_t *keys, size_t n, const nfh_cdf_model *model,
                                   int32_t *perm_out, uint64_t *stats);
 /* Mean-squared-error gradient of the network at params[1153] over (x, y),

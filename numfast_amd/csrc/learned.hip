@@ -54,7 +54,6 @@ constexpr int LS_IPT = 10;
 constexpr int LS_CAP = LS_BLOCK * LS_IPT;  // 5120 keys per bucket in LDS
 constexpr int LS_DBITS = 9;                // in-bucket digit width
 constexpr int LS_DIG = 1 << LS_DBITS;
-constexpr int LS_CHUNK = 4096;             // rows per workgroup, large buckets
 
 // ------------------------------------------------------------ code map
 // 32-bit integer arithmetic: with u = clamp(w - wmin, 0, span),
@@ -522,50 +521,219 @@ __global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __r
     }
     __syncthreads();
   }
+  // the bucket's row ids, read coalesced into LDS (s_w is free after the
+  // last pass), then gathered in sorted order
+  for (uint32_t i = uint32_t(tid); i < size; i += LS_BLOCK) s_w[i] = I[lo + i];
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < LS_IPT; ++j) {
     const uint32_t pos = wb + uint32_t(j * WAVE + lane);
-    if (pos < size) perm[lo + pos] = int32_t(I[lo + lp[j]]);
+    if (pos < size) perm[lo + pos] = int32_t(s_w[lp[j]]);
   }
 }
 
-// Large buckets, LS_CHUNK rows per workgroup: rows copied in their current
-// (stable) order and a flag raised when a bucket holds two different keys.
-// seg: per large bucket {lo, hi}; cstart: chunk prefix over the buckets.
+// Over-capacity buckets, first stage: chunk offsets over the large-bucket
+// list (one workgroup), then LS_CHUNK rows per workgroup copied in their
+// current (stable) order, with a flag raised for each bucket holding two
+// different keys.  Buckets of equal keys are then done; k_large_sort sorts
+// the flagged ones.
+constexpr int LS_CHUNK = 4096;
+constexpr int LS_PREP_BLOCK = 1024;
+constexpr int LS_COPY_GRID = 8192;
+
+__global__ __launch_bounds__(LS_PREP_BLOCK) void k_large_prep(
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ large,
+    const uint32_t* __restrict__ nlarge, uint32_t* __restrict__ cst,
+    uint32_t* __restrict__ unequal) {
+  __shared__ uint32_t s_tmp[LS_PREP_BLOCK / WAVE];
+  const uint32_t nl = *nlarge;
+  const uint32_t per = (nl + LS_PREP_BLOCK - 1) / LS_PREP_BLOCK;
+  const uint32_t q0 = threadIdx.x * per;
+  const uint32_t q1 = q0 + per < nl ? q0 + per : nl;
+  uint32_t c = 0;
+  for (uint32_t q = q0; q < q1; ++q) {
+    const uint32_t b = large[q];
+    c += (start[b + 1] - start[b] + LS_CHUNK - 1) / LS_CHUNK;
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<LS_PREP_BLOCK / WAVE>(c, s_tmp, &tot);
+  for (uint32_t q = q0; q < q1; ++q) {
+    const uint32_t b = large[q];
+    cst[q] = run;
+    run += (start[b + 1] - start[b] + LS_CHUNK - 1) / LS_CHUNK;
+    unequal[q] = 0;
+  }
+  if (threadIdx.x == 0) cst[nl] = tot;
+}
+
 __global__ __launch_bounds__(256) void k_large_copy(const uint32_t* __restrict__ W,
                                                     const uint32_t* __restrict__ I,
-                                                    const uint2* __restrict__ seg,
-                                                    const uint32_t* __restrict__ cstart,
-                                                    uint32_t nseg, int32_t* __restrict__ perm,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ large,
+                                                    const uint32_t* __restrict__ nlarge,
+                                                    const uint32_t* __restrict__ cst,
+                                                    int32_t* __restrict__ perm,
                                                     uint32_t* __restrict__ unequal) {
-  __shared__ uint32_t s_seg, s_any;
-  const uint32_t c = blockIdx.x;
-  if (threadIdx.x == 0) {
-    uint32_t lo = 0, hi = nseg;  // last segment with cstart <= c
+  const uint32_t nl = *nlarge;
+  if (nl == 0) return;
+  const uint32_t total = cst[nl];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    uint32_t lo = 0, hi = nl;  // last bucket with cst <= c
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
-      if (cstart[mid] <= c)
+      if (cst[mid] <= c)
         lo = mid;
       else
         hi = mid;
     }
-    s_seg = lo;
-    s_any = 0;
+    const uint32_t q = lo, b = large[q];
+    const uint32_t s0 = start[b], s1 = start[b + 1];
+    const uint32_t first = s0 + (c - cst[q]) * uint32_t(LS_CHUNK);
+    const uint32_t last = first + uint32_t(LS_CHUNK) < s1 ? first + uint32_t(LS_CHUNK) : s1;
+    const uint32_t w0 = W[s0];
+    bool diff = false;
+    for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
+      perm[i] = int32_t(I[i]);
+      diff |= W[i] != w0;
+    }
+    if (__ballot(diff) != 0 && lane_id() == 0) unequal[q] = 1u;
   }
-  __syncthreads();
-  const uint32_t k = s_seg;
-  const uint2 sg = seg[k];
-  const uint32_t first = sg.x + (c - cstart[k]) * uint32_t(LS_CHUNK);
-  const uint32_t last = first + uint32_t(LS_CHUNK) < sg.y ? first + uint32_t(LS_CHUNK) : sg.y;
-  const uint32_t w0 = W[sg.x];
-  bool diff = false;
-  for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
-    perm[i] = int32_t(I[i]);
-    diff |= W[i] != w0;
+}
+
+// Over-capacity buckets holding different keys, one workgroup each:
+//   * at most LS_LARGE_MAX keys: a stable LSD sort of w - min with 8-bit
+//     digits through global memory (the bucket's own range of the free
+//     ping-pong buffers).  Each pass counts the digits, then ranks chunks of
+//     LS_CAP keys in order (per-wave lane-ordered LDS counters, as in the
+//     in-LDS sort) on top of running digit offsets; the last pass writes
+//     the permutation;
+//   * larger and unequal: listed in `huge` for the host's plain radix sort.
+constexpr uint32_t LS_LARGE_MAX = 1u << 17;
+constexpr int LS_LARGE_GRID = 1024;
+
+__global__ __launch_bounds__(LS_BLOCK) void k_large_sort(
+    uint32_t* __restrict__ W, uint32_t* __restrict__ I, uint32_t* __restrict__ TW,
+    uint32_t* __restrict__ TI, const uint32_t* __restrict__ start,
+    const uint32_t* __restrict__ large, const uint32_t* __restrict__ nlarge,
+    const uint32_t* __restrict__ unequal, int32_t* __restrict__ perm, uint32_t* __restrict__ huge,
+    uint32_t* __restrict__ nhuge) {
+  __shared__ uint32_t s_cnt[LS_WAVES][256];
+  __shared__ uint32_t s_hist[256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_dst[256];
+  __shared__ uint32_t s_tmp[2 * LS_WAVES];
+  const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+  const uint32_t nl = *nlarge;
+  for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    if (!unequal[q]) continue;  // equal keys: copied by k_large_copy
+    const uint32_t b = large[q];
+    const uint32_t lo = start[b], m = start[b + 1] - lo;
+    if (m > LS_LARGE_MAX) {
+      if (threadIdx.x == 0) huge[atomicAdd(nhuge, 1u)] = b;
+      continue;
+    }
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    for (uint32_t i = uint32_t(tid); i < m; i += LS_BLOCK) {
+      const uint32_t w = W[lo + i];
+      mn = w < mn ? w : mn;
+      mx = w > mx ? w : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t a = __shfl_xor(mn, off), c = __shfl_xor(mx, off);
+      mn = a < mn ? a : mn;
+      mx = c > mx ? c : mx;
+    }
+    if (lane == 0) {
+      s_tmp[wave] = mn;
+      s_tmp[LS_WAVES + wave] = mx;
+    }
+    __syncthreads();
+    mn = 0xFFFFFFFFu;
+    mx = 0;
+#pragma unroll
+    for (int w = 0; w < LS_WAVES; ++w) {
+      mn = s_tmp[w] < mn ? s_tmp[w] : mn;
+      mx = s_tmp[LS_WAVES + w] > mx ? s_tmp[LS_WAVES + w] : mx;
+    }
+    __syncthreads();
+    const int bits = 32 - __clz(mx - mn);  // mx > mn: the bucket is flagged
+    const int passes = (bits + 7) / 8;
+    uint32_t* sw = W + lo;
+    uint32_t* si = I + lo;
+    uint32_t* dw = TW + lo;
+    uint32_t* di = TI + lo;
+    for (int p = 0; p < passes; ++p) {
+      const int shift = 8 * p;
+      const bool last = p == passes - 1;
+      if (tid < 256) s_hist[tid] = 0;
+      __syncthreads();
+      for (uint32_t i = uint32_t(tid); i < m; i += LS_BLOCK)
+        atomicAdd(&s_hist[((sw[i] - mn) >> shift) & 255u], 1u);
+      __syncthreads();
+      {
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<LS_WAVES>(tid < 256 ? s_hist[tid] : 0u, s_tmp, &tot);
+        if (tid < 256) s_run[tid] = ex;
+      }
+      __syncthreads();
+      for (uint32_t c0 = 0; c0 < m; c0 += uint32_t(LS_CAP)) {
+        for (int i = tid; i < LS_WAVES * 256; i += LS_BLOCK) (&s_cnt[0][0])[i] = 0;
+        __syncthreads();
+        const uint32_t wb = c0 + uint32_t(wave) * (WAVE * LS_IPT);
+        uint32_t kw[LS_IPT], kr[LS_IPT], rk[LS_IPT];
+#pragma unroll
+        for (int j = 0; j < LS_IPT; ++j) {
+          const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+          const bool in = pos < m;
+          kw[j] = in ? sw[pos] : 0u;
+          kr[j] = in ? si[pos] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < LS_IPT; ++j) {
+          const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+          const uint32_t d = ((kw[j] - mn) >> shift) & 255u;
+          const uint32_t r = pos < m ? atomicAdd(&s_cnt[wave][d], 1u) : 0u;
+          rk[j] = (r << 8) | d;
+        }
+        __syncthreads();
+        if (tid < 256) {
+          uint32_t c = 0;
+#pragma unroll
+          for (int w = 0; w < LS_WAVES; ++w) {
+            const uint32_t x = s_cnt[w][tid];
+            s_cnt[w][tid] = c;
+            c += x;
+          }
+          s_dst[tid] = s_run[tid];
+          s_run[tid] += c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < LS_IPT; ++j) {
+          const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+          if (pos < m) {
+            const uint32_t d = rk[j] & 255u;
+            const uint32_t g = s_dst[d] + s_cnt[wave][d] + (rk[j] >> 8);
+            if (last) {
+              perm[lo + g] = int32_t(kr[j]);
+            } else {
+              dw[g] = kw[j];
+              di[g] = kr[j];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // the next pass reads what this one wrote: the barrier that ended the
+      // last chunk orders those global writes before its loads
+      uint32_t* t = sw;
+      sw = dw;
+      dw = t;
+      t = si;
+      si = di;
+      di = t;
+    }
   }
-  if (__ballot(diff) != 0 && lane_id() == 0) s_any = 1;
-  __syncthreads();
-  if (threadIdx.x == 0 && s_any) unequal[k] = 1;
 }
 
 __global__ void k_seg_keys(const uint32_t* __restrict__ W, uint32_t lo, uint32_t m,
@@ -603,8 +771,10 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   }
   if (stats)
     for (int i = 0; i < 4; ++i) stats[i] = 0;
-  if (n < (1u << 14) || hm.span == 0) {
-    // too small to bucket, or a degenerate model: the plain sort
+  if (n < (1u << 14) || hm.span == 0 || !lds_rank_supported()) {
+    // too small to bucket, a degenerate model, or a device whose LDS
+    // atomics do not return in lane order (the in-bucket ranks rely on
+    // it): the plain sort
     uint32_t* t0 = scratch_t<uint32_t>(S_LSEG1, n);
     uint32_t* t1 = scratch_t<uint32_t>(S_LSEG2, n);
     int32_t* tp = scratch_t<int32_t>(S_LSEG3, n);
@@ -628,13 +798,18 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   map.shl = uint32_t(__builtin_clz(hm.span));
   map.mul = uint32_t((uint64_t(1) << 53) / ((uint64_t(hm.span) + 1u) << map.shl));
   map.maxcode = B - 1;
-  // code histograms and bin starts
-  uint32_t* meta = scratch_t<uint32_t>(S_LBUCKET, 1024 + 4 + (B + 1) + 2 * 4096);
+  // code histograms and bin starts; the large-bucket lists hold up to B
+  // entries each (at most one per bucket)
+  uint32_t* meta =
+      scratch_t<uint32_t>(S_LBUCKET, 1024 + 4 + (B + 1) + 2 * size_t(B) + 2 * (size_t(B) + 1));
   uint32_t* hist = meta;
   uint32_t* bins = meta + 512;
-  uint32_t* ctr = meta + 1024;  // 2 tile counters, nlarge
+  uint32_t* ctr = meta + 1024;  // 2 tile counters, large buckets, huge buckets
   uint32_t* bstart = ctr + 4;
   uint32_t* large = bstart + (B + 1);
+  uint32_t* huge = large + B;
+  uint32_t* cst = huge + B;
+  uint32_t* unequal = cst + (B + 1);
   fill_u32(meta, 0, 1024 + 4, s);
   {
     ProfScope prof("learned_code_hist", s, double(n) * 4.0);
@@ -673,11 +848,38 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
                        ctr + 2);
     NFH_LAUNCH_CHECK();
   }
-  const uint32_t nl = read_scalar(ctr + 2, s);
-  if (stats) {
-    std::vector<uint32_t> st(B + 1);
+  // over-capacity buckets on the device: copied in chunks (equal keys are
+  // then done), the others sorted by one workgroup each; the free
+  // ping-pong pair is the temporary of those global-memory passes
+  hipLaunchKernelGGL(k_large_prep, dim3(1), dim3(LS_PREP_BLOCK), 0, s, bstart, large, ctr + 2,
+                     cst, unequal);
+  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("learned_large_copy", s);
+    hipLaunchKernelGGL(k_large_copy, dim3(LS_COPY_GRID), dim3(256), 0, s, W, I, bstart, large,
+                       ctr + 2, cst, perm, unequal);
+    NFH_LAUNCH_CHECK();
+  }
+  {
+    ProfScope prof("learned_large_sort", s);
+    uint32_t* tw = W == w1 ? w0 : w1;
+    uint32_t* ti = W == w1 ? i0 : i1;
+    hipLaunchKernelGGL(k_large_sort, dim3(LS_LARGE_GRID), dim3(LS_BLOCK), 0, s,
+                       const_cast<uint32_t*>(W), const_cast<uint32_t*>(I), tw, ti, bstart, large,
+                       ctr + 2, unequal, perm, huge, ctr + 3);
+    NFH_LAUNCH_CHECK();
+  }
+  uint32_t counts[2];
+  NFH_CHECK(hipMemcpyAsync(counts, ctr + 2, 8, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipStreamSynchronize(s));
+  const uint32_t nl = counts[0], nh = counts[1];
+  std::vector<uint32_t> st;
+  if (stats || nh > 0) {
+    st.resize(B + 1);
     NFH_CHECK(hipMemcpyAsync(st.data(), bstart, (B + 1) * 4, hipMemcpyDeviceToHost, s));
     NFH_CHECK(hipStreamSynchronize(s));
+  }
+  if (stats) {
     uint64_t nonempty = 0, mx = 0;
     for (uint32_t b = 0; b < B; ++b) {
       const uint64_t sz = st[b + 1] - st[b];
@@ -687,71 +889,41 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
     stats[0] = nonempty;
     stats[1] = mx;
     stats[2] = nl;
+    stats[3] = nh;
   }
-  if (nl == 0) return;
-  // large buckets: copy rows, flag the ones holding different keys
-  std::vector<uint32_t> lst(nl), st(B + 1);
-  NFH_CHECK(hipMemcpyAsync(lst.data(), large, nl * 4, hipMemcpyDeviceToHost, s));
-  NFH_CHECK(hipMemcpyAsync(st.data(), bstart, (B + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (nh == 0) return;
+  // overflow fix-up for the buckets too large for one workgroup (and not
+  // all equal): they are concatenated and sorted by one plain radix
+  // argsort.  Buckets cover disjoint, increasing key ranges and keep their
+  // rows in input order, so the stable order of the concatenation is each
+  // bucket's stable order.
+  std::vector<uint32_t> lst(nh);
+  NFH_CHECK(hipMemcpyAsync(lst.data(), huge, nh * 4, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipStreamSynchronize(s));
   std::sort(lst.begin(), lst.end());
-  std::vector<uint32_t> segs(2 * nl), cst(nl + 1);
-  uint32_t chunks = 0;
-  for (uint32_t k = 0; k < nl; ++k) {
-    segs[2 * k] = st[lst[k]];
-    segs[2 * k + 1] = st[lst[k] + 1];
-    cst[k] = chunks;
-    chunks += (segs[2 * k + 1] - segs[2 * k] + LS_CHUNK - 1) / LS_CHUNK;
-  }
-  cst[nl] = chunks;
-  uint32_t* dseg = scratch_t<uint32_t>(S_LSEG0, 2 * nl + (nl + 1) + nl);
-  uint32_t* dcst = dseg + 2 * nl;
-  uint32_t* dflag = dcst + (nl + 1);
-  NFH_CHECK(hipMemcpyAsync(dseg, segs.data(), 2 * nl * 4, hipMemcpyHostToDevice, s));
-  NFH_CHECK(hipMemcpyAsync(dcst, cst.data(), (nl + 1) * 4, hipMemcpyHostToDevice, s));
-  fill_u32(dflag, 0, nl, s);
-  {
-    ProfScope prof("learned_large_copy", s);
-    hipLaunchKernelGGL(k_large_copy, dim3(chunks), dim3(256), 0, s, W, I,
-                       reinterpret_cast<const uint2*>(dseg), dcst, nl, perm, dflag);
-    NFH_LAUNCH_CHECK();
-  }
-  std::vector<uint32_t> flag(nl);
-  NFH_CHECK(hipMemcpyAsync(flag.data(), dflag, nl * 4, hipMemcpyDeviceToHost, s));
-  NFH_CHECK(hipStreamSynchronize(s));
-  // overflow fix-up: the buckets holding different keys are concatenated
-  // and sorted by one plain radix argsort.  Buckets cover disjoint,
-  // increasing key ranges and keep their rows in input order, so the
-  // stable order of the concatenation is each bucket's stable order.
-  uint64_t fixed = 0;
   uint32_t total = 0;
   std::vector<uint32_t> fseg;  // {lo, m, offset in the concatenation}
-  for (uint32_t k = 0; k < nl; ++k) {
-    if (!flag[k]) continue;
-    const uint32_t lo = segs[2 * k], m = segs[2 * k + 1] - lo;
+  for (uint32_t k = 0; k < nh; ++k) {
+    const uint32_t lo = st[lst[k]], m = st[lst[k] + 1] - lo;
     fseg.insert(fseg.end(), {lo, m, total});
     total += m;
-    ++fixed;
   }
-  if (total > 0) {
-    int32_t* sk = scratch_t<int32_t>(S_LSEG1, total);
-    int32_t* sp = scratch_t<int32_t>(S_LSEG2, total);
-    uint32_t* t0 = scratch_t<uint32_t>(S_LSEG3, total);
-    uint32_t* t1 = scratch_t<uint32_t>(S_LSEG4, total);
-    int32_t* tp = scratch_t<int32_t>(S_LTRAIN, total);
-    for (size_t q = 0; q < fseg.size(); q += 3) {
-      hipLaunchKernelGGL(k_seg_keys, dim3(grid_for(fseg[q + 1], 256, 4096)), dim3(256), 0, s, W,
-                         fseg[q], fseg[q + 1], sk + fseg[q + 2]);
-      NFH_LAUNCH_CHECK();
-    }
-    radix_argsort<int32_t>(sk, total, false, sp, t0, t1, tp, nullptr, s);
-    for (size_t q = 0; q < fseg.size(); q += 3) {
-      hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(fseg[q + 1], 256, 4096)), dim3(256), 0, s,
-                         sp + fseg[q + 2], I, fseg[q], fseg[q + 1], fseg[q + 2], perm);
-      NFH_LAUNCH_CHECK();
-    }
+  int32_t* sk = scratch_t<int32_t>(S_LSEG1, total);
+  int32_t* sp = scratch_t<int32_t>(S_LSEG2, total);
+  uint32_t* t0 = scratch_t<uint32_t>(S_LSEG3, total);
+  uint32_t* t1 = scratch_t<uint32_t>(S_LSEG4, total);
+  int32_t* tp = scratch_t<int32_t>(S_LTRAIN, total);
+  for (size_t q = 0; q < fseg.size(); q += 3) {
+    hipLaunchKernelGGL(k_seg_keys, dim3(grid_for(fseg[q + 1], 256, 4096)), dim3(256), 0, s, W,
+                       fseg[q], fseg[q + 1], sk + fseg[q + 2]);
+    NFH_LAUNCH_CHECK();
   }
-  if (stats) stats[3] = fixed;
+  radix_argsort<int32_t>(sk, total, false, sp, t0, t1, tp, nullptr, s);
+  for (size_t q = 0; q < fseg.size(); q += 3) {
+    hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(fseg[q + 1], 256, 4096)), dim3(256), 0, s,
+                       sp + fseg[q + 2], I, fseg[q], fseg[q + 1], fseg[q + 2], perm);
+    NFH_LAUNCH_CHECK();
+  }
 }
 
 // ------------------------------------------------------------ host: training

@@ -684,7 +684,8 @@ def cdf_train(keys, steps=100, linear_start=False):
 
 def sort_perm_learned(keys, model):
     """Stable ascending argsort via the learned-CDF bucket sort; returns
-    (perm, stats) with stats = (buckets, largest, over capacity, fixed)."""
+    (perm, stats) with stats = (buckets, largest, over the LDS capacity,
+    re-sorted by the plain-sort fix-up)."""
     keys = _c(keys, np.int32)
     perm = np.zeros(keys.size, np.int32)
     stats = np.zeros(4, np.uint64)

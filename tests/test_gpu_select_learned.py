@@ -136,6 +136,22 @@ def test_learned_sort_mismatched_model_uses_fixup(hip, kind):
     assert stats[2] > 0
 
 
+def test_learned_sort_large_bucket_paths(hip):
+    """Over-capacity buckets of both sizes: one of about 10^4 unequal keys
+    (sorted by one workgroup in global memory), one of 3 * 10^5 unequal keys
+    (the plain-sort fix-up) and one of equal keys (copied)."""
+    rng = np.random.default_rng(5)
+    base = _dist("uniform", 1 << 20, rng)
+    model, _ = native.cdf_train(base, steps=20)
+    k = np.concatenate([base, (np.arange(300_000) + 12_345).astype(np.int32),
+                        rng.integers(1 << 30, (1 << 30) + 1_000_000, 8000).astype(np.int32),
+                        np.full(20_000, -(1 << 29), np.int32)])
+    k = k[rng.permutation(k.size)]
+    perm, stats = native.sort_perm_learned(k, model)
+    assert np.array_equal(perm, np.argsort(k, kind="stable"))
+    assert stats[3] >= 1 and stats[2] >= stats[3] + 2, stats
+
+
 def test_learned_sort_garbage_model_is_exact(hip):
     rng = np.random.default_rng(2)
     k = _dist("uniform", 1 << 18, rng)
