@@ -787,9 +787,12 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   float* dP = scratch_t<float>(S_LMODEL, MLP_P + KNOTS + 8);
   uint32_t* knots = reinterpret_cast<uint32_t*>(dP + MLP_P + 3);
   NFH_CHECK(hipMemcpyAsync(dP, hm.p, sizeof(float) * MLP_P, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_cdf_knots, dim3(1), dim3(1024), 0, s, dP, float(B) * float(1 << KN_UNIT),
-                     knots);
-  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("learned_knots", s);
+    hipLaunchKernelGGL(k_cdf_knots, dim3(1), dim3(1024), 0, s, dP, float(B) * float(1 << KN_UNIT),
+                       knots);
+    NFH_LAUNCH_CHECK();
+  }
   LearnedMap map;
   map.knots = knots;
   map.wmin = hm.wmin;
@@ -817,8 +820,11 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
                        n, map, hist);
     NFH_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_bins2, dim3(2), dim3(256), 0, s, hist, bins);
-  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("learned_bins", s);
+    hipLaunchKernelGGL(k_bins2, dim3(2), dim3(256), 0, s, hist, bins);
+    NFH_LAUNCH_CHECK();
+  }
   // stable code passes: low byte (reads the raw keys), then high byte
   uint32_t* w0 = scratch_t<uint32_t>(S_L0, n);
   uint32_t* i0 = scratch_t<uint32_t>(S_L1, n);
@@ -839,9 +845,12 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
     W = w1;
     I = i1;
   }
-  hipLaunchKernelGGL(k_bucket_bounds, dim3((B + 1 + 255) / 256), dim3(256), 0, s, W, n, map, B,
-                     bstart);
-  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("learned_bounds", s);
+    hipLaunchKernelGGL(k_bucket_bounds, dim3((B + 1 + 255) / 256), dim3(256), 0, s, W, n, map, B,
+                       bstart);
+    NFH_LAUNCH_CHECK();
+  }
   {
     ProfScope prof("learned_bucket_sort", s, double(n) * 12.0);
     hipLaunchKernelGGL(k_bucket_sort, dim3(B), dim3(LS_BLOCK), 0, s, W, I, bstart, perm, large,
@@ -851,9 +860,12 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   // over-capacity buckets on the device: copied in chunks (equal keys are
   // then done), the others sorted by one workgroup each; the free
   // ping-pong pair is the temporary of those global-memory passes
-  hipLaunchKernelGGL(k_large_prep, dim3(1), dim3(LS_PREP_BLOCK), 0, s, bstart, large, ctr + 2,
-                     cst, unequal);
-  NFH_LAUNCH_CHECK();
+  {
+    ProfScope prof("learned_large_prep", s);
+    hipLaunchKernelGGL(k_large_prep, dim3(1), dim3(LS_PREP_BLOCK), 0, s, bstart, large, ctr + 2,
+                       cst, unequal);
+    NFH_LAUNCH_CHECK();
+  }
   {
     ProfScope prof("learned_large_copy", s);
     hipLaunchKernelGGL(k_large_copy, dim3(LS_COPY_GRID), dim3(256), 0, s, W, I, bstart, large,
