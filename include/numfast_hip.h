@@ -115,6 +115,39 @@ int32_t nf_mask_and(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out);
 int32_t nf_mask_or(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out);
 int32_t nf_mask_not(const uint8_t *a, size_t n, uint8_t *out);
 
+/* ---- partitioned pair-hash (grouped COUNT DISTINCT):
+ *      lib.rs:2266-2470, groupby/hash_grouped.rs; bound by
+ *      src/Drivers/GroupedHashMT/_lib/grouped_native_hash.py:95-128 ----
+ * int64 lanes; p a power of two in [1, 256], t in [1, 64]; nthreads sets
+ * the reference's sharding, which the outputs of count / scatter /
+ * occ_fill follow exactly.  0 ok, -1 null pointer, -3 bad geometry; -2
+ * where the reference would index out of bounds and -3 where its probe
+ * would not end (a table with no room for a new pair). */
+/* lib.rs:2266-2287 -- fp[i] = (k*C1 + (v ^ v>>33)*C2) & pmask (wrapping). */
+int32_t nf_ghash_fp_i64(const int64_t *k, const int64_t *v, size_t n, int64_t *fp,
+                        int64_t pmask, size_t nthreads);
+/* lib.rs:2299-2316 -- tc[tt*p + q] += rows of row chunk tt with fp == q. */
+int32_t nf_ghash_count_i64(const int64_t *fp, size_t n, int64_t *tc, size_t t, size_t p,
+                           size_t nthreads);
+/* lib.rs:2334-2360 -- chunk tt writes its rows in row order from the
+ * cursors tpos[tt*p + q] into pk / pv. */
+int32_t nf_ghash_scatter_i64(const int64_t *k, const int64_t *v, const int64_t *fp, size_t n,
+                             const int64_t *tpos, size_t t, size_t p, int64_t *pk, int64_t *pv,
+                             size_t nthreads);
+/* lib.rs:2374-2409 -- partition q's rows [starts[q], starts[q+1]) inserted
+ * in row order into its linear-probing table of tm[q]+1 slots at toff[q]
+ * (used zeroed by the caller); the same layout as sequential insertion. */
+int32_t nf_ghash_pins_i64(const int64_t *pk, const int64_t *pv, const int64_t *starts,
+                          const int64_t *toff, const int64_t *tm, int64_t *tk, int64_t *tv,
+                          uint8_t *used, size_t p, size_t nthreads);
+/* lib.rs:2419-2438 -- cnt[q] = used slots of partition q. */
+int32_t nf_ghash_occ_count(const uint8_t *used, const int64_t *toff, size_t p, int64_t *cnt,
+                           size_t nthreads);
+/* lib.rs:2448-2470 -- occupied keys in slot order, per shard from ok's
+ * consecutive pieces of ostart[hi] - ostart[lo]. */
+int32_t nf_ghash_occ_fill(const int64_t *tk, const uint8_t *used, const int64_t *toff,
+                          const int64_t *ostart, int64_t *ok, size_t p, size_t nthreads);
+
 /* ---- additive surface (no reference counterpart) --------------------- */
 
 /* Order this thread's calls on a caller stream (hipStream_t); NULL

@@ -88,6 +88,16 @@ GROUPS = {
         ("nf_mask_or", [_vp, _vp, _sz, _vp], _i32),
         ("nf_mask_not", [_vp, _sz, _vp], _i32),
     ],
+    # src/Drivers/GroupedHashMT/_lib/grouped_native_hash.py:95-128
+    # (partitioned pair-hash for grouped COUNT DISTINCT, lib.rs:2266-2470).
+    "ghash": [
+        ("nf_ghash_fp_i64", [_vp, _vp, _sz, _vp, _i64, _sz], _i32),
+        ("nf_ghash_count_i64", [_vp, _sz, _vp, _sz, _sz, _sz], _i32),
+        ("nf_ghash_scatter_i64", [_vp, _vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _sz], _i32),
+        ("nf_ghash_pins_i64", [_vp] * 8 + [_sz, _sz], _i32),
+        ("nf_ghash_occ_count", [_vp, _vp, _sz, _vp, _sz], _i32),
+        ("nf_ghash_occ_fill", [_vp] * 5 + [_sz, _sz], _i32),
+    ],
     # Additive: learned-CDF sort, its training, the sort pipeline.
     "learned": [
         ("nfh_cdf_train_i32", [_vp, _sz, _u32, _u32, _vp, _vp], _i32),

@@ -614,3 +614,171 @@ EXPORT int32_t nf_mask_not(const uint8_t *a, size_t n, uint8_t *out) {
     for (size_t i = 0; i < n; i++) out[i] = (uint8_t)(a[i] == 0);
     return NF_OK;
 }
+
+/* ------------------------------------------------------------------ ghash
+ * Partitioned pair-hash for grouped COUNT DISTINCT
+ * (numfast-native/src/groupby/hash_grouped.rs; FFI lib.rs:2266-2470).
+ * int64 lanes.  The reference shards rows (fp, count, scatter) or
+ * partitions (pins, occ_count, occ_fill) over `nthreads` threads; the
+ * restatement runs the shards one after another, with the same chunk
+ * boundaries, which is what makes count / scatter / occ_fill outputs
+ * depend on nthreads exactly as the reference's do.
+ * Refusals where the reference would index out of bounds (panic, abort)
+ * or probe forever: -2 for a partition id or reservation outside its
+ * buffer, -3 for a table too small for its distinct pairs. */
+#define GH_PART_MAX 256
+#define GH_THREADS_MAX 64
+
+/* hash_grouped.rs:55-68 */
+static uint64_t gh_pair_fp(int64_t k, int64_t v) {
+  const uint64_t uk = (uint64_t)k, uv = (uint64_t)v;
+  return uk * 0x9E3779B97F4A7C15ull + (uv ^ (uv >> 33)) * 0x00BF58476D1FF973ull;
+}
+
+/* hash_grouped.rs:70-74: clamp to [1, 64], capped by the work items */
+static size_t gh_fanout(size_t nthreads, size_t work) {
+  size_t t = nthreads < 1 ? 1 : (nthreads > GH_THREADS_MAX ? GH_THREADS_MAX : nthreads);
+  const size_t w = work < 1 ? 1 : work;
+  return t < w ? t : w;
+}
+
+static int gh_pow2(size_t p) { return p != 0 && p <= GH_PART_MAX && (p & (p - 1)) == 0; }
+
+/* lib.rs:2266-2287, hash_grouped.rs:78-130 */
+EXPORT int32_t nf_ghash_fp_i64(const int64_t *k, const int64_t *v, size_t n, int64_t *fp,
+                               int64_t pmask, size_t nthreads) {
+  (void)nthreads;
+  if (n == 0) return NF_OK;
+  if (!k || !v || !fp) return NF_NULL_OR_ABORT;
+  for (size_t i = 0; i < n; ++i) fp[i] = (int64_t)(gh_pair_fp(k[i], v[i]) & (uint64_t)pmask);
+  return NF_OK;
+}
+
+/* lib.rs:2299-2316, hash_grouped.rs:136-190: chunk tt of
+ * nt = min(fanout(nthreads, n), t) equal row chunks (ceil) adds its
+ * partition histogram into tc[tt * p + q] */
+EXPORT int32_t nf_ghash_count_i64(const int64_t *fp, size_t n, int64_t *tc, size_t t, size_t p,
+                                  size_t nthreads) {
+  if (n == 0) return NF_OK;
+  if (!fp || !tc) return NF_NULL_OR_ABORT;
+  if (!gh_pow2(p) || t == 0 || t > GH_THREADS_MAX) return NF_MALFORMED;
+  size_t nt = gh_fanout(nthreads, n);
+  if (nt > t) nt = t;
+  const size_t cs = (n + nt - 1) / nt;
+  for (size_t i = 0; i < n; ++i)
+    if (fp[i] < 0 || (uint64_t)fp[i] >= p) return NF_BAD_RANGE;
+  for (size_t tt = 0; tt < nt; ++tt) {
+    const size_t s0 = tt * cs < n ? tt * cs : n, s1 = (tt + 1) * cs < n ? (tt + 1) * cs : n;
+    for (size_t i = s0; i < s1; ++i) tc[tt * p + (size_t)fp[i]] += 1;
+  }
+  return NF_OK;
+}
+
+/* lib.rs:2334-2360, hash_grouped.rs:196-290: chunk tt writes its rows in
+ * row order from the cursor row tpos[tt * p ..] */
+EXPORT int32_t nf_ghash_scatter_i64(const int64_t *k, const int64_t *v, const int64_t *fp,
+                                    size_t n, const int64_t *tpos, size_t t, size_t p,
+                                    int64_t *pk, int64_t *pv, size_t nthreads) {
+  if (n == 0) return NF_OK;
+  if (!k || !v || !fp || !tpos || !pk || !pv) return NF_NULL_OR_ABORT;
+  if (!gh_pow2(p) || t == 0 || t > GH_THREADS_MAX) return NF_MALFORMED;
+  size_t nt = gh_fanout(nthreads, n);
+  if (nt > t) nt = t;
+  const size_t cs = (n + nt - 1) / nt;
+  int64_t cur[GH_PART_MAX];
+  for (size_t tt = 0; tt < nt; ++tt) {
+    const size_t s0 = tt * cs < n ? tt * cs : n, s1 = (tt + 1) * cs < n ? (tt + 1) * cs : n;
+    for (size_t q = 0; q < p; ++q) cur[q] = tpos[tt * p + q];
+    for (size_t i = s0; i < s1; ++i) {
+      if (fp[i] < 0 || (uint64_t)fp[i] >= p) return NF_BAD_RANGE;
+      const int64_t lane = cur[fp[i]]++;
+      if (lane < 0 || (uint64_t)lane >= n) return NF_BAD_RANGE;
+      pk[lane] = k[i];
+      pv[lane] = v[i];
+    }
+  }
+  return NF_OK;
+}
+
+/* lib.rs:2374-2409, hash_grouped.rs:296-440: each partition inserts its
+ * rows, in order, into its linear-probing table (full pair compare) */
+EXPORT int32_t nf_ghash_pins_i64(const int64_t *pk, const int64_t *pv, const int64_t *starts,
+                                 const int64_t *toff, const int64_t *tm, int64_t *tk,
+                                 int64_t *tv, uint8_t *used, size_t p, size_t nthreads) {
+  (void)nthreads;
+  if (!pk || !pv || !starts || !toff || !tm || !tk || !tv || !used) return NF_NULL_OR_ABORT;
+  if (!gh_pow2(p)) return NF_MALFORMED;
+  for (size_t q = 0; q < p; ++q) {
+    const uint64_t slots = (uint64_t)tm[q] + 1;
+    if (tm[q] < 0 || (slots & (slots - 1)) != 0 || starts[q] > starts[q + 1] ||
+        toff[q] + (int64_t)slots > toff[q + 1] || starts[q] < 0 || toff[q] < 0)
+      return NF_MALFORMED;
+  }
+  for (size_t q = 0; q < p; ++q) {
+    const uint64_t mask = (uint64_t)tm[q];
+    const int64_t tbase = toff[q];
+    for (int64_t j = starts[q]; j < starts[q + 1]; ++j) {
+      const int64_t rk = pk[j], vv = pv[j];
+      uint64_t h = gh_pair_fp(rk, vv) & mask;
+      for (uint64_t probes = 0;; ++probes) {
+        /* every slot seen, none free or equal: the reference probes
+         * forever here */
+        if (probes > mask) return NF_MALFORMED;
+        const int64_t s = tbase + (int64_t)h;
+        if (used[s] == 0) {
+          used[s] = 1;
+          tk[s] = rk;
+          tv[s] = vv;
+          break;
+        } else if (tk[s] == rk && tv[s] == vv) {
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+    }
+  }
+  return NF_OK;
+}
+
+/* lib.rs:2419-2438, hash_grouped.rs:460-520 */
+EXPORT int32_t nf_ghash_occ_count(const uint8_t *used, const int64_t *toff, size_t p,
+                                  int64_t *cnt, size_t nthreads) {
+  (void)nthreads;
+  if (!used || !toff || !cnt) return NF_NULL_OR_ABORT;
+  if (!gh_pow2(p)) return NF_MALFORMED;
+  for (size_t q = 0; q < p; ++q) {
+    int64_t c = 0;
+    for (int64_t s = toff[q]; s < toff[q + 1]; ++s) c += used[s] != 0;
+    cnt[q] = c;
+  }
+  return NF_OK;
+}
+
+/* lib.rs:2448-2470, hash_grouped.rs:522-634: ok[0, nocc) is carved into
+ * consecutive pieces, one per shard tt (partitions [tt*cs, ..),
+ * cs = ceil(p / fanout(nthreads, p))) of ostart[hi] - ostart[lo] entries;
+ * each shard writes its partitions' occupied keys, in slot order, into
+ * its piece.  A carve or write past a piece panics in the reference. */
+EXPORT int32_t nf_ghash_occ_fill(const int64_t *tk, const uint8_t *used, const int64_t *toff,
+                                 const int64_t *ostart, int64_t *ok, size_t p, size_t nthreads) {
+  if (!tk || !used || !toff || !ostart || !ok) return NF_NULL_OR_ABORT;
+  if (!gh_pow2(p)) return NF_MALFORMED;
+  const size_t t = gh_fanout(nthreads, p), cs = (p + t - 1) / t;
+  if (ostart[0] < 0) return NF_BAD_RANGE;
+  for (size_t plo = 0; plo < p; plo += cs) {
+    const size_t phi = plo + cs < p ? plo + cs : p;
+    if (ostart[phi] < ostart[plo]) return NF_BAD_RANGE;
+  }
+  for (size_t plo = 0; plo < p; plo += cs) {
+    const size_t phi = plo + cs < p ? plo + cs : p;
+    const int64_t off = ostart[plo] - ostart[0], len = ostart[phi] - ostart[plo];
+    int64_t w = 0;
+    for (size_t q = plo; q < phi; ++q)
+      for (int64_t s = toff[q]; s < toff[q + 1]; ++s)
+        if (used[s] != 0) {
+          if (w >= len) return NF_BAD_RANGE;
+          ok[off + w++] = tk[s];
+        }
+  }
+  return NF_OK;
+}
