@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--rows-log2", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="only the group-by + unique step")
-    ap.add_argument("--legs", default="step,argsort,learned,batch,selector,uniform",
+    ap.add_argument("--legs", default="step,argsort,learned,batch,selector,uniform,ghash",
                     help="comma list of legs to run (for profiling one workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for rocprofv3 "
@@ -154,6 +154,91 @@ def cpu_baseline(n_sample=1 << 22):
                        "/ 2^24-id / 5%%-NULL distribution; C restatement of the reference "
                        "radix unique + NumPy bincount/minimum.at (reference route), median "
                        "of 3" % n_sample)}
+
+
+def ghash_leg(torch, L, n, steps):
+    """Grouped COUNT DISTINCT through the six nf_ghash_* calls on
+    device-resident int64 (key, value) pairs: key uniform over 2^22 ids,
+    value uniform over 64 (about 15 distinct values per key).  The driver's
+    host steps (prefix sums, table geometry) run as torch ops on the device
+    between the calls; the table geometry comes back to the host once per
+    call, as in the reference driver.  CPU baseline: the oracle's C
+    restatement through the same driver on a 2^22-pair sample, one thread."""
+    import numpy as np
+    from numfast_amd import ghash
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    k = torch.randint(0, 1 << 22, (n,), generator=g, device="cuda", dtype=torch.int64)
+    v = torch.randint(0, 64, (n,), generator=g, device="cuda", dtype=torch.int64)
+    P, T = ghash.partitions_for(n), 16
+    fp = torch.empty(n, dtype=torch.int64, device="cuda")
+    pk, pv = torch.empty_like(fp), torch.empty_like(fp)
+    state = {}
+
+    def run():
+        assert L.nf_ghash_fp_i64(k.data_ptr(), v.data_ptr(), n, fp.data_ptr(), P - 1, T) == 0
+        tc = torch.zeros(T * P, dtype=torch.int64, device="cuda")
+        assert L.nf_ghash_count_i64(fp.data_ptr(), n, tc.data_ptr(), T, P, T) == 0
+        col = tc.view(T, P).cumsum(0)
+        counts = col[-1]
+        starts = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
+        starts[1:] = counts.cumsum(0)
+        tpos = torch.empty(T, P, dtype=torch.int64, device="cuda")
+        tpos[0] = starts[:-1]
+        tpos[1:] = starts[:-1] + col[:-1]
+        assert L.nf_ghash_scatter_i64(k.data_ptr(), v.data_ptr(), fp.data_ptr(), n,
+                                      tpos.data_ptr(), T, P, pk.data_ptr(), pv.data_ptr(), T) == 0
+        hc = counts.cpu().numpy()
+        mp = np.array([ghash._next_pow2(max(64, int(np.ceil(int(c) * 1.5)))) for c in hc],
+                      np.int64)
+        toff_h = np.zeros(P + 1, np.int64)
+        toff_h[1:] = np.cumsum(mp)
+        m = int(toff_h[-1])
+        toff = torch.from_numpy(toff_h).cuda()
+        tm = torch.from_numpy(mp - 1).cuda()
+        tk = torch.empty(m, dtype=torch.int64, device="cuda")
+        tv = torch.empty_like(tk)
+        used = torch.zeros(m, dtype=torch.uint8, device="cuda")
+        assert L.nf_ghash_pins_i64(pk.data_ptr(), pv.data_ptr(), starts.data_ptr(),
+                                   toff.data_ptr(), tm.data_ptr(), tk.data_ptr(), tv.data_ptr(),
+                                   used.data_ptr(), P, T) == 0
+        cnt = torch.empty(P, dtype=torch.int64, device="cuda")
+        assert L.nf_ghash_occ_count(used.data_ptr(), toff.data_ptr(), P, cnt.data_ptr(), T) == 0
+        ostart = torch.zeros(P + 1, dtype=torch.int64, device="cuda")
+        ostart[1:] = cnt.cumsum(0)
+        nocc = int(ostart[-1])
+        ok = torch.empty(max(nocc, 1), dtype=torch.int64, device="cuda")
+        assert L.nf_ghash_occ_fill(tk.data_ptr(), used.data_ptr(), toff.data_ptr(),
+                                   ostart.data_ptr(), ok.data_ptr(), P, T) == 0
+        state["nocc"] = nocc
+        state["ok"] = ok[:nocc]
+
+    L.nfh_profile_reset()
+    L.nfh_profile_enable(1)
+    t = timed(torch, run, steps, 1)
+    L.nfh_profile_enable(0)
+    kt = kernel_table(L)
+    # exactness at this size: distinct pairs == the occupied entries
+    code = torch.unique(k * 64 + v)
+    same = int(code.numel()) == state["nocc"]
+    uk = torch.unique(state["ok"])
+    same = same and torch.equal(uk, torch.unique(code // 64))
+    ghk = {a: b for a, b in kt.items() if a.startswith("ghash")}
+    # CPU baseline: the oracle through the same driver, 2^22 pairs
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.executor import load_c_oracle
+    ora, _ = load_c_oracle()
+    ns = 1 << 22
+    ks, vs = k[:ns].cpu().numpy(), v[:ns].cpu().numpy()
+    t0 = time.perf_counter()
+    ghash.grouped_distinct_hash_native(ks, vs, nthreads=1, lib=ora)
+    tc_cpu = time.perf_counter() - t0
+    return {"pairs": n, "partitions": P, "threads_geometry": T, "distinct_pairs": state["nocc"],
+            "ms": round(t * 1e3, 3), "pairs_per_s": n / t, "checked_against_torch_unique": same,
+            "kernels": ghk, "roofline": roofline_of(ghk, steps),
+            "cpu_baseline": {"value": ns / tc_cpu, "unit": "pairs/s", "cores": 1, "kind": "port",
+                             "sample": "first 2^22 pairs, oracle C restatement through the "
+                                       "same driver (nthreads=1)"}}
 
 
 def sort_pipeline_leg(torch, L, W, native, n, steps):
@@ -366,6 +451,9 @@ def main():
         from numfast_amd import selector as SEL
         X, T, ran, _ = SEL.label_columns(torch, native, 400, lo=18, hi=22, seed=7)
         extra["strategy_selector_400col"], _ = SEL.evaluate(native, X, T, ran, steps=400)
+    if "ghash" in legs:
+        # partitioned pair-hash (grouped COUNT DISTINCT), 2^26 int64 pairs
+        extra["ghash_count_distinct_2^26"] = ghash_leg(torch, L, 1 << 26, max(3, args.steps // 2))
     if "uniform" in legs:
         # uniform-2^24 group-by variant
         keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")

@@ -31,6 +31,7 @@
 
 #include "device.hpp"
 #include "runtime.hpp"
+#include "sort.hpp"
 
 namespace nfh {
 
@@ -279,7 +280,9 @@ __device__ __forceinline__ uint32_t ld_holder(const uint32_t* p) {
 }
 
 // 1. dedupe: each distinct pair of a partition ends in one slot of `hold`
-// holding its smallest row id.
+// holding its smallest row id.  Any layout will do here, so the home is
+// the fingerprint's top bits (well mixed), not the reference's low bits
+// (which within a partition repeat its id and make long probe runs).
 __global__ __launch_bounds__(GH_BLOCK) void k_gh_dedupe(const int64_t* __restrict__ pk,
                                                         const int64_t* __restrict__ pv,
                                                         PinsGeom g, uint32_t* __restrict__ hold,
@@ -293,7 +296,8 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_dedupe(const int64_t* __restric
     const uint64_t mask = uint64_t(g.tm[q]);
     uint32_t* tab = hold + g.toff[q];
     const int64_t rk = pk[j], vv = pv[j];
-    uint64_t h = gh_pair_fp(rk, vv) & mask;
+    const int lg = 64 - __clzll((unsigned long long)mask);  // log2(slots)
+    uint64_t h = lg == 0 ? 0 : (gh_pair_fp(rk, vv) >> (64 - lg));
     bool placed = false;
     for (uint64_t probes = 0; probes <= mask; ++probes) {
       uint32_t cur = ld_holder(tab + h);
@@ -321,6 +325,250 @@ __global__ void k_gh_first(const uint32_t* __restrict__ hold, uint64_t m,
   for (uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < m; s += stride) {
     const uint32_t h = hold[s];
     if (h != GH_EMPTY) first[h] = 1;
+  }
+}
+
+// 2. place, in LDS windows.  A partition's table is cut into windows of
+// min(slots, GH_WIN) slots, the grid offset by (q mod slots): the
+// reference's homes in partition q all sit at q modulo P, so windows that
+// start at a home rarely begin inside a cluster.  The first rows are
+// sorted by the window of
+// their reference home (fp & mask), stable in row order.  Each window runs
+// the priority claim in LDS (atomic min of row ids, evicted ids carried
+// on) over its slots plus GH_EXT extension slots; the ids that end in the
+// extension spill into the next window (cyclically), where they start at
+// the first slot: every slot they passed holds a smaller id.  A window of
+// the whole table wraps inside LDS and never spills.  Windows are first
+// solved alone (spill-ins empty); then one workgroup per partition walks
+// its windows in cyclic order from one whose predecessor spilled nothing
+// and re-solves each window whose predecessor spilled, with those spills.
+// A cluster longer than the extension sets the overflow flag, and the
+// partition-free global form (k_gh_place) solves the whole call instead.
+constexpr int GH_WIN = 8192;
+constexpr int GH_EXT = 1024;
+constexpr int GH_WBLOCK = 256;
+
+struct WinGeom {
+  const int64_t* toff;       // p + 1
+  const int64_t* tm;         // p
+  const uint32_t* winbase;   // p + 1: first window of each partition
+  uint32_t p;
+  uint32_t nwin;
+};
+
+__device__ __forceinline__ uint32_t gh_win_part(const uint32_t* s_wb, uint32_t p, uint32_t w) {
+  uint32_t lo = 0, hi = p;  // last q with winbase[q] <= w
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (s_wb[mid] <= w)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Window id of each row's reference home; GH_EMPTY for rows that are not
+// their pair's first row (sorted last).
+__global__ void k_gh_winkey(const int64_t* __restrict__ pk, const int64_t* __restrict__ pv,
+                            const uint8_t* __restrict__ first, PinsGeom g,
+                            const uint32_t* __restrict__ winbase, int32_t* __restrict__ key) {
+  __shared__ int64_t s_starts[GH_PART_MAX + 1];
+  for (uint32_t q = threadIdx.x; q <= g.p; q += blockDim.x) s_starts[q] = g.starts[q];
+  __syncthreads();
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < g.n; j += stride) {
+    if (!first[j]) {
+      key[j] = int32_t(GH_EMPTY ^ 0x80000000u);  // sorts after every window id
+      continue;
+    }
+    const uint32_t q = gh_part_of(s_starts, g.p, j);
+    const uint64_t mask = uint64_t(g.tm[q]);
+    const uint64_t wsz = mask + 1 < uint64_t(GH_WIN) ? mask + 1 : uint64_t(GH_WIN);
+    const uint64_t h = gh_pair_fp(pk[j], pv[j]) & mask;
+    const uint64_t off = uint64_t(q) & mask;
+    key[j] = int32_t((winbase[q] + uint32_t(((h - off) & mask) / wsz)) ^ 0x80000000u);
+  }
+}
+
+// wstart[w] = first sorted position of window w (w <= nwin).
+__global__ void k_gh_winstart(const int32_t* __restrict__ skey, uint64_t n, uint32_t nwin,
+                              uint32_t* __restrict__ wstart) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > nwin) return;
+  const int32_t target = int32_t(w ^ 0x80000000u);
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (skey[mid] < target)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  wstart[w] = uint32_t(lo);
+}
+
+// Solve window w in LDS: its rows perm[wstart[w] .. wstart[w+1]) from
+// their homes, plus `nin` spilled ids from slot 0.  Writes the window's
+// slots to `hold` and its extension's ids (slot order) to spill[w];
+// returns through *changed whether the spill list differs from before.
+__device__ void gh_solve_window(uint32_t w, const WinGeom& g, const uint32_t* s_wb,
+                                const int64_t* __restrict__ pk, const int64_t* __restrict__ pv,
+                                const int32_t* __restrict__ perm,
+                                const uint32_t* __restrict__ wstart,
+                                const uint32_t* __restrict__ spill_in, uint32_t nin,
+                                uint32_t* __restrict__ hold, uint32_t* __restrict__ spill,
+                                uint32_t* __restrict__ nspill, uint32_t* s_hold,
+                                uint32_t* s_misc, uint32_t* __restrict__ err) {
+  const uint32_t q = gh_win_part(s_wb, g.p, w);
+  const uint64_t mask = uint64_t(g.tm[q]);
+  const uint32_t wsz = mask + 1 < uint64_t(GH_WIN) ? uint32_t(mask + 1) : uint32_t(GH_WIN);
+  const bool whole = uint64_t(wsz) == mask + 1;
+  const uint32_t ext = whole ? 0u : uint32_t(GH_EXT);
+  // first slot, within the partition (cyclic)
+  const uint64_t a = (uint64_t(q) + uint64_t(w - s_wb[q]) * wsz) & mask;
+  const uint32_t span = wsz + ext;
+  for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) s_hold[i] = GH_EMPTY;
+  if (threadIdx.x == 0) {
+    s_misc[0] = 0;
+    s_misc[1] = 0;
+  }
+  __syncthreads();
+  const uint32_t r0 = wstart[w], r1 = wstart[w + 1];
+  const uint32_t total = (r1 - r0) + nin;
+  bool over = false;
+  for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+    uint32_t carried, c;
+    if (t < nin) {
+      carried = spill_in[t];
+      c = 0;
+    } else {
+      carried = uint32_t(perm[r0 + (t - nin)]);
+      c = uint32_t(((gh_pair_fp(pk[carried], pv[carried]) & mask) - a) & mask);
+    }
+    for (uint32_t it = 0;; ++it) {
+      if (c >= span) {
+        if (!whole) {
+          over = true;
+          break;
+        }
+        c &= wsz - 1;
+      }
+      if (it > 4 * span + total) {  // cannot happen while the table has room
+        over = true;
+        break;
+      }
+      const uint32_t old = atomicMin(&s_hold[c], carried);
+      if (old == GH_EMPTY) break;
+      if (old > carried) carried = old;
+      ++c;
+    }
+  }
+  if (over) atomicOr(err, 32u);
+  __syncthreads();
+  uint32_t* tab = hold + g.toff[q];
+  for (uint32_t i = threadIdx.x; i < wsz; i += blockDim.x) tab[(a + i) & mask] = s_hold[i];
+  if (ext == 0) {
+    if (threadIdx.x == 0) {
+      if (nspill[w] != 0) s_misc[1] = 1;
+      nspill[w] = 0;
+    }
+    __syncthreads();
+    return;
+  }
+  // extension ids in slot order (one wave per step of 64 slots)
+  uint32_t* out = spill + size_t(w) * GH_EXT;
+  const uint32_t prev = nspill[w];
+  __syncthreads();
+  if (threadIdx.x < WAVE) {
+    uint32_t base = 0;
+    bool diff = false;
+    for (uint32_t s0 = 0; s0 < ext; s0 += WAVE) {
+      const uint32_t v = s_hold[wsz + s0 + threadIdx.x];
+      const bool occ = v != GH_EMPTY;
+      const uint64_t bal = __ballot(occ);
+      if (occ) {
+        const uint32_t pos = base + uint32_t(__popcll(bal & lanemask_lt()));
+        if (pos >= prev || out[pos] != v) diff = true;
+        out[pos] = v;
+      }
+      base += uint32_t(__popcll(bal));
+    }
+    diff = __ballot(diff) != 0 || base != prev;
+    if (threadIdx.x == 0) {
+      nspill[w] = base;
+      s_misc[1] = diff ? 1u : 0u;
+    }
+  }
+  __syncthreads();
+}
+
+// Phase A: every window alone.
+__global__ __launch_bounds__(GH_WBLOCK) void k_gh_win(WinGeom g, const int64_t* __restrict__ pk,
+                                                      const int64_t* __restrict__ pv,
+                                                      const int32_t* __restrict__ perm,
+                                                      const uint32_t* __restrict__ wstart,
+                                                      uint32_t* __restrict__ hold,
+                                                      uint32_t* __restrict__ spill,
+                                                      uint32_t* __restrict__ nspill,
+                                                      uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_hold[GH_WIN + GH_EXT];
+  __shared__ uint32_t s_wb[GH_PART_MAX + 1];
+  __shared__ uint32_t s_misc[2];
+  for (uint32_t q = threadIdx.x; q <= g.p; q += blockDim.x) s_wb[q] = g.winbase[q];
+  __syncthreads();
+  gh_solve_window(blockIdx.x, g, s_wb, pk, pv, perm, wstart, nullptr, 0, hold, spill, nspill,
+                  s_hold, s_misc, err);
+}
+
+// Phase B: one workgroup per partition of several windows walks them in
+// cyclic order from a window whose predecessor spilled nothing (else from
+// window 0, three laps), re-solving each window whose predecessor spills.
+// A spill list still changing on the last lap sets the overflow flag.
+__global__ __launch_bounds__(GH_WBLOCK) void k_gh_win_fix(WinGeom g, const int64_t* __restrict__ pk,
+                                                          const int64_t* __restrict__ pv,
+                                                          const int32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ wstart,
+                                                          uint32_t* __restrict__ hold,
+                                                          uint32_t* __restrict__ spill,
+                                                          uint32_t* __restrict__ nspill,
+                                                          uint32_t* __restrict__ tmp,
+                                                          uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_hold[GH_WIN + GH_EXT];
+  __shared__ uint32_t s_wb[GH_PART_MAX + 1];
+  __shared__ uint32_t s_misc[2];
+  __shared__ uint32_t s_start;
+  for (uint32_t q = threadIdx.x; q <= g.p; q += blockDim.x) s_wb[q] = g.winbase[q];
+  __syncthreads();
+  const uint32_t q = blockIdx.x;
+  const uint32_t w0 = s_wb[q], nw = s_wb[q + 1] - w0;
+  if (nw <= 1) return;
+  bool any = false;
+  for (uint32_t i = 0; i < nw; ++i) any |= nspill[w0 + i] != 0;
+  if (!any) return;
+  if (threadIdx.x == 0) {
+    uint32_t st = nw;  // a window whose predecessor spilled nothing
+    for (uint32_t i = 0; i < nw && st == nw; ++i)
+      if (nspill[w0 + (i + nw - 1) % nw] == 0) st = i;
+    s_start = st;
+  }
+  __syncthreads();
+  const uint32_t st = s_start == nw ? 0 : s_start;
+  const uint32_t laps = s_start == nw ? 3 : 1;
+  // the spill-ins of a window are copied out first: the window rewrites
+  // its own list, and with one window per partition it is its own
+  // predecessor
+  uint32_t* in = tmp + size_t(q) * GH_EXT;
+  for (uint32_t step = 0; step < laps * nw; ++step) {
+    const uint32_t wi = (st + step) % nw, w = w0 + wi, pw = w0 + (wi + nw - 1) % nw;
+    const uint32_t nin = nspill[pw];
+    if (nin == 0 && step < nw && laps == 1) continue;  // its alone-solution stands
+    for (uint32_t i = threadIdx.x; i < nin; i += blockDim.x) in[i] = spill[size_t(pw) * GH_EXT + i];
+    __syncthreads();
+    gh_solve_window(w, g, s_wb, pk, pv, perm, wstart, in, nin, hold, spill, nspill, s_hold,
+                    s_misc, err);
+    if (laps == 3 && step >= 2 * nw && s_misc[1] && threadIdx.x == 0) atomicOr(err, 32u);
+    __syncthreads();
   }
 }
 
@@ -375,57 +623,113 @@ __global__ void k_gh_materialize(const uint32_t* __restrict__ hold, uint64_t m,
 }
 
 // ------------------------------------------------------------ occupancy
-// One workgroup per partition: used slots of its table window.
-__global__ __launch_bounds__(GH_BLOCK) void k_gh_occ_count(const uint8_t* __restrict__ used,
-                                                           const int64_t* __restrict__ toff,
-                                                           int64_t* __restrict__ cnt) {
-  __shared__ unsigned long long s_w[GH_BLOCK / WAVE];
-  const uint32_t q = blockIdx.x;
-  const int64_t s0 = toff[q], s1 = toff[q + 1];
-  unsigned long long c = 0;
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += GH_BLOCK) c += used[s] != 0;
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  if (lane_id() == 0) s_w[wave_id()] = c;
+// Table windows cut into GH_CH-slot chunks: chunk c of partition q is
+// cb[q] + c, covering [toff[q] + c*GH_CH, min(.., toff[q+1])).
+constexpr int GH_CH = 4096;
+constexpr int GH_CH_IPT = GH_CH / GH_BLOCK;
+
+__device__ __forceinline__ uint32_t gh_chunk_part(const uint32_t* s_cb, uint32_t p, uint32_t c) {
+  uint32_t lo = 0, hi = p;  // last q with cb[q] <= c
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (s_cb[mid] <= c)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Used slots per chunk.
+__global__ __launch_bounds__(GH_BLOCK) void k_gh_chunk_count(const uint8_t* __restrict__ used,
+                                                             const int64_t* __restrict__ toff,
+                                                             const uint32_t* __restrict__ cb,
+                                                             uint32_t p, uint32_t* __restrict__ cc) {
+  __shared__ uint32_t s_cb[GH_PART_MAX + 1];
+  __shared__ uint32_t s_w[GH_BLOCK / WAVE];
+  for (uint32_t q = threadIdx.x; q <= p; q += GH_BLOCK) s_cb[q] = cb[q];
+  __syncthreads();
+  const uint32_t c = blockIdx.x, q = gh_chunk_part(s_cb, p, c);
+  const int64_t s0 = toff[q] + int64_t(c - s_cb[q]) * GH_CH;
+  const int64_t s1 = s0 + GH_CH < toff[q + 1] ? s0 + GH_CH : toff[q + 1];
+  uint32_t x = 0;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += GH_BLOCK) x += used[s] != 0;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  if (lane_id() == 0) s_w[wave_id()] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint64_t t = 0;
+    uint32_t t = 0;
     for (int w = 0; w < GH_BLOCK / WAVE; ++w) t += s_w[w];
-    cnt[q] = int64_t(t);
+    cc[c] = t;
   }
 }
 
-// One wave per partition: its occupied keys in slot order from
-// ok[base[q]], 64 slots per step (ballot ranks, running offset).  The
-// reference carves ok into one consecutive piece per shard
-// ([ostart[lo] - ostart[0], ostart[hi] - ostart[0])); base[q] is the
-// shard's piece start plus the counts of the shard's earlier partitions,
-// lim[q] the piece end.
-__global__ __launch_bounds__(WAVE) void k_gh_occ_fill(const int64_t* __restrict__ tk,
-                                                      const uint8_t* __restrict__ used,
-                                                      const int64_t* __restrict__ toff,
-                                                      const int64_t* __restrict__ base,
-                                                      const int64_t* __restrict__ lim,
-                                                      int64_t* __restrict__ ok,
-                                                      uint32_t* __restrict__ err) {
+// Per partition: the total of its chunks (cnt[q]) and, when cpos is set,
+// each chunk's first output position base[q] + the earlier chunks' counts.
+__global__ __launch_bounds__(GH_BLOCK) void k_gh_chunk_scan(const uint32_t* __restrict__ cc,
+                                                            const uint32_t* __restrict__ cb,
+                                                            const int64_t* __restrict__ base,
+                                                            int64_t* __restrict__ cpos,
+                                                            int64_t* __restrict__ cnt) {
+  __shared__ uint32_t s_tmp[GH_BLOCK / WAVE];
   const uint32_t q = blockIdx.x;
-  const int64_t s0 = toff[q], s1 = toff[q + 1];
-  const int64_t end = lim[q];
-  int64_t w = base[q];
-  bool bad = false;
-  for (int64_t s = s0; s < s1; s += WAVE) {
-    const int64_t i = s + lane_id();
-    const bool u = i < s1 && used[i] != 0;
-    const uint64_t bal = __ballot(u);
-    if (u) {
-      const int64_t pos = w + __popcll(bal & lanemask_lt());
-      if (pos < 0 || pos >= end)
-        bad = true;
-      else
-        ok[pos] = tk[i];
-    }
-    w += __popcll(bal);
+  const uint32_t c0 = cb[q], c1 = cb[q + 1];
+  int64_t run = base ? base[q] : 0;
+  int64_t tot_all = 0;
+  for (uint32_t b0 = c0; b0 < c1; b0 += GH_BLOCK) {
+    const uint32_t c = b0 + threadIdx.x;
+    const uint32_t x = c < c1 ? cc[c] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<GH_BLOCK / WAVE>(x, s_tmp, &tot);
+    if (cpos && c < c1) cpos[c] = run + int64_t(ex);
+    run += int64_t(tot);
+    tot_all += int64_t(tot);
+    __syncthreads();
   }
-  if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(err, 16u);
+  if (cnt && threadIdx.x == 0) cnt[q] = tot_all;
+}
+
+// One workgroup per chunk: its occupied keys in slot order from
+// ok[cpos[chunk]] (thread t owns 16 consecutive slots), bounded by the
+// shard piece's end lim[q] (the reference panics past it).
+__global__ __launch_bounds__(GH_BLOCK) void k_gh_occ_fill(const int64_t* __restrict__ tk,
+                                                          const uint8_t* __restrict__ used,
+                                                          const int64_t* __restrict__ toff,
+                                                          const uint32_t* __restrict__ cb,
+                                                          uint32_t p,
+                                                          const int64_t* __restrict__ cpos,
+                                                          const int64_t* __restrict__ lim,
+                                                          int64_t* __restrict__ ok,
+                                                          uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_cb[GH_PART_MAX + 1];
+  __shared__ uint32_t s_tmp[GH_BLOCK / WAVE];
+  for (uint32_t q = threadIdx.x; q <= p; q += GH_BLOCK) s_cb[q] = cb[q];
+  __syncthreads();
+  const uint32_t c = blockIdx.x, q = gh_chunk_part(s_cb, p, c);
+  const int64_t s0 = toff[q] + int64_t(c - s_cb[q]) * GH_CH;
+  const int64_t s1 = s0 + GH_CH < toff[q + 1] ? s0 + GH_CH : toff[q + 1];
+  const int64_t my = s0 + int64_t(threadIdx.x) * GH_CH_IPT;
+  uint32_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < GH_CH_IPT; ++j) {
+    const int64_t s = my + j;
+    mine |= uint32_t(s < s1 && used[s] != 0) << j;
+  }
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan<GH_BLOCK / WAVE>(uint32_t(__popc(mine)), s_tmp, &tot);
+  int64_t pos = cpos[c] + int64_t(ex);
+  const int64_t end = lim[q];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < GH_CH_IPT; ++j) {
+    if (!((mine >> j) & 1u)) continue;
+    if (pos < 0 || pos >= end)
+      bad = true;
+    else
+      ok[pos] = tk[my + j];
+    ++pos;
+  }
+  if (bad) atomicOr(err, 16u);
 }
 
 template <class T>
@@ -607,9 +911,55 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
       return err_rc(e0);
     }
     fill_u32(hold, GH_EMPTY, m, st.s);
-    hipLaunchKernelGGL(k_gh_place, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, first, g, hold,
-                       err);
-    NFH_LAUNCH_CHECK();
+    // windows of min(slots, GH_WIN) slots per partition
+    std::vector<uint32_t> wb(p + 1, 0);
+    for (size_t q = 0; q < p; ++q) {
+      const uint64_t slots = uint64_t(hm[q]) + 1;
+      wb[q + 1] = wb[q] + uint32_t(slots / std::min<uint64_t>(slots, GH_WIN));
+    }
+    const uint32_t nwin = wb[p];
+    bool windowed = uint64_t(nwin) * GH_EXT * 4 <= (uint64_t(1) << 30);
+    if (windowed) {
+      int32_t* key = scratch_t<int32_t>(S_T3, n);
+      int32_t* perm = scratch_t<int32_t>(S_T4, n);
+      uint32_t* t0 = scratch_t<uint32_t>(S_T5, n);
+      uint32_t* t1 = scratch_t<uint32_t>(S_WORK0, n);
+      int32_t* tp = scratch_t<int32_t>(S_WORK1, n);
+      int32_t* skey = scratch_t<int32_t>(S_WORK2, n);
+      uint32_t* small = scratch_t<uint32_t>(S_WORK3, (p + 1) + (nwin + 1) + nwin + p * GH_EXT);
+      uint32_t* dwb = small;
+      uint32_t* wstart = dwb + p + 1;
+      uint32_t* nspill = wstart + nwin + 1;
+      uint32_t* tmpin = nspill + nwin;
+      uint32_t* spill = scratch_t<uint32_t>(S_MISC, size_t(nwin) * GH_EXT);
+      NFH_CHECK(hipMemcpyAsync(dwb, wb.data(), (p + 1) * 4, hipMemcpyHostToDevice, st.s));
+      fill_u32(nspill, 0, nwin, st.s);
+      hipLaunchKernelGGL(k_gh_winkey, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, first, g, dwb,
+                         key);
+      NFH_LAUNCH_CHECK();
+      radix_argsort<int32_t>(key, n, false, perm, t0, t1, tp, skey, st.s);
+      hipLaunchKernelGGL(k_gh_winstart, dim3((nwin + 1 + 255) / 256), dim3(256), 0, st.s, skey,
+                         uint64_t(n), nwin, wstart);
+      NFH_LAUNCH_CHECK();
+      const WinGeom wg{g.toff, g.tm, dwb, uint32_t(p), nwin};
+      hipLaunchKernelGGL(k_gh_win, dim3(nwin), dim3(GH_WBLOCK), 0, st.s, wg, dpk, dpv, perm,
+                         wstart, hold, spill, nspill, err);
+      NFH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gh_win_fix, dim3(p), dim3(GH_WBLOCK), 0, st.s, wg, dpk, dpv, perm,
+                         wstart, hold, spill, nspill, tmpin, err);
+      NFH_LAUNCH_CHECK();
+      const uint32_t e1 = read_scalar(err, st.s);
+      if (e1 & 32u) {  // a cluster longer than the extension: global form
+        fill_u32(err, e1 & ~32u, 1, st.s);
+        fill_u32(hold, GH_EMPTY, m, st.s);
+        windowed = false;
+      }
+    }
+    if (!windowed) {
+      hipLaunchKernelGGL(k_gh_place, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, first, g,
+                         hold, err);
+      NFH_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_gh_materialize, dim3(gslot), dim3(GH_BLOCK), 0, st.s, hold, m, dpk, dpv,
                        dtk, dtv, dused);
     NFH_LAUNCH_CHECK();
@@ -633,12 +983,25 @@ NF_EXPORT int32_t nf_ghash_occ_count(const uint8_t* used, const int64_t* toff, s
     if (ho[q] < 0 || ho[q] > ho[q + 1]) return MALFORMED;
   const uint64_t m = uint64_t(ho[p]);
   const uint8_t* du = m ? st.in(used, m, S_IN0) : nullptr;
+  std::vector<uint32_t> hcb(p + 1, 0);
+  for (size_t q = 0; q < p; ++q)
+    hcb[q + 1] = hcb[q] + uint32_t((ho[q + 1] - ho[q] + GH_CH - 1) / GH_CH);
+  const uint32_t nch = hcb[p];
   int64_t* dtoff = scratch_t<int64_t>(S_T0, p + 1);
+  uint32_t* dcb = scratch_t<uint32_t>(S_T1, (p + 1) + size_t(nch) + 1);
+  uint32_t* cc = dcb + p + 1;
   NFH_CHECK(hipMemcpyAsync(dtoff, ho.data(), (p + 1) * 8, hipMemcpyHostToDevice, st.s));
+  NFH_CHECK(hipMemcpyAsync(dcb, hcb.data(), (p + 1) * 4, hipMemcpyHostToDevice, st.s));
   int64_t* dcnt = st.out(cnt, p, S_OUT0);
   {
     ProfScope prof("ghash_occ_count", st.s, double(m));
-    hipLaunchKernelGGL(k_gh_occ_count, dim3(p), dim3(GH_BLOCK), 0, st.s, du, dtoff, dcnt);
+    if (nch > 0) {
+      hipLaunchKernelGGL(k_gh_chunk_count, dim3(nch), dim3(GH_BLOCK), 0, st.s, du, dtoff, dcb,
+                         uint32_t(p), cc);
+      NFH_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_gh_chunk_scan, dim3(p), dim3(GH_BLOCK), 0, st.s, cc, dcb, nullptr,
+                       nullptr, dcnt);
     NFH_LAUNCH_CHECK();
   }
   st.finish();
@@ -671,14 +1034,28 @@ NF_EXPORT int32_t nf_ghash_occ_fill(const int64_t* tk, const uint8_t* used, cons
   }
   const uint8_t* du = m ? st.in(used, m, S_IN1) : nullptr;
   const int64_t* dtk = m ? st.in(tk, m, S_IN0) : nullptr;
-  int64_t* dtoff = scratch_t<int64_t>(S_T0, 3 * p + 3 + 2);
+  std::vector<uint32_t> hcb(p + 1, 0);
+  for (size_t q = 0; q < p; ++q)
+    hcb[q + 1] = hcb[q] + uint32_t((ho[q + 1] - ho[q] + GH_CH - 1) / GH_CH);
+  const uint32_t nch = hcb[p];
+  int64_t* dtoff = scratch_t<int64_t>(S_T0, 4 * p + 4 + 2);
   int64_t* dbase = dtoff + p + 1;
   int64_t* dlim = dbase + p + 1;
-  uint32_t* err = reinterpret_cast<uint32_t*>(dlim + p + 1);
+  int64_t* dcnt = dlim + p + 1;
+  uint32_t* err = reinterpret_cast<uint32_t*>(dcnt + p + 1);
+  uint32_t* dcb = scratch_t<uint32_t>(S_T1, (p + 1) + size_t(nch) + 1);
+  uint32_t* cc = dcb + p + 1;
+  int64_t* cpos = scratch_t<int64_t>(S_T2, size_t(nch) + 1);
   NFH_CHECK(hipMemcpyAsync(dtoff, ho.data(), (p + 1) * 8, hipMemcpyHostToDevice, st.s));
-  // per-partition counts, then each shard's running bases
-  int64_t* dcnt = scratch_t<int64_t>(S_T1, p);
-  hipLaunchKernelGGL(k_gh_occ_count, dim3(p), dim3(GH_BLOCK), 0, st.s, du, dtoff, dcnt);
+  NFH_CHECK(hipMemcpyAsync(dcb, hcb.data(), (p + 1) * 4, hipMemcpyHostToDevice, st.s));
+  // per-chunk and per-partition counts, then each shard's running bases
+  if (nch > 0) {
+    hipLaunchKernelGGL(k_gh_chunk_count, dim3(nch), dim3(GH_BLOCK), 0, st.s, du, dtoff, dcb,
+                       uint32_t(p), cc);
+    NFH_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_gh_chunk_scan, dim3(p), dim3(GH_BLOCK), 0, st.s, cc, dcb, nullptr, nullptr,
+                     dcnt);
   NFH_LAUNCH_CHECK();
   const std::vector<int64_t> hc = host_copy(dcnt, p, st.s);
   std::vector<int64_t> hb(p + 1, 0), hl(p + 1, 0);
@@ -689,13 +1066,16 @@ NF_EXPORT int32_t nf_ghash_occ_fill(const int64_t* tk, const uint8_t* used, cons
   }
   NFH_CHECK(hipMemcpyAsync(dbase, hb.data(), (p + 1) * 8, hipMemcpyHostToDevice, st.s));
   NFH_CHECK(hipMemcpyAsync(dlim, hl.data(), (p + 1) * 8, hipMemcpyHostToDevice, st.s));
+  hipLaunchKernelGGL(k_gh_chunk_scan, dim3(p), dim3(GH_BLOCK), 0, st.s, cc, dcb, dbase, cpos,
+                     nullptr);
+  NFH_LAUNCH_CHECK();
   int64_t* dok = st.out(ok, size_t(std::max<int64_t>(nocc, 1)), S_OUT0, nocc > 0);
   st.set_out_bytes(ok, size_t(std::max<int64_t>(nocc, 0)) * 8);
   fill_u32(err, 0, 1, st.s);
-  {
+  if (nch > 0) {
     ProfScope prof("ghash_occ_fill", st.s, double(m) * 9.0);
-    hipLaunchKernelGGL(k_gh_occ_fill, dim3(p), dim3(WAVE), 0, st.s, dtk, du, dtoff, dbase, dlim,
-                       dok, err);
+    hipLaunchKernelGGL(k_gh_occ_fill, dim3(nch), dim3(GH_BLOCK), 0, st.s, dtk, du, dtoff, dcb,
+                       uint32_t(p), cpos, dlim, dok, err);
     NFH_LAUNCH_CHECK();
   }
   const uint32_t e = read_scalar(err, st.s);
