@@ -152,30 +152,48 @@ __global__ void k_sample(const int32_t* __restrict__ keys, uint32_t n, uint32_t 
 }
 
 // Run length at each run head of the sorted sample (0 elsewhere).
-__global__ void k_hot_runs(const int32_t* __restrict__ sk, uint32_t S, uint32_t* __restrict__ len) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S) return;
-  uint32_t L = 0;
-  if (i == 0 || sk[i] != sk[i - 1]) {
-    // end of the run: first position holding a larger key (binary search)
-    const int32_t k = sk[i];
-    uint32_t lo = i + 1, hi = S;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (sk[mid] == k)
-        lo = mid + 1;
-      else
-        hi = mid;
+// Hot-key counting: the sample is split over HOT_PARTS workgroups by a
+// hash of the key; each counts its part in an LDS table of 64-bit words
+// {key : 32 | count : 32} (count 0 = empty; claimed by compare-and-swap,
+// counted by atomic add, no waiting on other lanes) and appends the keys
+// seen at least HOT_MIN_COUNT times to the candidate list.
+constexpr int HOT_PARTS = 16;
+constexpr int HOT_PART_SLOTS = 8192;  // a part holds about S / 16 sampled keys
+
+__global__ __launch_bounds__(1024) void k_hot_count(const int32_t* __restrict__ sample, uint32_t S,
+                                                    int2* __restrict__ cand,
+                                                    uint32_t* __restrict__ ncand) {
+  __shared__ unsigned long long t[HOT_PART_SLOTS];
+  for (int i = threadIdx.x; i < HOT_PART_SLOTS; i += blockDim.x) t[i] = 0ull;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+    const int32_t k = sample[i];
+    const uint32_t h = hot_mix(k);
+    if ((h >> 28) != blockIdx.x) continue;
+    const unsigned long long mine = ((unsigned long long)uint32_t(k) << 32) | 1ull;
+    uint32_t slot = h & (HOT_PART_SLOTS - 1);
+    for (int probe = 0; probe < HOT_PART_SLOTS; ++probe) {
+      const unsigned long long old = atomicCAS(&t[slot], 0ull, mine);
+      if (old == 0ull) break;                          // claimed, count 1
+      if (uint32_t(old >> 32) == uint32_t(k)) {       // counted
+        atomicAdd(&t[slot], 1ull);
+        break;
+      }
+      slot = (slot + 1) & (HOT_PART_SLOTS - 1);
     }
-    L = lo - i;
   }
-  len[i] = L;
+  __syncthreads();
+  for (int i = threadIdx.x; i < HOT_PART_SLOTS; i += blockDim.x) {
+    const unsigned long long e = t[i];
+    const uint32_t c = uint32_t(e);
+    if (c >= HOT_MIN_COUNT) cand[atomicAdd(ncand, 1u)] = make_int2(int32_t(e >> 32), int32_t(c));
+  }
 }
 
-// One block: keep the keys sampled most often (at most HOT_MAX, each seen
-// at least HOT_MIN_COUNT times) and build their open-addressing table.
-__global__ __launch_bounds__(1024) void k_hot_pick(const int32_t* __restrict__ sk,
-                                                   const uint32_t* __restrict__ len, uint32_t S,
+// One block: keep the candidates counted most often (at most HOT_MAX) and
+// build their cuckoo table.
+__global__ __launch_bounds__(1024) void k_hot_pick(const int2* __restrict__ cand,
+                                                   const uint32_t* __restrict__ ncand,
                                                    int32_t* __restrict__ hot_keys,
                                                    int2* __restrict__ table,
                                                    uint32_t* __restrict__ hot_n) {
@@ -185,11 +203,12 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int32_t* __restrict__ s
   __shared__ int32_t t_idx[HOT_TBL];
   __shared__ int32_t t_key[HOT_TBL];
   __shared__ uint32_t s_thr, s_cnt;
+  const uint32_t nc = *ncand;
   for (int b = threadIdx.x; b < HOT_LEN_BUCKETS; b += blockDim.x) hist[b] = 0;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-    const uint32_t L = len[i];
-    if (L >= HOT_MIN_COUNT) atomicAdd(&hist[L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1], 1u);
+  for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
+    const uint32_t L = uint32_t(cand[i].y);
+    atomicAdd(&hist[L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1], 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -204,13 +223,14 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int32_t* __restrict__ s
   }
   __syncthreads();
   const uint32_t thr = s_thr;
-  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-    const uint32_t L = len[i];
+  for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
+    const int2 c = cand[i];
+    const uint32_t L = uint32_t(c.y);
     const uint32_t Lb = L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1;
-    if (L >= HOT_MIN_COUNT && Lb >= thr) {
+    if (Lb >= thr) {
       const uint32_t idx = atomicAdd(&s_cnt, 1u);
-      hot_keys[idx] = sk[i];
-      s_hot[idx] = sk[i];
+      hot_keys[idx] = c.x;
+      s_hot[idx] = c.x;
     }
   }
   __syncthreads();
@@ -453,23 +473,19 @@ static bool build_hot_table(const int32_t* keys, size_t n, int2** table, int32_t
   if (n < size_t(HOT_SAMPLE) * 8) return false;
   const uint32_t S = HOT_SAMPLE;
   unsigned char* base = static_cast<unsigned char*>(
-      scratch(S_WORK3, size_t(S) * 4 * 7 + HOT_TBL * 8 + HOT_MAX * 4 + 256));
+      scratch(S_WORK3, size_t(S) * 4 + size_t(S) * 8 + HOT_TBL * 8 + HOT_MAX * 4 + 256));
   int32_t* sample = reinterpret_cast<int32_t*>(base);
-  int32_t* sorted = sample + S;
-  int32_t* perm = sorted + S;
-  uint32_t* t0 = reinterpret_cast<uint32_t*>(perm + S);
-  uint32_t* t1 = t0 + S;
-  int32_t* tp = reinterpret_cast<int32_t*>(t1 + S);
-  uint32_t* len = reinterpret_cast<uint32_t*>(tp + S);
-  *table = reinterpret_cast<int2*>(len + S);
+  int2* cand = reinterpret_cast<int2*>(sample + S);
+  *table = cand + S;
   *hot_keys = reinterpret_cast<int32_t*>(*table + HOT_TBL);
   *hot_n = reinterpret_cast<uint32_t*>(*hot_keys + HOT_MAX);
+  uint32_t* ncand = *hot_n + 4;
+  fill_u32(ncand, 0, 1, s);
   hipLaunchKernelGGL(k_sample, dim3(S / 256), dim3(256), 0, s, keys, uint32_t(n), S, sample);
   NFH_LAUNCH_CHECK();
-  radix_argsort<int32_t>(sample, S, false, perm, t0, t1, tp, sorted, s);
-  hipLaunchKernelGGL(k_hot_runs, dim3(S / 256), dim3(256), 0, s, sorted, S, len);
+  hipLaunchKernelGGL(k_hot_count, dim3(HOT_PARTS), dim3(1024), 0, s, sample, S, cand, ncand);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_hot_pick, dim3(1), dim3(1024), 0, s, sorted, len, S, *hot_keys, *table,
+  hipLaunchKernelGGL(k_hot_pick, dim3(1), dim3(1024), 0, s, cand, ncand, *hot_keys, *table,
                      *hot_n);
   NFH_LAUNCH_CHECK();
   return true;
