@@ -72,6 +72,51 @@ def kernel_table(L):
     return out
 
 
+# HBM traffic per launch of a dominant kernel, from the rocprofv3 counter
+# passes summarised in profiles/ (FETCH_SIZE x 2 + WRITE_SIZE, calibrated
+# against a 1 GiB copy): profiled scope -> (summary file, kernel rows, how).
+# "mean": one kernel per launch of the scope, dispatch-weighted over the
+# rows; "sum": every row runs once per launch of the scope.
+TRAFFIC_ROWS = {
+    "gp_part_first": ("groupby_unique_zipf_2p28.txt", ["k_gp_part<true, true>"], "mean"),
+    "radix_onesweep": ("argsort_uniform_2p28.txt",
+                       ["k_onesweep<int, true, false, true, RadixDigit>",
+                        "k_onesweep<int, false, false, true, RadixDigit>",
+                        "k_onesweep<int, false, true, true, RadixDigit>"], "mean"),
+    "learned_pass": ("learned_sort_2p28.txt",
+                     ["k_onesweep<int, true, false, true, LearnedDigit>",
+                      "k_onesweep<int, false, false, true, LearnedDigit>"], "mean"),
+    "ghash_pins": ("ghash_count_distinct_2p26.txt",
+                   ["k_gh_dedupe", "k_gh_first", "k_gh_winkey", "k_gh_win",
+                    "k_gh_materialize"], "sum"),
+}
+
+
+def traffic_of(scope):
+    """Bytes per launch (fetch + write) of the scope's kernels, or None."""
+    if scope not in TRAFFIC_ROWS:
+        return None
+    fname, rows, how = TRAFFIC_ROWS[scope]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", fname)
+    tot = calls = 0.0
+    per = {}
+    try:
+        for line in open(path):
+            name = line[:58].rstrip()
+            if name in rows:
+                parts = line[58:].split()
+                c = float(parts[0])
+                b = (float(parts[3]) + float(parts[4])) * 2 ** 20
+                tot += c * b
+                calls += c
+                per[name] = b
+    except (OSError, ValueError, IndexError):
+        return None
+    if how == "sum":
+        return sum(per.values()) if len(per) == len(rows) else None
+    return tot / calls if calls else None
+
+
 def roofline_of(kernels, steps):
     """Roofline object for the kernel with the largest total time that
     carries an algorithmic byte count."""
@@ -80,8 +125,10 @@ def roofline_of(kernels, steps):
         return None
     _, name = max(cands)
     v = kernels[name]
+    tr = traffic_of(name)
     return {"bound": "hbm", "kernel": name, "achieved": v["GBps"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(v["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(v["GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": None if tr is None else round(tr),
             "alg_bytes_per_launch": v["alg_bytes_per_launch"], "kernel_avg_ms": v["avg_ms"],
             "launches_per_step": round(v["launches"] / max(1, steps), 2)}
 
