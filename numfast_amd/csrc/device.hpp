@@ -37,6 +37,8 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {
 // above is used when it does not hold).  Invalid lanes must be the highest
 // lanes of the wave (a tail).  When every valid lane has the same digit the
 // wave updates the counter once instead of serialising on one address.
+// (Peeling several shared digits off one by one measured slower, on
+// uniform keys and on 1024 distinct keys alike.)
 __device__ __forceinline__ uint32_t rank_lds(uint32_t* ctr, uint32_t d, bool valid) {
   const uint64_t act = __ballot(valid);
   const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);

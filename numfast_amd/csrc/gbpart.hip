@@ -212,27 +212,38 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     // key and mask loads first (independent, coalesced per item), then the
     // filter, then the value loads of the records only -- all issued before
     // the ranking so their latency hides behind it
-    uint32_t okm = 0;
+    // Every lane loads from an in-range row (past the end: the last row),
+    // so all loads are in flight at once; values are loaded for every row
+    // (a 64-byte segment with no record at all is rare, so this fetches
+    // the same lines as loading records only).
+    uint32_t mb[GP_IPT];
 #pragma unroll
     for (int j = 0; j < GP_IPT; ++j) {
       const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      const bool in = pos < n;
-      ov[j] = in ? uint32_t(keys[pos]) : 0u;
-      okm |= uint32_t(in && (vm == nullptr || vm[pos] != 0)) << j;
+      const uint32_t pc = pos < n ? pos : n - 1u;
+      ov[j] = uint32_t(keys[pc]);
+      vv[j] = vals[pc];
+      mb[j] = 1u;
+    }
+    if (vm != nullptr) {
+#pragma unroll
+      for (int j = 0; j < GP_IPT; ++j) {
+        const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+        mb[j] = vm[pos < n ? pos : n - 1u];
+      }
     }
 #pragma unroll
     for (int j = 0; j < GP_IPT; ++j) {
-      if ((okm >> j) & 1u) {
+      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+      if (pos < n && mb[j] != 0u) {
         const int32_t k = int32_t(ov[j]);
         if (!table || hot_slot_of(t_key, k) < 0) {
           ov[j] = uint32_t(k - kmin);
           live |= 1u << j;
         }
       }
+      if (!((live >> j) & 1u)) vv[j] = 0;
     }
-#pragma unroll
-    for (int j = 0; j < GP_IPT; ++j)
-      vv[j] = ((live >> j) & 1u) ? vals[wbase + uint32_t(j * WAVE + lane)] : 0;
   }
   if constexpr (!FIRST) {
 #pragma unroll

@@ -641,11 +641,13 @@ __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
           if (((m >> (8 * c)) & 0xFFu) && cold(kk[c])) need |= 1u << (4 * u + c);
         }
       }
+      // every lane loads (rows that need no flag read the in-range byte 0),
+      // so the 4 * RQ random reads are all in flight at once
 #pragma unroll
-      for (int j = 0; j < 4 * RQ; ++j) f[j] = ((need >> j) & 1u) ? flag[off[j]] : 1u;
+      for (int j = 0; j < 4 * RQ; ++j) f[j] = flag[((need >> j) & 1u) ? off[j] : 0u];
 #pragma unroll
       for (int j = 0; j < 4 * RQ; ++j)
-        if (f[j] == 0u) flag[off[j]] = 1u;
+        if (((need >> j) & 1u) && f[j] == 0u) flag[off[j]] = 1u;
     }
     for (; q < nq; q += stride) {
       const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
@@ -765,14 +767,33 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t* __restrict__ keys,
         m[u] = v ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(v) + qq)
                  : 0x01010101u;
       }
-      int4 o[RQ];
+      // per row: -1 (NULL), the hot code (LDS), or a word-table lookup;
+      // every lane issues its lookup (rows without one read word 0), so
+      // the 4 * RQ random reads are all in flight at once
+      int32_t c[4 * RQ];
+      uint32_t off[4 * RQ];
 #pragma unroll
       for (int u = 0; u < RQ; ++u) {
-        o[u].x = code(k4[u].x, m[u] & 0xFFu);
-        o[u].y = code(k4[u].y, m[u] & 0xFF00u);
-        o[u].z = code(k4[u].z, m[u] & 0xFF0000u);
-        o[u].w = code(k4[u].w, m[u] & 0xFF000000u);
+        const int32_t kk[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool ok = (m[u] >> (8 * e)) & 0xFFu;
+          const int32_t idx = (ok && nh != 0) ? hot_index(t_key, t_idx, kk[e]) : -1;
+          c[4 * u + e] = !ok ? -1 : (idx >= 0 ? s_code[idx] : INT32_MIN);
+          off[4 * u + e] = (ok && idx < 0) ? uint32_t(kk[e] - kmin) : 0u;
+        }
       }
+      u64 wd[4 * RQ];
+#pragma unroll
+      for (int j = 0; j < 4 * RQ; ++j) wd[j] = words[off[j] >> 5];
+#pragma unroll
+      for (int j = 0; j < 4 * RQ; ++j)
+        if (c[j] == INT32_MIN)
+          c[j] = int32_t(uint32_t(wd[j] >> 32) +
+                         uint32_t(__popc(uint32_t(wd[j]) & ((1u << (off[j] & 31u)) - 1u))));
+      int4 o[RQ];
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) o[u] = make_int4(c[4 * u], c[4 * u + 1], c[4 * u + 2], c[4 * u + 3]);
 #pragma unroll
       for (int u = 0; u < RQ; ++u) st_stream4(inv, q + uint32_t(u) * stride, o[u]);
     }

@@ -18,7 +18,7 @@
 // depends on the fit.
 //
 // Sort (learned_argsort_i32):
-//   1. the model is tabulated at 2049 knots into integer codes scaled to
+//   1. the model is tabulated at 1025 knots into integer codes scaled to
 //      [0, 2^bits * 16], made non-decreasing by a running max;
 //   2. the bucket code of a key word interpolates the knots in integer
 //      arithmetic, so code(w) is non-decreasing in w: every key of bucket b
@@ -46,7 +46,7 @@
 
 namespace nfh {
 
-constexpr int KNOTS = 2048;
+constexpr int KNOTS = 1024;
 constexpr int LS_SAMPLE = 1 << 16;
 constexpr int LS_BLOCK = 512;
 constexpr int LS_WAVES = LS_BLOCK / WAVE;
@@ -59,9 +59,9 @@ constexpr int LS_DIG = 1 << LS_DBITS;
 // 32-bit integer arithmetic: with u = clamp(w - wmin, 0, span),
 // p = umulhi(u << shl, mul) = floor(u * 2^21 / (span + 1)) up to a relative
 // 2^-21 (shl = clz(span) normalises, mul = 2^53 / ((span + 1) << shl) in
-// [2^21, 2^22]), a knot index . 10-bit fraction below 2^21; knots are in
+// [2^21, 2^22]), a knot index . 11-bit fraction below 2^21; knots are in
 // units of code / 16 (<= 2^20), so every product fits 32 bits.
-constexpr int KN_FRAC = 10;  // fraction bits of the knot position
+constexpr int KN_FRAC = 11;  // fraction bits of the knot position
 constexpr int KN_UNIT = 4;   // knot values carry 4 fractional code bits
 struct LearnedMap {
   const uint32_t* knots;  // KNOTS + 1 non-decreasing, units of code / 16
@@ -85,17 +85,19 @@ __device__ __forceinline__ uint32_t lcode(uint32_t w, const uint32_t* kn, const 
   return c < m.maxcode ? c : m.maxcode;
 }
 
+// Digit generator of the onesweep passes: the pass's digit is byte
+// shift / 8 of the key's bucket code (the knots live in the pass's LDS).
 struct LearnedDigit {
   LearnedMap m;
-  int shift;
+  int shift, next_shift;
   static constexpr size_t EXTRA_LDS = ((KNOTS + 1) * 4 + 15) / 16 * 16;
   __device__ void init(unsigned char* s) const {
     uint32_t* k = reinterpret_cast<uint32_t*>(s);
     for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) k[i] = m.knots[i];
   }
   template <class W>
-  __device__ uint32_t digit(W w, const unsigned char* s) const {
-    return (lcode(uint32_t(w), reinterpret_cast<const uint32_t*>(s), m) >> shift) & 255u;
+  __device__ uint32_t code(W w, const unsigned char* s) const {
+    return lcode(uint32_t(w), reinterpret_cast<const uint32_t*>(s), m);
   }
 };
 
@@ -325,70 +327,6 @@ __global__ __launch_bounds__(1024) void k_cdf_knots(const float* __restrict__ P,
 }
 
 // ------------------------------------------------------------ sort kernels
-// Histograms of the code's low and high byte.  A wave reads CH_RUN
-// consecutive 1 KiB stretches (coalesced quads, lane-contiguous); each lane
-// adds runs of equal codes at once, so nearly sorted keys (long runs of one
-// code) do not serialise on one LDS counter.
-constexpr int CH_RUN = 16;
-__global__ __launch_bounds__(256) void k_code_hist(const int32_t* __restrict__ keys, uint32_t n,
-                                                   LearnedMap m, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_kn[KNOTS + 1];
-  __shared__ uint32_t s_h[512];
-  for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) s_kn[i] = m.knots[i];
-  for (int i = threadIdx.x; i < 512; i += blockDim.x) s_h[i] = 0;
-  __syncthreads();
-  auto flush = [&](uint32_t c, uint32_t cnt) {
-    if (cnt == 0) return;
-    atomicAdd(&s_h[c & 255u], cnt);
-    atomicAdd(&s_h[256 + ((c >> 8) & 255u)], cnt);
-  };
-  uint32_t cur = 0, cnt = 0;
-  auto add = [&](int32_t key) {
-    const uint32_t code = lcode(uint32_t(key) ^ 0x80000000u, s_kn, m);
-    if (code == cur) {
-      ++cnt;
-    } else {
-      flush(cur, cnt);
-      cur = code;
-      cnt = 1;
-    }
-  };
-  const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
-  const uint32_t nq = vec ? n / 4 : 0;
-  const uint32_t wchunk = WAVE * CH_RUN;  // quads per wave stretch
-  const uint32_t gw = blockIdx.x * (blockDim.x / WAVE) + wave_id();
-  const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
-  for (uint32_t base = gw * wchunk; base < nq; base += nw * wchunk) {
-#pragma unroll 4
-    for (int r = 0; r < CH_RUN; ++r) {
-      const uint32_t q = base + uint32_t(r * WAVE + lane_id());
-      if (q >= nq) break;
-      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
-      add(k4.x);
-      add(k4.y);
-      add(k4.z);
-      add(k4.w);
-    }
-  }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = nq * 4 + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    add(keys[i]);
-  flush(cur, cnt);
-  __syncthreads();
-  for (int i = threadIdx.x; i < 512; i += blockDim.x)
-    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
-}
-
-// Exclusive scans of the two 256-bin histograms.
-__global__ __launch_bounds__(256) void k_bins2(const uint32_t* __restrict__ hist,
-                                               uint32_t* __restrict__ bins) {
-  __shared__ uint32_t s_tmp[4];
-  const int p = blockIdx.x;
-  uint32_t total;
-  const uint32_t ex = block_excl_scan<4>(hist[p * 256 + threadIdx.x], s_tmp, &total);
-  bins[p * 256 + threadIdx.x] = ex;
-}
-
 // start[b] = first position whose code >= b (b in [0, B]); start[B] = n.
 __global__ __launch_bounds__(256) void k_bucket_bounds(const uint32_t* __restrict__ W, uint32_t n,
                                                        LearnedMap m, uint32_t B,
@@ -801,47 +739,48 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   map.shl = uint32_t(__builtin_clz(hm.span));
   map.mul = uint32_t((uint64_t(1) << 53) / ((uint64_t(hm.span) + 1u) << map.shl));
   map.maxcode = B - 1;
-  // code histograms and bin starts; the large-bucket lists hold up to B
-  // entries each (at most one per bucket)
+  // per-chain code histograms and bin starts of the two code passes, tile
+  // counters, large / huge bucket counts; the large-bucket lists hold up to
+  // B entries each (at most one per bucket)
+  constexpr size_t CH = size_t(NCHAIN) * 256;
+  const size_t head = 4 * CH + 2 * NCHAIN + 2;
   uint32_t* meta =
-      scratch_t<uint32_t>(S_LBUCKET, 1024 + 4 + (B + 1) + 2 * size_t(B) + 2 * (size_t(B) + 1));
-  uint32_t* hist = meta;
-  uint32_t* bins = meta + 512;
-  uint32_t* ctr = meta + 1024;  // 2 tile counters, large buckets, huge buckets
-  uint32_t* bstart = ctr + 4;
+      scratch_t<uint32_t>(S_LBUCKET, head + (B + 1) + 2 * size_t(B) + 2 * (size_t(B) + 1));
+  uint32_t* hist = meta;            // [2][NCHAIN][256]
+  uint32_t* bins = meta + 2 * CH;   // [2][NCHAIN][256]
+  uint32_t* ctr = meta + 4 * CH;    // [2][NCHAIN] tile counters
+  uint32_t* nlh = ctr + 2 * NCHAIN; // large buckets, huge buckets
+  uint32_t* bstart = meta + head;
   uint32_t* large = bstart + (B + 1);
   uint32_t* huge = large + B;
   uint32_t* cst = huge + B;
   uint32_t* unequal = cst + (B + 1);
-  fill_u32(meta, 0, 1024 + 4, s);
-  {
-    ProfScope prof("learned_code_hist", s, double(n) * 4.0);
-    hipLaunchKernelGGL(k_code_hist, dim3(grid_for(n, 256, 4 * num_cus())), dim3(256), 0, s, keys,
-                       n, map, hist);
-    NFH_LAUNCH_CHECK();
-  }
-  {
-    ProfScope prof("learned_bins", s);
-    hipLaunchKernelGGL(k_bins2, dim3(2), dim3(256), 0, s, hist, bins);
-    NFH_LAUNCH_CHECK();
-  }
-  // stable code passes: low byte (reads the raw keys), then high byte
+  fill_u32(meta, 0, head, s);
+  const bool two = bits > 8;
+  chain_hist_first<int32_t>(keys, n, LearnedDigit{map, 0, 0}, false, hist,
+                            static_cast<uint32_t*>(nullptr), s, "learned_code_hist");
+  // stable code passes: low byte (reads the raw keys; counts the high byte
+  // per chain of the second pass), then high byte
   uint32_t* w0 = scratch_t<uint32_t>(S_L0, n);
   uint32_t* i0 = scratch_t<uint32_t>(S_L1, n);
   uint32_t* w1 = scratch_t<uint32_t>(S_L2, n);
   uint32_t* i1 = scratch_t<uint32_t>(S_L3, n);
   const size_t tiles = (n + SortGeom<int32_t>::TILE - 1) / SortGeom<int32_t>::TILE;
   uint64_t* lb = lookback_words(tiles * 256, s);
+  hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, hist, bins);
+  NFH_LAUNCH_CHECK();
   onesweep_pass<int32_t, true, false>(keys, nullptr, nullptr, w0, reinterpret_cast<int32_t*>(i0),
-                                      nullptr, bins, lb, ctr, n, LearnedDigit{map, 0}, false, s,
-                                      "learned_pass");
+                                      nullptr, bins, lb, ctr, n, LearnedDigit{map, 0, 8},
+                                      two ? hist + CH : nullptr, false, s, "learned_pass");
   const uint32_t* W = w0;
   const uint32_t* I = i0;
-  if (bits > 8) {
+  if (two) {
+    hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, hist + CH, bins + CH);
+    NFH_LAUNCH_CHECK();
     onesweep_pass<int32_t, false, false>(nullptr, w0, reinterpret_cast<const int32_t*>(i0), w1,
-                                         reinterpret_cast<int32_t*>(i1), nullptr, bins + 256, lb,
-                                         ctr + 1, n, LearnedDigit{map, 8}, false, s,
-                                         "learned_pass");
+                                         reinterpret_cast<int32_t*>(i1), nullptr, bins + CH, lb,
+                                         ctr + NCHAIN, n, LearnedDigit{map, 8, 0}, nullptr,
+                                         false, s, "learned_pass");
     W = w1;
     I = i1;
   }
@@ -854,7 +793,7 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   {
     ProfScope prof("learned_bucket_sort", s, double(n) * 12.0);
     hipLaunchKernelGGL(k_bucket_sort, dim3(B), dim3(LS_BLOCK), 0, s, W, I, bstart, perm, large,
-                       ctr + 2);
+                       nlh);
     NFH_LAUNCH_CHECK();
   }
   // over-capacity buckets on the device: copied in chunks (equal keys are
@@ -862,14 +801,14 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   // ping-pong pair is the temporary of those global-memory passes
   {
     ProfScope prof("learned_large_prep", s);
-    hipLaunchKernelGGL(k_large_prep, dim3(1), dim3(LS_PREP_BLOCK), 0, s, bstart, large, ctr + 2,
+    hipLaunchKernelGGL(k_large_prep, dim3(1), dim3(LS_PREP_BLOCK), 0, s, bstart, large, nlh,
                        cst, unequal);
     NFH_LAUNCH_CHECK();
   }
   {
     ProfScope prof("learned_large_copy", s);
     hipLaunchKernelGGL(k_large_copy, dim3(LS_COPY_GRID), dim3(256), 0, s, W, I, bstart, large,
-                       ctr + 2, cst, perm, unequal);
+                       nlh, cst, perm, unequal);
     NFH_LAUNCH_CHECK();
   }
   {
@@ -878,11 +817,11 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
     uint32_t* ti = W == w1 ? i0 : i1;
     hipLaunchKernelGGL(k_large_sort, dim3(LS_LARGE_GRID), dim3(LS_BLOCK), 0, s,
                        const_cast<uint32_t*>(W), const_cast<uint32_t*>(I), tw, ti, bstart, large,
-                       ctr + 2, unequal, perm, huge, ctr + 3);
+                       nlh, unequal, perm, huge, nlh + 1);
     NFH_LAUNCH_CHECK();
   }
   uint32_t counts[2];
-  NFH_CHECK(hipMemcpyAsync(counts, ctr + 2, 8, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipMemcpyAsync(counts, nlh, 8, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipStreamSynchronize(s));
   const uint32_t nl = counts[0], nh = counts[1];
   std::vector<uint32_t> st;

@@ -49,21 +49,6 @@ __global__ __launch_bounds__(256) void k_hist(const K* __restrict__ keys, uint32
     if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
-// One block per digit position: exclusive scan of its 256 bins.
-__global__ __launch_bounds__(256) void k_bins(const uint32_t* __restrict__ hist, uint32_t n,
-                                              uint32_t* __restrict__ bin_start,
-                                              uint32_t* __restrict__ trivial) {
-  __shared__ uint32_t s_tmp[4];
-  const int p = blockIdx.x;
-  const uint32_t c = hist[p * 256 + threadIdx.x];
-  uint32_t total;
-  const uint32_t ex = block_excl_scan<4>(c, s_tmp, &total);
-  bin_start[p * 256 + threadIdx.x] = ex;
-  if (threadIdx.x == 0) trivial[p] = 0;
-  __syncthreads();
-  if (c == n) trivial[p] = 1;
-}
-
 template <class K>
 __global__ void k_iota(int32_t* __restrict__ perm, K* __restrict__ kout,
                        const K* __restrict__ keys, uint32_t n) {
@@ -98,65 +83,75 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
   using T = Key<K>;
   using W = typename T::W;
   constexpr int P = T::PASSES;
+  constexpr size_t CH = size_t(NCHAIN) * 256;
   const uint32_t n = uint32_t(n_);
   if (n == 0) return;
-  // meta: hist[P*256] | bin_start[P*256] | tile counters[P] | trivial[P]
-  uint32_t* meta = scratch_t<uint32_t>(S_META, size_t(P) * 512 + 2 * P);
-  uint32_t* hist = meta;
-  uint32_t* bins = meta + P * 256;
-  uint32_t* ctrs = meta + P * 512;
-  uint32_t* triv = ctrs + P;
-  fill_u32(meta, 0, size_t(P) * 512 + 2 * P, s);
-  const unsigned hb = grid_for(n, 256, 4u * unsigned(num_cus()));
-  {
-    ProfScope prof("radix_hist", s, double(n) * sizeof(K));
-    hipLaunchKernelGGL((k_hist<K>), dim3(hb), dim3(256), 0, s, keys, n, desc, hist);
-    NFH_LAUNCH_CHECK();
-  }
-  hipLaunchKernelGGL(k_bins, dim3(P), dim3(256), 0, s, hist, n, bins, triv);
-  NFH_LAUNCH_CHECK();
-  uint32_t* h_triv = static_cast<uint32_t*>(pinned(P * sizeof(uint32_t)));
-  NFH_CHECK(hipMemcpyAsync(h_triv, triv, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  // meta: or / and words | per pass: per-chain hist | per pass: bins | tile counters
+  constexpr size_t OW = 2 * sizeof(W) / 4;
+  uint32_t* meta = scratch_t<uint32_t>(S_META, OW + 2 * P * CH + P * NCHAIN);
+  W* orand = reinterpret_cast<W*>(meta);
+  uint32_t* hist = meta + OW;
+  uint32_t* bins = hist + P * CH;
+  uint32_t* ctrs = bins + P * CH;
+  fill_u32(meta, 0, OW / 2, s);
+  fill_u32(meta + OW / 2, 0xFFFFFFFFu, OW / 2, s);
+  fill_u32(hist, 0, 2 * P * CH + P * NCHAIN, s);
+  // per-chain histogram of digit 0 and the OR / AND of all key words
+  chain_hist_first<K>(keys, n, RadixDigit{0, 0}, desc, hist, orand, s, "radix_hist");
+  W* h_oa = static_cast<W*>(pinned(2 * sizeof(W)));
+  NFH_CHECK(hipMemcpyAsync(h_oa, orand, 2 * sizeof(W), hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipStreamSynchronize(s));
+  const W differ = h_oa[0] ^ h_oa[1];
   int active[P];
   int na = 0;
-  for (int p = 0; p < P; ++p)
-    if (!h_triv[p]) active[na++] = p;
+  for (int p = 0; p < P; ++p)  // a digit every key shares is an identity pass
+    if ((differ >> (8 * p)) & W(255)) active[na++] = p;
   if (na == 0) {
     hipLaunchKernelGGL((k_iota<K>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, perm,
                        sorted_out, keys, n);
     NFH_LAUNCH_CHECK();
     return;
   }
+  if (active[0] != 0)  // rare: digit 0 is shared, count the first active one
+    chain_hist_first<K>(keys, n, RadixDigit{8 * active[0], 0}, desc, hist + CH,
+                        static_cast<W*>(nullptr), s, "radix_hist");
   const size_t tiles = (n + SortGeom<K>::TILE - 1) / SortGeom<K>::TILE;
   uint64_t* lb = lookback_words(tiles * 256, s);
   // Intermediate pairs: A = (t1, tp), B = (t0, perm).  The last pass must
   // read A so that it can write perm; k = na-1 intermediate outputs.
+  // Pass a's per-chain counts are hist + CH * slot(a): slot 0 holds digit
+  // 0's from the first histogram, so pass a uses slot a + (active[0] != 0).
   const int k = na - 1;
+  const int off = active[0] != 0 ? 1 : 0;
   const W* cw = nullptr;
   const int32_t* ci = nullptr;
   for (int a = 0; a < na; ++a) {
     const int p = active[a];
     const bool first = (a == 0), last = (a == na - 1);
-    const uint32_t* bs = bins + p * 256;
-    uint32_t* ctr = ctrs + p;
+    uint32_t* h = hist + CH * size_t(a + off);
+    uint32_t* nh = last ? nullptr : hist + CH * size_t(a + 1 + off);
+    uint32_t* bs = bins + CH * size_t(a);
+    uint32_t* ctr = ctrs + NCHAIN * a;
+    const RadixDigit dg{8 * p, last ? 0 : 8 * active[a + 1]};
+    hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, h, bs);
+    NFH_LAUNCH_CHECK();
     if (last) {
       if (first)
         onesweep_pass<K, true, true>(keys, nullptr, nullptr, nullptr, perm, sorted_out, bs, lb,
-                                     ctr, n, RadixDigit{8 * p}, desc, s);
+                                     ctr, n, dg, nh, desc, s);
       else
         onesweep_pass<K, false, true>(nullptr, cw, ci, nullptr, perm, sorted_out, bs, lb, ctr,
-                                      n, RadixDigit{8 * p}, desc, s);
+                                      n, dg, nh, desc, s);
     } else {
       const bool use_a = ((k - 1 - a) % 2) == 0;
       W* ow = use_a ? t1 : t0;
       int32_t* oi = use_a ? tp : perm;
       if (first)
         onesweep_pass<K, true, false>(keys, nullptr, nullptr, ow, oi, nullptr, bs, lb, ctr, n,
-                                      RadixDigit{8 * p}, desc, s);
+                                      dg, nh, desc, s);
       else
-        onesweep_pass<K, false, false>(nullptr, cw, ci, ow, oi, nullptr, bs, lb, ctr, n,
-                                       RadixDigit{8 * p}, desc, s);
+        onesweep_pass<K, false, false>(nullptr, cw, ci, ow, oi, nullptr, bs, lb, ctr, n, dg, nh,
+                                       desc, s);
       cw = ow;
       ci = oi;
     }
