@@ -20,6 +20,7 @@
 #include "device.hpp"
 #include "gbpart.hpp"
 #include "hot.hpp"
+#include "onesweep.hpp"
 #include "runtime.hpp"
 #include "sort.hpp"
 
@@ -28,6 +29,9 @@ namespace nfh {
 typedef unsigned long long u64;
 
 constexpr int GP_BLOCK = 256;
+// the histogram pass: 512 threads, two workgroups per CU (about 54 KB of
+// LDS each with a 2^24-key span)
+constexpr int GP_HBLOCK = 512;
 // Partition passes use 8192-row tiles (512 threads, ~70 KB of LDS) so a
 // digit's run of records fills whole 128-byte lines when written out; 512
 // threads leave the kernel a 256-VGPR budget (1024 would cap it at 128 and
@@ -37,7 +41,7 @@ constexpr int GP_WAVES = GP_PBLOCK / WAVE;
 constexpr int GP_IPT = 16;
 constexpr int GP_TILE = GP_PBLOCK * GP_IPT;
 constexpr size_t GP_PART_LDS = size_t(GP_TILE) * 8 + GP_WAVES * 128 * 4 + 128 * 4 * 2 +
-                               GP_WAVES * 4 + 16;
+                               GP_WAVES * 4 + 16 + size_t(NCHAIN) * 128 * 4;
 static_assert(HOT_MAX == 512 && HOT_TBL == 1024, "LDS budget of the passes below");
 
 // ------------------------------------------------------------ histogram
@@ -52,17 +56,19 @@ struct HotAcc {
 // One persistent pass over the rows: hot rows aggregate in LDS (flushed
 // once per workgroup), every other valid row is counted in its 2048-key
 // bucket.
-__global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
+__global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n, int32_t kmin, uint32_t nb,
     const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc,
-    bool vec) {
+    uint32_t* __restrict__ chist, uint32_t ptiles, bool vec) {
   extern __shared__ uint32_t s_h[];  // nb counters
+  __shared__ uint32_t s_c0[NCHAIN][128];  // digit-0 counts per chain of the first pass
   __shared__ int32_t t_key[HOT_TBL], t_idx[HOT_TBL];
   __shared__ u64 h_sum[HOT_MAX];
   __shared__ uint32_t h_cnt[HOT_MAX];
   __shared__ int32_t h_mn[HOT_MAX], h_mx[HOT_MAX];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) s_h[b] = 0;
+  for (uint32_t b = threadIdx.x; b < NCHAIN * 128; b += blockDim.x) (&s_c0[0][0])[b] = 0;
   hot_load(table, t_key, t_idx);
   for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
     h_sum[b] = 0;
@@ -71,7 +77,9 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
     h_mx[b] = INT32_MIN;
   }
   __syncthreads();
-  auto row = [&](int32_t k, int32_t x, bool ok) {
+  // chain of row i's tile in the first partition pass
+  auto chain_of_row = [&](uint32_t i) { return chain_of_tile(ptiles, i / uint32_t(GP_TILE)); };
+  auto row = [&](int32_t k, int32_t x, bool ok, uint32_t ch) {
     if (!ok) return;
     const int hi = table ? hot_index(t_key, t_idx, k) : -1;
     if (hi >= 0) {
@@ -80,7 +88,9 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
       atomicMin(&h_mn[hi], x);
       atomicMax(&h_mx[hi], x);
     } else {
-      atomicAdd(&s_h[uint32_t(k - kmin) >> GP_BUCKET_BITS], 1u);
+      const uint32_t b = uint32_t(k - kmin) >> GP_BUCKET_BITS;
+      atomicAdd(&s_h[b], 1u);
+      atomicAdd(&s_c0[ch][b & 127u], 1u);
     }
   };
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,30 +108,37 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
       const int4 vb = reinterpret_cast<const int4*>(vals)[q + stride];
       const uint32_t ma = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
       const uint32_t mb = vm ? reinterpret_cast<const uint32_t*>(vm)[q + stride] : 0x01010101u;
-      row(ka.x, va.x, ma & 0xFFu);
-      row(ka.y, va.y, ma & 0xFF00u);
-      row(ka.z, va.z, ma & 0xFF0000u);
-      row(ka.w, va.w, ma & 0xFF000000u);
-      row(kb.x, vb.x, mb & 0xFFu);
-      row(kb.y, vb.y, mb & 0xFF00u);
-      row(kb.z, vb.z, mb & 0xFF0000u);
-      row(kb.w, vb.w, mb & 0xFF000000u);
+      const uint32_t ca = chain_of_row(4 * q), cb = chain_of_row(4 * (q + stride));
+      row(ka.x, va.x, ma & 0xFFu, ca);
+      row(ka.y, va.y, ma & 0xFF00u, ca);
+      row(ka.z, va.z, ma & 0xFF0000u, ca);
+      row(ka.w, va.w, ma & 0xFF000000u, ca);
+      row(kb.x, vb.x, mb & 0xFFu, cb);
+      row(kb.y, vb.y, mb & 0xFF00u, cb);
+      row(kb.z, vb.z, mb & 0xFF0000u, cb);
+      row(kb.w, vb.w, mb & 0xFF000000u, cb);
     }
     for (; q < nq; q += stride) {
       const int4 ka = reinterpret_cast<const int4*>(keys)[q];
       const int4 va = reinterpret_cast<const int4*>(vals)[q];
       const uint32_t ma = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
-      row(ka.x, va.x, ma & 0xFFu);
-      row(ka.y, va.y, ma & 0xFF00u);
-      row(ka.z, va.z, ma & 0xFF0000u);
-      row(ka.w, va.w, ma & 0xFF000000u);
+      const uint32_t ca = chain_of_row(4 * q);
+      row(ka.x, va.x, ma & 0xFFu, ca);
+      row(ka.y, va.y, ma & 0xFF00u, ca);
+      row(ka.z, va.z, ma & 0xFF0000u, ca);
+      row(ka.w, va.w, ma & 0xFF000000u, ca);
     }
     tail = nq * 4;
   }
-  for (uint32_t i = tail + tid; i < n; i += stride) row(keys[i], vals[i], !vm || vm[i]);
+  for (uint32_t i = tail + tid; i < n; i += stride)
+    row(keys[i], vals[i], !vm || vm[i], chain_of_row(i));
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
+  for (uint32_t b = threadIdx.x; b < NCHAIN * 128; b += blockDim.x) {
+    const uint32_t c = (&s_c0[0][0])[b];
+    if (c) atomicAdd(&chist[b], c);
+  }
   if (table) {
     for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
       const uint32_t c = h_cnt[b];
@@ -134,15 +151,11 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_hist(
   }
 }
 
-// Bucket starts (exclusive scan of hist) and the per-digit bin starts of
-// the partition passes (digit p = bits [7p, 7p+7) of the bucket id).
+// Bucket starts: exclusive scan of hist (bstart[nb] = number of records).
 __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ hist, uint32_t nb,
-                                                  uint32_t* __restrict__ bstart,
-                                                  uint32_t* __restrict__ dstart /* 2 x 128 */) {
+                                                  uint32_t* __restrict__ bstart) {
   __shared__ uint32_t s_tmp[16];
   __shared__ uint32_t s_carry;
-  __shared__ uint32_t s_d[2][128];
-  if (threadIdx.x < 256) s_d[threadIdx.x >> 7][threadIdx.x & 127] = 0;
   if (threadIdx.x == 0) s_carry = 0;
   __syncthreads();
   for (uint32_t base = 0; base < nb; base += blockDim.x) {
@@ -150,41 +163,33 @@ __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ h
     const uint32_t c = b < nb ? hist[b] : 0u;
     uint32_t tot;
     const uint32_t ex = block_excl_scan<16>(c, s_tmp, &tot);
-    if (b < nb) {
-      bstart[b] = s_carry + ex;
-      if (c) {
-        atomicAdd(&s_d[0][b & 127], c);
-        atomicAdd(&s_d[1][(b >> 7) & 127], c);
-      }
-    }
+    if (b < nb) bstart[b] = s_carry + ex;
     __syncthreads();
     if (threadIdx.x == 0) s_carry += tot;
     __syncthreads();
   }
   if (threadIdx.x == 0) bstart[nb] = s_carry;
-  if (threadIdx.x < 2) {
-    uint32_t run = 0;
-    for (int d = 0; d < 128; ++d) {
-      const uint32_t c = s_d[threadIdx.x][d];
-      dstart[threadIdx.x * 128 + d] = run;
-      run += c;
-    }
-  }
 }
 
 // ------------------------------------------------------------ partition pass
 
 // One onesweep pass over records ordered by digit (o >> (11 + 7p)) & 127.
 // FIRST reads raw rows: NULL rows and hot rows (already aggregated by
-// k_gp_hist) are dropped, the others become records.
-template <bool FIRST, bool LR>
+// k_gp_hist) are dropped, the others become records.  The pass runs in
+// NCHAIN look-back chains of tiles (one per XCD, as in onesweep.hpp):
+// dstart = [NCHAIN][128] global starts from per-chain digit counts,
+// tile_ctr = NCHAIN zeroed counters; persistent workgroups loop over their
+// chain's tiles.  NEXT: count each record's next digit per chain of the
+// next pass (next_tiles tiles of the same size) into next_hist[NCHAIN][128].
+template <bool FIRST, bool LR, bool NEXT>
 __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n_rows, int32_t kmin,
     const uint32_t* __restrict__ oin, const int32_t* __restrict__ vin, uint32_t n_rec,
     uint32_t* __restrict__ oout, int32_t* __restrict__ vout, int shift,
     const uint32_t* __restrict__ dstart, uint64_t* __restrict__ lookback,
-    uint32_t* __restrict__ tile_ctr, uint32_t epoch, const int2* __restrict__ table) {
+    uint32_t* __restrict__ tile_ctr, uint32_t epoch, const int2* __restrict__ table,
+    uint32_t next_tiles, uint32_t* __restrict__ next_hist) {
   extern __shared__ __align__(16) unsigned char s_raw[];
   uint32_t* s_o = reinterpret_cast<uint32_t*>(s_raw);
   int32_t* s_v = reinterpret_cast<int32_t*>(s_o + GP_TILE);
@@ -193,133 +198,162 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
   uint32_t* s_gbase = s_tstart + 128;
   uint32_t* s_tmp = s_gbase + 128;
   uint32_t* s_tile = s_tmp + GP_WAVES;
+  uint32_t* s_nh = s_tile + 4;  // [NCHAIN][128]
   __shared__ int32_t t_key[FIRST ? HOT_TBL : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-  if (tid == 0) *s_tile = atomicAdd(tile_ctr, 1u);
-  for (int i = tid; i < GP_WAVES * 128; i += GP_PBLOCK) s_wh[i >> 7][i & 127] = 0;
-  if constexpr (FIRST) hot_load(table, t_key, nullptr);
-  __syncthreads();
-  const uint32_t tile = *s_tile;
   const uint32_t n = FIRST ? n_rows : n_rec;
-  const uint32_t base = tile * uint32_t(GP_TILE);
-  const uint32_t wbase = base + uint32_t(wave) * (WAVE * GP_IPT);
-
-  uint32_t ov[GP_IPT];
-  int32_t vv[GP_IPT];
-  uint32_t live = 0;  // bit j: item j is a record of this pass
-  if constexpr (FIRST) {
-    // key and mask loads first (independent, coalesced per item), then the
-    // filter, then the value loads of the records only -- all issued before
-    // the ranking so their latency hides behind it
-    // Every lane loads from an in-range row (past the end: the last row),
-    // so all loads are in flight at once; values are loaded for every row
-    // (a 64-byte segment with no record at all is rare, so this fetches
-    // the same lines as loading records only).
-    uint32_t mb[GP_IPT];
+  const uint32_t T = (n + GP_TILE - 1) / GP_TILE;
+  const uint32_t chain = blockIdx.x % NCHAIN;
+  const uint32_t cfirst = chain_first(T, chain);
+  const uint32_t ctiles = chain_tiles(T, chain);
+  const uint32_t* bins = dstart + chain * 128;
+  uint32_t nb[NCHAIN];
 #pragma unroll
-    for (int j = 0; j < GP_IPT; ++j) {
-      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      const uint32_t pc = pos < n ? pos : n - 1u;
-      ov[j] = uint32_t(keys[pc]);
-      vv[j] = vals[pc];
-      mb[j] = 1u;
-    }
-    if (vm != nullptr) {
+  for (int k = 0; k < NCHAIN; ++k) nb[k] = chain_first(next_tiles, k) * uint32_t(GP_TILE);
+  if constexpr (NEXT)
+    for (int i = tid; i < NCHAIN * 128; i += GP_PBLOCK) s_nh[i] = 0;
+  if constexpr (FIRST) hot_load(table, t_key, nullptr);
+
+  for (;;) {
+    if (tid == 0) *s_tile = atomicAdd(&tile_ctr[chain], 1u);
+    for (int i = tid; i < GP_WAVES * 128; i += GP_PBLOCK) s_wh[i >> 7][i & 127] = 0;
+    __syncthreads();
+    const uint32_t local = *s_tile;
+    if (local >= ctiles) break;  // uniform over the workgroup
+    const uint32_t tile = cfirst + local;
+    const bool head = local == 0;
+    const uint32_t base = tile * uint32_t(GP_TILE);
+    const uint32_t wbase = base + uint32_t(wave) * (WAVE * GP_IPT);
+
+    uint32_t ov[GP_IPT];
+    int32_t vv[GP_IPT];
+    uint32_t live = 0;  // bit j: item j is a record of this pass
+    if constexpr (FIRST) {
+      // Every lane loads from an in-range row (past the end: the last
+      // row), so all loads are in flight at once; values are loaded for
+      // every row (a 64-byte segment with no record at all is rare, so
+      // this fetches the same lines as loading records only).
+      uint32_t mb[GP_IPT];
 #pragma unroll
       for (int j = 0; j < GP_IPT; ++j) {
         const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-        mb[j] = vm[pos < n ? pos : n - 1u];
+        const uint32_t pc = pos < n ? pos : n - 1u;
+        ov[j] = uint32_t(keys[pc]);
+        vv[j] = vals[pc];
+        mb[j] = 1u;
       }
-    }
+      if (vm != nullptr) {
 #pragma unroll
-    for (int j = 0; j < GP_IPT; ++j) {
-      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      if (pos < n && mb[j] != 0u) {
-        const int32_t k = int32_t(ov[j]);
-        if (!table || hot_slot_of(t_key, k) < 0) {
-          ov[j] = uint32_t(k - kmin);
-          live |= 1u << j;
+        for (int j = 0; j < GP_IPT; ++j) {
+          const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+          mb[j] = vm[pos < n ? pos : n - 1u];
         }
       }
-      if (!((live >> j) & 1u)) vv[j] = 0;
+#pragma unroll
+      for (int j = 0; j < GP_IPT; ++j) {
+        const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+        if (pos < n && mb[j] != 0u) {
+          const int32_t k = int32_t(ov[j]);
+          if (!table || hot_slot_of(t_key, k) < 0) {
+            ov[j] = uint32_t(k - kmin);
+            live |= 1u << j;
+          }
+        }
+        if (!((live >> j) & 1u)) vv[j] = 0;
+      }
     }
-  }
-  if constexpr (!FIRST) {
+    if constexpr (!FIRST) {
+#pragma unroll
+      for (int j = 0; j < GP_IPT; ++j) {
+        const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+        const uint32_t pc = pos < n ? pos : n - 1u;
+        ov[j] = oin[pc];
+        vv[j] = vin[pc];
+        live |= uint32_t(pos < n) << j;
+      }
+    }
+
+    // Stable in-wave ranks (position order: item j of lane l, then lane
+    // l+1, then item j+1).  LR: one lane-ordered LDS atomic per record,
+    // else the ballot digit match.  Dropped rows are not a tail of the
+    // wave, so the atomic form is used directly.
+    uint32_t rk[GP_IPT];
 #pragma unroll
     for (int j = 0; j < GP_IPT; ++j) {
-      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      const bool in = pos < n;
-      ov[j] = in ? oin[pos] : 0u;
-      vv[j] = in ? vin[pos] : 0;
-      live |= uint32_t(in) << j;
-    }
-  }
-
-  // Stable in-wave ranks (position order: item j of lane l, then lane l+1,
-  // then item j+1).  LR: one lane-ordered LDS atomic per record, else the
-  // ballot digit match.  Dropped rows are not a tail of the wave, so the
-  // atomic form is used directly (no uniform-digit shortcut).
-  uint32_t rk[GP_IPT];
-#pragma unroll
-  for (int j = 0; j < GP_IPT; ++j) {
-    const bool valid = (live >> j) & 1u;
-    const uint32_t d = (ov[j] >> shift) & 127u;
-    if constexpr (LR) {
-      rk[j] = valid ? atomicAdd(&s_wh[wave][d], 1u) : 0u;
-    } else {
-      const uint64_t peers = match_digit8(d, valid);
-      const uint32_t r = __popcll(peers & lanemask_lt());
-      const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
-      uint32_t old = 0;
-      if (valid && r == 0) {
-        old = s_wh[wave][d];
-        s_wh[wave][d] = old + uint32_t(__popcll(peers));
-      }
-      old = __shfl(old, leader);
-      rk[j] = old + r;
-    }
-  }
-  __syncthreads();
-
-  uint32_t cnt = 0;
-  if (tid < 128) {
-#pragma unroll
-    for (int w = 0; w < GP_WAVES; ++w) {
-      const uint32_t c = s_wh[w][tid];
-      s_wh[w][tid] = cnt;
-      cnt += c;
-    }
-    lb_store(lookback + size_t(tile) * 128 + tid,
-             lb_pack(epoch, tile == 0 ? LB_PREFIX : LB_AGG, cnt));
-  }
-  uint32_t tile_total;
-  const uint32_t tstart = block_excl_scan<GP_WAVES>(tid < 128 ? cnt : 0u, s_tmp, &tile_total);
-  if (tid < 128) s_tstart[tid] = tstart;
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < GP_IPT; ++j) {
-    if ((live >> j) & 1u) {
+      const bool valid = (live >> j) & 1u;
       const uint32_t d = (ov[j] >> shift) & 127u;
-      const uint32_t p = s_tstart[d] + s_wh[wave][d] + rk[j];
-      s_o[p] = ov[j];
-      s_v[p] = vv[j];
+      if constexpr (LR) {
+        rk[j] = valid ? atomicAdd(&s_wh[wave][d], 1u) : 0u;
+      } else {
+        const uint64_t peers = match_digit8(d, valid);
+        const uint32_t r = __popcll(peers & lanemask_lt());
+        const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+        uint32_t old = 0;
+        if (valid && r == 0) {
+          old = s_wh[wave][d];
+          s_wh[wave][d] = old + uint32_t(__popcll(peers));
+        }
+        old = __shfl(old, leader);
+        rk[j] = old + r;
+      }
     }
-  }
-  if (tid < 128) {
-    uint32_t excl = 0;
-    if (tile > 0) {
-      excl = lb_lookback(lookback + tid, tile, 128, epoch);
-      lb_store(lookback + size_t(tile) * 128 + tid, lb_pack(epoch, LB_PREFIX, excl + cnt));
+    __syncthreads();
+
+    uint32_t cnt = 0;
+    if (tid < 128) {
+#pragma unroll
+      for (int w = 0; w < GP_WAVES; ++w) {
+        const uint32_t c = s_wh[w][tid];
+        s_wh[w][tid] = cnt;
+        cnt += c;
+      }
+      lb_store(lookback + size_t(tile) * 128 + tid,
+               lb_pack(epoch, head ? LB_PREFIX : LB_AGG, cnt));
     }
-    s_gbase[tid] = dstart[tid] + excl - tstart;
+    uint32_t tile_total;
+    const uint32_t tstart = block_excl_scan<GP_WAVES>(tid < 128 ? cnt : 0u, s_tmp, &tile_total);
+    if (tid < 128) s_tstart[tid] = tstart;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < GP_IPT; ++j) {
+      if ((live >> j) & 1u) {
+        const uint32_t d = (ov[j] >> shift) & 127u;
+        const uint32_t p = s_tstart[d] + s_wh[wave][d] + rk[j];
+        s_o[p] = ov[j];
+        s_v[p] = vv[j];
+      }
+    }
+    if (tid < 128) {
+      uint32_t excl = 0;
+      if (!head) {
+        excl = lb_lookback(lookback + tid, tile, 128, epoch);
+        lb_store(lookback + size_t(tile) * 128 + tid, lb_pack(epoch, LB_PREFIX, excl + cnt));
+      }
+      s_gbase[tid] = bins[tid] + excl - tstart;
+    }
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < tile_total; i0 += GP_PBLOCK) {  // uniform trip count
+      const uint32_t i = i0 + uint32_t(tid);
+      const bool valid = i < tile_total;
+      const uint32_t o = valid ? s_o[i] : 0u;
+      const uint32_t g = s_gbase[(o >> shift) & 127u] + i;
+      if (valid) {
+        oout[g] = o;
+        vout[g] = s_v[i];
+      }
+      if constexpr (NEXT) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+        hist_add(s_nh, c * 128u + ((o >> (shift + 7)) & 127u), valid);
+      }
+    }
+    __syncthreads();  // the tile buffers are reused by the next tile
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < tile_total; i += GP_PBLOCK) {
-    const uint32_t o = s_o[i];
-    const uint32_t g = s_gbase[(o >> shift) & 127u] + i;
-    oout[g] = o;
-    vout[g] = s_v[i];
+  if constexpr (NEXT) {
+    for (int i = tid; i < NCHAIN * 128; i += GP_PBLOCK)
+      if (s_nh[i]) atomicAdd(&next_hist[i], s_nh[i]);
   }
 }
 
@@ -402,18 +436,22 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   const uint32_t n = uint32_t(n_);
   const uint32_t nb = uint32_t((span + (1u << GP_BUCKET_BITS) - 1) >> GP_BUCKET_BITS);
   const int passes = nb <= 1 ? 0 : (nb <= 128 ? 1 : 2);
-  // scratch: hist[nb] | bstart[nb+1] | dstart[256] | ctr[2] | hot_acc
-  const size_t meta_words = size_t(nb) * 2 + 1 + 256 + 4;
-  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + HOT_MAX * 6);
+  constexpr size_t CH = size_t(NCHAIN) * 128;
+  // scratch: hist[nb] | bstart[nb+1] | per-chain digit counts [2][CH] |
+  // per-chain digit starts [2][CH] | tile counters [2][NCHAIN] | hot_acc
+  const size_t meta_words = size_t(nb) * 2 + 1 + 4 * CH + 2 * NCHAIN;
+  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + 4 + HOT_MAX * 6);
   uint32_t* hist = meta;
   uint32_t* bstart = hist + nb;
-  uint32_t* dstart = bstart + nb + 1;
-  uint32_t* ctr = dstart + 256;
-  HotAcc* hot_acc = reinterpret_cast<HotAcc*>(ctr + 4 + (4 - ((meta_words) & 3)));
+  uint32_t* chist = bstart + nb + 1;
+  uint32_t* dstart = chist + 2 * CH;
+  uint32_t* ctr = dstart + 2 * CH;
+  HotAcc* hot_acc = reinterpret_cast<HotAcc*>(meta + ((meta_words + 3) & ~size_t(3)));
   fill_u32(meta, 0, meta_words, s);
   hipLaunchKernelGGL(k_gp_hotinit, dim3(2), dim3(256), 0, s, hot_acc, uint32_t(HOT_MAX));
   NFH_LAUNCH_CHECK();
   const double mask_b = vm ? 1.0 : 0.0;
+  const uint32_t tiles = (n + GP_TILE - 1) / GP_TILE;
   {
     // algorithmic bytes: key + value + validity per row
     ProfScope prof("gp_hist", s, double(n) * (8.0 + mask_b));
@@ -421,12 +459,14 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
     const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16) == 0 &&
                      (reinterpret_cast<uintptr_t>(vals) % 16) == 0 &&
                      (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
-    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_BLOCK, 3 * num_cus())),
-                       dim3(GP_BLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
-                       vec);
+    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_HBLOCK, 2 * num_cus())),
+                       dim3(GP_HBLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
+                       chist, tiles, vec);
     NFH_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart, dstart);
+  hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_chain_bins<128>, dim3(1), dim3(128), 0, s, chist, dstart);
   NFH_LAUNCH_CHECK();
   // records = valid non-hot rows (bstart[nb])
   uint32_t* h_m = static_cast<uint32_t*>(pinned(4));
@@ -438,42 +478,52 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   int32_t* v0 = scratch_t<int32_t>(S_T1, n);
   uint32_t* o1 = scratch_t<uint32_t>(S_T2, n);
   int32_t* v1 = scratch_t<int32_t>(S_T3, n);
-  const uint32_t tiles = (n + GP_TILE - 1) / GP_TILE;
   uint64_t* lb = lookback_words(size_t(tiles) * 128, s);
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
-    const void* fns[4] = {reinterpret_cast<const void*>(&k_gp_part<true, true>),
-                          reinterpret_cast<const void*>(&k_gp_part<true, false>),
-                          reinterpret_cast<const void*>(&k_gp_part<false, true>),
-                          reinterpret_cast<const void*>(&k_gp_part<false, false>)};
+    const void* fns[8] = {reinterpret_cast<const void*>(&k_gp_part<true, true, true>),
+                          reinterpret_cast<const void*>(&k_gp_part<true, false, true>),
+                          reinterpret_cast<const void*>(&k_gp_part<false, true, true>),
+                          reinterpret_cast<const void*>(&k_gp_part<false, false, true>),
+                          reinterpret_cast<const void*>(&k_gp_part<true, true, false>),
+                          reinterpret_cast<const void*>(&k_gp_part<true, false, false>),
+                          reinterpret_cast<const void*>(&k_gp_part<false, true, false>),
+                          reinterpret_cast<const void*>(&k_gp_part<false, false, false>)};
     for (const void* f : fns)
       NFH_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(GP_PART_LDS)));
   });
   const bool lr = lds_rank_supported();
-  const auto part_first = lr ? &k_gp_part<true, true> : &k_gp_part<true, false>;
-  const auto part_next = lr ? &k_gp_part<false, true> : &k_gp_part<false, false>;
+  const bool two = passes == 2 && m > 0;
+  const uint32_t t2 = (m + GP_TILE - 1) / GP_TILE;
+  // persistent: two workgroups per CU (about 75 KB of LDS each), a
+  // multiple of NCHAIN
+  const unsigned grid = unsigned(std::max<int>(NCHAIN, 2 * num_cus() / NCHAIN * NCHAIN));
   // pass 1 always runs: it drops NULL and hot rows (digit 0 when nb <= 1)
   {
     // reads key + validity of every row and the value of each record,
     // writes (o, value) per record
     ProfScope prof("gp_part_first", s, double(n) * (4.0 + mask_b) + double(m) * 12.0);
     const uint32_t epoch = lookback_epoch(s);
-    hipLaunchKernelGGL(part_first, dim3(tiles), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n,
-                       kmin, nullptr, nullptr, 0u, o0, v0, GP_BUCKET_BITS, dstart, lb, ctr, epoch,
-                       table);
+    const auto fn = two ? (lr ? &k_gp_part<true, true, true> : &k_gp_part<true, false, true>)
+                        : (lr ? &k_gp_part<true, true, false> : &k_gp_part<true, false, false>);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n, kmin,
+                       nullptr, nullptr, 0u, o0, v0, GP_BUCKET_BITS, dstart, lb, ctr, epoch, table,
+                       t2, chist + CH);
     NFH_LAUNCH_CHECK();
   }
   const uint32_t* fo = o0;
   const int32_t* fv = v0;
   if (passes == 2) {
     if (m > 0) {
+      hipLaunchKernelGGL(k_chain_bins<128>, dim3(1), dim3(128), 0, s, chist + CH, dstart + CH);
+      NFH_LAUNCH_CHECK();
       ProfScope prof("gp_part", s, double(m) * 16.0);
-      const uint32_t t2 = (m + GP_TILE - 1) / GP_TILE;
       const uint32_t epoch = lookback_epoch(s);
-      hipLaunchKernelGGL(part_next, dim3(t2), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
-                         nullptr, 0u, kmin, o0, v0, m, o1, v1, GP_BUCKET_BITS + 7, dstart + 128,
-                         lb, ctr + 1, epoch, nullptr);
+      const auto fn = lr ? &k_gp_part<false, true, false> : &k_gp_part<false, false, false>;
+      hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
+                         nullptr, 0u, kmin, o0, v0, m, o1, v1, GP_BUCKET_BITS + 7, dstart + CH,
+                         lb, ctr + NCHAIN, epoch, nullptr, 0u, nullptr);
       NFH_LAUNCH_CHECK();
     }
     fo = o1;
