@@ -91,6 +91,8 @@ struct LearnedDigit {
   LearnedMap m;
   int shift, next_shift;
   static constexpr size_t EXTRA_LDS = ((KNOTS + 1) * 4 + 15) / 16 * 16;
+  // nearly sorted keys share one bucket code over long runs of rows
+  static constexpr bool PEEL = true;
   __device__ void init(unsigned char* s) const {
     uint32_t* k = reinterpret_cast<uint32_t*>(s);
     for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) k[i] = m.knots[i];
@@ -353,12 +355,12 @@ __global__ __launch_bounds__(256) void k_bucket_bounds(const uint32_t* __restric
 
 // One workgroup per bucket: stable LSD sort in LDS of the bucket-relative
 // key w - min (9-bit digits, only as many passes as max - min needs); the
-// items carry their 16-bit position in the bucket, and the rows are
-// gathered by it at the end (6 bytes of LDS per key: three workgroups per
-// CU).  Buckets over LS_CAP are listed for the large-bucket path.  Items
+// items carry their 16-bit position in the bucket, and the rows (held in
+// registers since the start) are gathered by it at the end (6 bytes of LDS
+// per key: three workgroups per CU).  Buckets over LS_CAP are listed for the large-bucket path.  Items
 // are wave-contiguous (wave w holds positions [w * 64 * LS_IPT,
 // (w + 1) * 64 * LS_IPT)) and ranked with lane-ordered LDS atomics.
-__global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __restrict__ W,
+__global__ __launch_bounds__(LS_BLOCK, 6) void k_bucket_sort(const uint32_t* __restrict__ W,
                                                              const uint32_t* __restrict__ I,
                                                              const uint32_t* __restrict__ start,
                                                              int32_t* __restrict__ perm,
@@ -379,14 +381,30 @@ __global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __r
     return;
   }
   const uint32_t wb = uint32_t(wave) * (WAVE * LS_IPT);
-  uint32_t kw[LS_IPT], lp[LS_IPT];
+  // words and row ids are both loaded now (every lane from an in-range
+  // position, so all loads are in flight at once); the rows are gathered
+  // through LDS at the end without a second trip to memory
+  // lp: the items' positions in the bucket (below 2^13), two per register
+  uint32_t kw[LS_IPT], lp[LS_IPT / 2], ix[LS_IPT];
+  static_assert(LS_IPT % 2 == 0, "positions are packed in pairs");
+  auto lp_get = [&](int j) -> uint32_t { return (lp[j >> 1] >> (16 * (j & 1))) & 0xFFFFu; };
+  auto lp_set = [&](int j, uint32_t v) {
+    lp[j >> 1] = (lp[j >> 1] & ~(0xFFFFu << (16 * (j & 1)))) | (v << (16 * (j & 1)));
+  };
+#pragma unroll
+  for (int j = 0; j < LS_IPT; ++j) {
+    const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+    const uint32_t pc = lo + (pos < size ? pos : size - 1u);
+    kw[j] = W[pc];
+    ix[j] = I[pc];
+  }
+  for (int i = tid; i < LS_WAVES * LS_DIG; i += LS_BLOCK) (&s_cnt[0][0])[i] = 0;
   uint32_t mn = 0xFFFFFFFFu, mx = 0;
 #pragma unroll
   for (int j = 0; j < LS_IPT; ++j) {
     const uint32_t pos = wb + uint32_t(j * WAVE + lane);
-    lp[j] = pos;
+    lp_set(j, pos);
     if (pos < size) {
-      kw[j] = W[lo + pos];
       mn = kw[j] < mn ? kw[j] : mn;
       mx = kw[j] > mx ? kw[j] : mx;
     } else {
@@ -414,8 +432,7 @@ __global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __r
   const int bits = range == 0 ? 0 : 32 - __clz(range);
   __syncthreads();
   for (int shift = 0; shift < bits; shift += LS_DBITS) {  // uniform over the workgroup
-    for (int i = tid; i < LS_WAVES * LS_DIG; i += LS_BLOCK) (&s_cnt[0][0])[i] = 0;
-    __syncthreads();
+    // (the counters are zero here: cleared at the start and after each pass)
     uint32_t rk[LS_IPT];
 #pragma unroll
     for (int j = 0; j < LS_IPT; ++j) {
@@ -445,7 +462,7 @@ __global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __r
         const uint32_t d = rk[j] & uint32_t(LS_DIG - 1);
         const uint32_t p = s_dst[d] + s_cnt[wave][d] + (rk[j] >> LS_DBITS);
         s_w[p] = kw[j];
-        s_l[p] = uint16_t(lp[j]);
+        s_l[p] = uint16_t(lp_get(j));
       }
     }
     __syncthreads();
@@ -454,19 +471,24 @@ __global__ __launch_bounds__(LS_BLOCK, 3) void k_bucket_sort(const uint32_t* __r
       const uint32_t pos = wb + uint32_t(j * WAVE + lane);
       if (pos < size) {
         kw[j] = s_w[pos];
-        lp[j] = s_l[pos];
+        lp_set(j, s_l[pos]);
       }
     }
+    for (int i = tid; i < LS_WAVES * LS_DIG; i += LS_BLOCK) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
   }
-  // the bucket's row ids, read coalesced into LDS (s_w is free after the
-  // last pass), then gathered in sorted order
-  for (uint32_t i = uint32_t(tid); i < size; i += LS_BLOCK) s_w[i] = I[lo + i];
+  // the bucket's row ids into LDS by position (s_w is free after the last
+  // pass), then gathered in sorted order
+#pragma unroll
+  for (int j = 0; j < LS_IPT; ++j) {
+    const uint32_t pos = wb + uint32_t(j * WAVE + lane);
+    if (pos < size) s_w[pos] = ix[j];
+  }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < LS_IPT; ++j) {
     const uint32_t pos = wb + uint32_t(j * WAVE + lane);
-    if (pos < size) perm[lo + pos] = int32_t(s_w[lp[j]]);
+    if (pos < size) perm[lo + pos] = int32_t(s_w[lp_get(j)]);
   }
 }
 

@@ -99,20 +99,24 @@ __global__ __launch_bounds__(R) void k_chain_bins(const uint32_t* __restrict__ h
   }
 }
 
-// Add one item's bin to an LDS histogram; a wave whose valid lanes all
-// hold one bin (nearly sorted keys) updates the counter once instead of
-// serialising its atomics on one address.  Every lane of the wave must
-// call it.
+// Add one item's bin to an LDS histogram.  A wave whose valid lanes all
+// hold one bin updates the counter once instead of serialising its atomics
+// on one address; with PEEL (see rank_lds) also the group of lanes sharing
+// the first valid lane's bin when it is at least half of the wave.  Every
+// lane of the wave must call it.
+template <bool PEEL = false>
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) {
   const uint64_t act = __ballot(valid);
   if (act == 0) return;
   const int first = __ffsll((unsigned long long)act) - 1;
   const uint32_t b0 = __builtin_amdgcn_readlane(bin, first);
-  if (__ballot(valid && bin == b0) == act) {
-    if (lane_id() == first) atomicAdd(&h[b0], uint32_t(__popcll(act)));
-  } else if (valid) {
-    atomicAdd(&h[bin], 1u);
+  const uint64_t same = __ballot(valid && bin == b0);
+  const uint32_t c = uint32_t(__popcll(same));
+  if (same == act || (PEEL && c >= 32u)) {
+    if (lane_id() == first) atomicAdd(&h[b0], c);
+    if ((same >> lane_id()) & 1u) return;
   }
+  if (valid) atomicAdd(&h[bin], 1u);
 }
 
 // Tiles are large (16384 int32 keys, 8192 int64 keys) so that each digit's
@@ -134,6 +138,7 @@ struct SortGeom {
 struct RadixDigit {
   int shift, next_shift;
   static constexpr size_t EXTRA_LDS = 0;
+  static constexpr bool PEEL = false;
   __device__ void init(unsigned char*) const {}
   template <class W>
   __device__ W code(W w, const unsigned char*) const {
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(256) void k_chain_hist(const K* __restrict__ keys,
           o |= w[j];
           a &= w[j];
         }
-        hist_add(s_h, uint32_t(dg.code(w[j], s_dg) >> dg.shift) & 255u, valid);
+        hist_add<DG::PEEL>(s_h, uint32_t(dg.code(w[j], s_dg) >> dg.shift) & 255u, valid);
       }
     }
   }
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
       const bool valid = pos < n;
       const uint32_t d = uint32_t(dg.code(kw[j], s_dg) >> dg.shift) & 255u;
       if constexpr (LR) {
-        rk[j] = (rank_lds(s_wh[wave], d, valid) << 8) | d;
+        rk[j] = (rank_lds<DG::PEEL>(s_wh[wave], d, valid) << 8) | d;
       } else {
         const uint64_t peers = match_digit8(d, valid);
         const uint32_t r = __popcll(peers & lanemask_lt());
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
         uint32_t c = 0;
 #pragma unroll
         for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
-        hist_add(&s_nh[0][0], c * 256u + (uint32_t(code >> dg.next_shift) & 255u), valid);
+        hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + (uint32_t(code >> dg.next_shift) & 255u), valid);
       }
     }
     __syncthreads();  // the tile buffers are reused by the next tile
