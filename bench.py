@@ -73,19 +73,20 @@ def kernel_table(L):
 
 
 # HBM traffic per launch of a dominant kernel, from the rocprofv3 counter
-# passes summarised in profiles/ (FETCH_SIZE x 2 + WRITE_SIZE, calibrated
-# against a 1 GiB copy): profiled scope -> (summary file, kernel rows, how).
-# "mean": one kernel per launch of the scope, dispatch-weighted over the
-# rows; "sum": every row runs once per launch of the scope.
+# passes summarised in profiles/ (L2->memory read and write requests by
+# size): profiled scope -> (summary file, kernel rows, how).  "mean": one
+# kernel per launch of the scope, dispatch-weighted over the rows; "sum":
+# every row runs once per launch of the scope.
 TRAFFIC_ROWS = {
-    "gp_part_first": ("groupby_unique_zipf_2p28.txt", ["k_gp_part<true, true>"], "mean"),
+    "unique_mark": ("groupby_unique_zipf_2p28.txt", ["k_mark<true>"], "mean"),
+    "gp_part_first": ("groupby_unique_zipf_2p28.txt", ["k_gp_part<true, true, true>"], "mean"),
     "radix_onesweep": ("argsort_uniform_2p28.txt",
-                       ["k_onesweep<int, true, false, true, RadixDigit>",
-                        "k_onesweep<int, false, false, true, RadixDigit>",
-                        "k_onesweep<int, false, true, true, RadixDigit>"], "mean"),
+                       ["k_onesweep<int, true, false, true, true, RadixDigit>",
+                        "k_onesweep<int, false, false, true, true, RadixDigit>",
+                        "k_onesweep<int, false, true, true, false, RadixDigit>"], "mean"),
     "learned_pass": ("learned_sort_2p28.txt",
-                     ["k_onesweep<int, true, false, true, LearnedDigit>",
-                      "k_onesweep<int, false, false, true, LearnedDigit>"], "mean"),
+                     ["k_onesweep<int, true, false, true, true, LearnedDigit>",
+                      "k_onesweep<int, false, false, true, false, LearnedDigit>"], "mean"),
     "ghash_pins": ("ghash_count_distinct_2p26.txt",
                    ["k_gh_dedupe", "k_gh_first", "k_gh_winkey", "k_gh_win",
                     "k_gh_materialize"], "sum"),
