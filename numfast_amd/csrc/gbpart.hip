@@ -64,6 +64,10 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   extern __shared__ uint32_t s_h[];  // nb counters
   __shared__ uint32_t s_c0[NCHAIN][128];  // digit-0 counts per chain of the first pass
   __shared__ int32_t t_key[HOT_TBL], t_idx[HOT_TBL];
+  // hot aggregates: values in [-2^15, 2^15) add (1 << 40) | (x + 2^15) to
+  // h_pk (count in bits 40..63, biased sum below: exact while a workgroup
+  // sees fewer than 2^24 rows), others go to h_sum / h_cnt
+  __shared__ u64 h_pk[HOT_MAX];
   __shared__ u64 h_sum[HOT_MAX];
   __shared__ uint32_t h_cnt[HOT_MAX];
   __shared__ int32_t h_mn[HOT_MAX], h_mx[HOT_MAX];
@@ -71,6 +75,7 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   for (uint32_t b = threadIdx.x; b < NCHAIN * 128; b += blockDim.x) (&s_c0[0][0])[b] = 0;
   hot_load(table, t_key, t_idx);
   for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
+    h_pk[b] = 0;
     h_sum[b] = 0;
     h_cnt[b] = 0;
     h_mn[b] = INT32_MAX;
@@ -83,8 +88,12 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     if (!ok) return;
     const int hi = table ? hot_index(t_key, t_idx, k) : -1;
     if (hi >= 0) {
-      atomicAdd(&h_sum[hi], (u64)(long long)x);
-      atomicAdd(&h_cnt[hi], 1u);
+      if (x >= -32768 && x < 32768) {
+        atomicAdd(&h_pk[hi], (u64(1) << 40) | u64(uint32_t(x + 32768)));
+      } else {
+        atomicAdd(&h_sum[hi], (u64)(long long)x);
+        atomicAdd(&h_cnt[hi], 1u);
+      }
       atomicMin(&h_mn[hi], x);
       atomicMax(&h_mx[hi], x);
     } else {
@@ -98,25 +107,29 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   uint32_t tail = 0;
   if (vec) {
     // quads: 128-bit key / value loads, a 32-bit load of four mask bytes,
-    // two quads in flight per thread
+    // HQ quads in flight per thread (all loaded before any row is added)
+    constexpr int HQ = 4;
     const uint32_t nq = n / 4;
     uint32_t q = tid;
-    for (; q + stride < nq; q += 2 * stride) {
-      const int4 ka = reinterpret_cast<const int4*>(keys)[q];
-      const int4 kb = reinterpret_cast<const int4*>(keys)[q + stride];
-      const int4 va = reinterpret_cast<const int4*>(vals)[q];
-      const int4 vb = reinterpret_cast<const int4*>(vals)[q + stride];
-      const uint32_t ma = vm ? reinterpret_cast<const uint32_t*>(vm)[q] : 0x01010101u;
-      const uint32_t mb = vm ? reinterpret_cast<const uint32_t*>(vm)[q + stride] : 0x01010101u;
-      const uint32_t ca = chain_of_row(4 * q), cb = chain_of_row(4 * (q + stride));
-      row(ka.x, va.x, ma & 0xFFu, ca);
-      row(ka.y, va.y, ma & 0xFF00u, ca);
-      row(ka.z, va.z, ma & 0xFF0000u, ca);
-      row(ka.w, va.w, ma & 0xFF000000u, ca);
-      row(kb.x, vb.x, mb & 0xFFu, cb);
-      row(kb.y, vb.y, mb & 0xFF00u, cb);
-      row(kb.z, vb.z, mb & 0xFF0000u, cb);
-      row(kb.w, vb.w, mb & 0xFF000000u, cb);
+    for (; q + (HQ - 1) * stride < nq; q += HQ * stride) {
+      int4 ka[HQ], va[HQ];
+      uint32_t ma[HQ];
+#pragma unroll
+      for (int u = 0; u < HQ; ++u) {
+        const uint32_t qq = q + uint32_t(u) * stride;
+        ka[u] = reinterpret_cast<const int4*>(keys)[qq];
+        va[u] = reinterpret_cast<const int4*>(vals)[qq];
+        ma[u] = vm ? reinterpret_cast<const uint32_t*>(vm)[qq] : 0x01010101u;
+      }
+      asm volatile("" ::: "memory");  // keep every load above the adds
+#pragma unroll
+      for (int u = 0; u < HQ; ++u) {
+        const uint32_t ca = chain_of_row(4 * (q + uint32_t(u) * stride));
+        row(ka[u].x, va[u].x, ma[u] & 0xFFu, ca);
+        row(ka[u].y, va[u].y, ma[u] & 0xFF00u, ca);
+        row(ka[u].z, va[u].z, ma[u] & 0xFF0000u, ca);
+        row(ka[u].w, va[u].w, ma[u] & 0xFF000000u, ca);
+      }
     }
     for (; q < nq; q += stride) {
       const int4 ka = reinterpret_cast<const int4*>(keys)[q];
@@ -141,9 +154,13 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   }
   if (table) {
     for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {
-      const uint32_t c = h_cnt[b];
+      const u64 pk = h_pk[b];
+      const uint32_t pc = uint32_t(pk >> 40);
+      const uint32_t c = h_cnt[b] + pc;
       if (c == 0) continue;
-      atomicAdd(&hot_acc[b].sum, h_sum[b]);
+      // biased sum minus 2^15 per packed row (wrapping u64 arithmetic)
+      const u64 sum = h_sum[b] + (pk & ((u64(1) << 40) - 1)) - u64(pc) * 32768u;
+      atomicAdd(&hot_acc[b].sum, sum);
       atomicAdd(&hot_acc[b].cnt, c);
       atomicMin(&hot_acc[b].mn, h_mn[b]);
       atomicMax(&hot_acc[b].mx, h_mx[b]);
@@ -377,13 +394,30 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_bucket(const uint32_t* __restri
   }
   __syncthreads();
   const uint32_t lo = bstart[b], hi = bstart[b + 1];
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += GP_BLOCK) {
-    const uint32_t s = o[i] & (W - 1);
-    const int32_t x = v[i];
-    atomicAdd(&a_sum[s], (u64)(long long)x);
-    atomicAdd(&a_cnt[s], 1u);
-    atomicMin(&a_mn[s], x);
-    atomicMax(&a_mx[s], x);
+  // RB records per thread per step, all loaded before any is added (every
+  // lane loads an in-range record), so their latency overlaps
+  constexpr int RB = 8;
+  for (uint32_t i0 = lo; i0 < hi; i0 += RB * GP_BLOCK) {
+    uint32_t os[RB];
+    int32_t xs[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const uint32_t i = i0 + uint32_t(r * GP_BLOCK) + threadIdx.x;
+      const uint32_t ic = i < hi ? i : hi - 1u;
+      os[r] = o[ic];
+      xs[r] = v[ic];
+    }
+    asm volatile("" ::: "memory");  // keep every load above the adds
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (i0 + uint32_t(r * GP_BLOCK) + threadIdx.x >= hi) continue;
+      const uint32_t s = os[r] & (W - 1);
+      const int32_t x = xs[r];
+      atomicAdd(&a_sum[s], (u64)(long long)x);
+      atomicAdd(&a_cnt[s], 1u);
+      atomicMin(&a_mn[s], x);
+      atomicMax(&a_mx[s], x);
+    }
   }
   __syncthreads();
   const uint64_t first = uint64_t(b) << GP_BUCKET_BITS;

@@ -350,7 +350,8 @@ __global__ __launch_bounds__(SC2_BLOCK) void k_slot_compact(
 #pragma unroll
   for (int j = 0; j < SC2_IPT; ++j) {
     const uint64_t s = wbase + uint64_t(j * WAVE + lane);
-    const bool present = s < m && slots[s].cnt != 0;
+    // every lane loads (past the end: the last slot), so the loads overlap
+    const bool present = slots[s < m ? s : m - 1].cnt != 0 && s < m;
     bal[j] = __ballot(present);
     wtot += uint32_t(__popcll(bal[j]));
   }
