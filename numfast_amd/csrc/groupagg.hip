@@ -21,10 +21,14 @@
 //    when the row would change them).  One scan over the slots compacts the
 //    present groups in key order.
 //  * sparse (huge span): the valid rows are compacted, radix-unique'd
-//    (unique.hip) and aggregated by code; ukeys = the unique values.
+//    (unique.hip) and aggregated by code; ukeys = the unique values;
+//  * hash-scatter (any span, few distinct keys; only when forced or
+//    selected): per-workgroup LDS hash tables flushed into one global
+//    hash table, then the present keys sorted (gbhash.hip).
 #include <algorithm>
 
 #include "device.hpp"
+#include "gbhash.hpp"
 #include "gbpart.hpp"
 #include "hot.hpp"
 #include "groupagg.hpp"
@@ -495,16 +499,17 @@ static bool build_hot_table(const int32_t* keys, size_t n, int2** table, int32_t
 // Strategy of the next group-by on this thread (0: the rule below) and the
 // one the last group-by used: 1 dense (LDS hot keys + global spill), 2
 // partitioned (bucket passes, no per-row global atomics), 3 sort-based
-// (radix unique + aggregation by code).  A forced strategy that cannot run
-// on the input (span too large, key mask for the partitioned path) falls
-// back to the rule.
+// (radix unique + aggregation by code), 4 hash-scatter (LDS hash tables +
+// one global hash table).  A forced strategy that cannot run on the input
+// (span too large, key mask for the partitioned path) falls back to the
+// rule; a hash table that overflows falls back to the sort-based path.
 namespace {
 thread_local int t_strategy = 0;
 thread_local int t_last_strategy = 0;
 }  // namespace
 int set_groupby_strategy(int s) {
   const int old = t_strategy;
-  t_strategy = (s >= 0 && s <= 3) ? s : 0;
+  t_strategy = (s >= 0 && s <= 4) ? s : 0;
   return old;
 }
 int last_groupby_strategy() { return t_last_strategy; }
@@ -559,6 +564,15 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
       NFH_LAUNCH_CHECK();
     }
     return compact_slots(slots, span, nullptr, kmin, o, s);
+  }
+  if (strat == 4) {
+    int2* table = nullptr;
+    int32_t* hot_keys = nullptr;
+    uint32_t* hot_n = nullptr;
+    if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
+    const int64_t ng = hash_group_agg(keys, vals, v, kv, n, span, table, hot_keys, hot_n, o, s);
+    if (ng >= 0) return ng;
+    t_last_strategy = 3;  // the table overflowed: sort-based
   }
   // sparse: compact valid rows, radix unique, aggregate by code
   int32_t* ck = scratch_t<int32_t>(S_WORK1, n);

@@ -7,15 +7,15 @@
 // at least 4 times, fraction of ascending neighbour pairs, log2(distinct) /
 // 16, and a constant 1.
 //
-// Classifier: a 64 -> 64 (ReLU) -> 3 fp32 MLP with softmax cross-entropy,
+// Classifier: a 64 -> 64 (ReLU) -> 4 fp32 MLP with softmax cross-entropy,
 // trained on the device: k_sel_grad (forward + backward, one sample per
 // thread, per-workgroup partial gradients in a fixed order) and k_sel_adam
 // (reduction + Adam).  Class c is group-by strategy c + 1 (dense,
-// partitioned, sort-based; groupagg.hip).  A wrong pick costs time only:
+// partitioned, sort-based, hash-scatter; groupagg.hip).  A wrong pick costs time only:
 // every strategy returns the same result.
 //
 // Parameter layout (floats): W1[64][64] (row = hidden unit), b1[64],
-// W2[3][64], b2[3].
+// W2[4][64], b2[4].
 #include <cmath>
 
 #include "device.hpp"
@@ -25,7 +25,7 @@ namespace nfh {
 
 constexpr int SF = 64;  // features
 constexpr int SH = 64;  // hidden
-constexpr int SC = 3;   // classes
+constexpr int SC = 4;   // classes
 constexpr int S_W1 = 0, S_B1 = SF * SH, S_W2 = S_B1 + SH, S_B2 = S_W2 + SC * SH,
               S_P = S_B2 + SC;
 constexpr int SEL_SAMPLE = 4096;
@@ -129,13 +129,17 @@ __global__ __launch_bounds__(1024) void k_sel_features(const int32_t* __restrict
 constexpr int SG_BLOCK = 64;  // one wave
 constexpr int SG_GRID = 64;
 
-// Forward + backward of the mean cross-entropy over (X[N][64], y[N]).
-// Workgroups loop over chunks of 64 samples; per-chunk sums in a fixed
-// order, accumulated in registers; part[blockIdx.x * (S_P + 1) ...] gets the
-// workgroup's gradient and loss share.  pred (optional): class scores.
+// Forward + backward over (X[N][64], y[N]) of the mean cross-entropy
+// (classifier), or, with Tt (targets[N][SC]) non-null, of the mean of
+// 1/2 sum_c (z_c - Tt_c)^2 (cost model: z_c predicts strategy c's centred
+// log time and the pick is the smallest).  Workgroups loop over chunks of
+// 64 samples; per-chunk sums in a fixed order, accumulated in registers;
+// part[blockIdx.x * (S_P + 1) ...] gets the workgroup's gradient and loss
+// share.
 __global__ __launch_bounds__(SG_BLOCK) void k_sel_grad(const float* __restrict__ P,
                                                        const float* __restrict__ X,
                                                        const int32_t* __restrict__ Y,
+                                                       const float* __restrict__ Tt,
                                                        uint32_t N, float inv_n,
                                                        float* __restrict__ part) {
   __shared__ float s_x[SG_BLOCK][SF + 1];
@@ -147,7 +151,9 @@ __global__ __launch_bounds__(SG_BLOCK) void k_sel_grad(const float* __restrict__
   float acc_w1[SH];  // column t of the W1 gradient
 #pragma unroll
   for (int m = 0; m < SH; ++m) acc_w1[m] = 0.f;
-  float acc_b1 = 0.f, acc_w2[SC] = {0.f, 0.f, 0.f}, acc_b2 = 0.f, acc_l = 0.f;
+  float acc_b1 = 0.f, acc_w2[SC], acc_b2 = 0.f, acc_l = 0.f;
+#pragma unroll
+  for (int c = 0; c < SC; ++c) acc_w2[c] = 0.f;
   for (uint32_t c0 = blockIdx.x * SG_BLOCK; c0 < N; c0 += gridDim.x * SG_BLOCK) {
     const uint32_t s = c0 + t;
     const bool valid = s < N;
@@ -165,22 +171,36 @@ __global__ __launch_bounds__(SG_BLOCK) void k_sel_grad(const float* __restrict__
 #pragma unroll
       for (int c = 0; c < SC; ++c) z[c] = fmaf(P[S_W2 + c * SH + i], h, z[c]);
     }
-    const float zm = fmaxf(z[0], fmaxf(z[1], z[2]));
-    float e[SC], se = 0.f;
-#pragma unroll
-    for (int c = 0; c < SC; ++c) {
-      e[c] = expf(z[c] - zm);
-      se += e[c];
-    }
-    const int y = valid ? Y[s] : 0;
     float dz[SC];
+    if (Tt != nullptr) {  // cost model (uniform over the launch)
+      float l = 0.f;
 #pragma unroll
-    for (int c = 0; c < SC; ++c) {
-      const float p = e[c] / se;
-      dz[c] = valid ? (p - (c == y ? 1.f : 0.f)) * inv_n : 0.f;
-      s_dz[t][c] = dz[c];
+      for (int c = 0; c < SC; ++c) {
+        const float d = valid ? z[c] - Tt[size_t(s) * SC + c] : 0.f;
+        dz[c] = d * inv_n;
+        l = fmaf(0.5f * d, d, l);
+        s_dz[t][c] = dz[c];
+      }
+      s_l[t] = l * inv_n;
+    } else {
+      float zm = z[0];
+#pragma unroll
+      for (int c = 1; c < SC; ++c) zm = fmaxf(zm, z[c]);
+      float e[SC], se = 0.f;
+#pragma unroll
+      for (int c = 0; c < SC; ++c) {
+        e[c] = expf(z[c] - zm);
+        se += e[c];
+      }
+      const int y = valid ? Y[s] : 0;
+#pragma unroll
+      for (int c = 0; c < SC; ++c) {
+        const float p = e[c] / se;
+        dz[c] = valid ? (p - (c == y ? 1.f : 0.f)) * inv_n : 0.f;
+        s_dz[t][c] = dz[c];
+      }
+      s_l[t] = valid ? (logf(se) + zm - z[y]) * inv_n : 0.f;
     }
-    s_l[t] = valid ? (logf(se) + zm - z[y]) * inv_n : 0.f;
 #pragma unroll 1
     for (int i = 0; i < SH; ++i) {
       float g = 0.f;
@@ -239,8 +259,10 @@ __global__ __launch_bounds__(256) void k_sel_adam(float* __restrict__ P, float* 
   }
 }
 
+// Pick per sample: the largest score (classifier) or, with lowest, the
+// smallest predicted cost.
 __global__ void k_sel_predict(const float* __restrict__ P, const float* __restrict__ X,
-                              uint32_t N, int32_t* __restrict__ cls) {
+                              uint32_t N, bool lowest, int32_t* __restrict__ cls) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   float z[SC];
@@ -253,7 +275,7 @@ __global__ void k_sel_predict(const float* __restrict__ P, const float* __restri
   }
   int best = 0;
   for (int c = 1; c < SC; ++c)
-    if (z[c] > z[best]) best = c;
+    if (lowest ? z[c] < z[best] : z[c] > z[best]) best = c;
   cls[s] = best;
 }
 
@@ -277,22 +299,21 @@ NF_EXPORT int32_t nfh_groupby_features_i32(const int32_t* keys, size_t n, float*
   NFH_GUARD_END(int32_t)
 }
 
-// Mean cross-entropy gradient of the selector at params[4355] over
-// (X[n][64], y[n]) by the training kernels (grad_out[4355]).
-NF_EXPORT int32_t nfh_selector_grad(const float* params, const float* X, const int32_t* y,
-                                    size_t n, float* grad_out, float* loss_out) {
-  if (n == 0 || !params || !X || !y || !grad_out) return NULL_OR_ABORT;
+static int32_t sel_grad(const float* params, const float* X, const int32_t* y, const float* T,
+                        size_t n, float* grad_out, float* loss_out) {
+  if (n == 0 || !params || !X || (!y && !T) || !grad_out) return NULL_OR_ABORT;
   if (n > size_t(INT32_MAX)) return MALFORMED;
   NFH_GUARD_BEGIN
   Stage st;
   const float* P = st.in(params, S_P, S_IN0);
   const float* dx = st.in(X, n * SF, S_IN1);
-  const int32_t* dy = st.in(y, n, S_IN2);
+  const int32_t* dy = y ? st.in(y, n, S_IN2) : nullptr;
+  const float* dt = T ? st.in(T, n * SC, S_IN3) : nullptr;
   float* g = st.out(grad_out, S_P, S_OUT0);
   float* l = loss_out ? st.out(loss_out, 1, S_OUT1) : nullptr;
   const uint32_t nb = std::min<uint32_t>(SG_GRID, uint32_t((n + SG_BLOCK - 1) / SG_BLOCK));
   float* part = scratch_t<float>(S_WORK0, size_t(nb) * (S_P + 1));
-  hipLaunchKernelGGL(k_sel_grad, dim3(nb), dim3(SG_BLOCK), 0, st.s, P, dx, dy, uint32_t(n),
+  hipLaunchKernelGGL(k_sel_grad, dim3(nb), dim3(SG_BLOCK), 0, st.s, P, dx, dy, dt, uint32_t(n),
                      1.f / float(n), part);
   NFH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sel_adam, dim3(1), dim3(256), 0, st.s, nullptr, nullptr, nullptr, part, nb,
@@ -303,17 +324,15 @@ NF_EXPORT int32_t nfh_selector_grad(const float* params, const float* X, const i
   NFH_GUARD_END(int32_t)
 }
 
-// Train the selector in place: `steps` full-batch Adam steps (learning
-// rate lr) from params_inout[4355] on (X[n][64], y[n] in 0..2).
-// *loss_out: the loss before the last step.
-NF_EXPORT int32_t nfh_selector_train(const float* X, const int32_t* y, size_t n, uint32_t steps,
-                                     float lr, float* params_inout, float* loss_out) {
-  if (n == 0 || !params_inout || !X || !y) return NULL_OR_ABORT;
+static int32_t sel_train(const float* X, const int32_t* y, const float* T, size_t n,
+                         uint32_t steps, float lr, float* params_inout, float* loss_out) {
+  if (n == 0 || !params_inout || !X || (!y && !T)) return NULL_OR_ABORT;
   if (n > size_t(INT32_MAX)) return MALFORMED;
   NFH_GUARD_BEGIN
   Stage st;
   const float* dx = st.in(X, n * SF, S_IN1);
-  const int32_t* dy = st.in(y, n, S_IN2);
+  const int32_t* dy = y ? st.in(y, n, S_IN2) : nullptr;
+  const float* dt = T ? st.in(T, n * SC, S_IN3) : nullptr;
   float* P = st.out(params_inout, S_P, S_OUT0, /*copy_in=*/true);
   float* mm = scratch_t<float>(S_WORK1, 2 * S_P + 8);
   fill_u32(mm, 0, 2 * S_P, st.s);
@@ -324,8 +343,8 @@ NF_EXPORT int32_t nfh_selector_train(const float* X, const int32_t* y, size_t n,
   for (uint32_t it = 0; it < steps; ++it) {
     b1 *= 0.9;
     b2 *= 0.999;
-    hipLaunchKernelGGL(k_sel_grad, dim3(nb), dim3(SG_BLOCK), 0, st.s, P, dx, dy, uint32_t(n),
-                       1.f / float(n), part);
+    hipLaunchKernelGGL(k_sel_grad, dim3(nb), dim3(SG_BLOCK), 0, st.s, P, dx, dy, dt,
+                       uint32_t(n), 1.f / float(n), part);
     NFH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sel_adam, dim3(1), dim3(256), 0, st.s, P, mm, mm + S_P, part, nb, lr,
                        float(1.0 - b1), float(1.0 - b2), nullptr, l);
@@ -336,9 +355,8 @@ NF_EXPORT int32_t nfh_selector_train(const float* X, const int32_t* y, size_t n,
   NFH_GUARD_END(int32_t)
 }
 
-// Predicted class (0..2) per sample of X[n][64].
-NF_EXPORT int32_t nfh_selector_predict(const float* params, const float* X, size_t n,
-                                       int32_t* cls_out) {
+static int32_t sel_predict(const float* params, const float* X, size_t n, bool lowest,
+                           int32_t* cls_out) {
   if (n == 0) return OK;
   if (!params || !X || !cls_out) return NULL_OR_ABORT;
   if (n > size_t(INT32_MAX)) return MALFORMED;
@@ -348,9 +366,50 @@ NF_EXPORT int32_t nfh_selector_predict(const float* params, const float* X, size
   const float* dx = st.in(X, n * SF, S_IN1);
   int32_t* c = st.out(cls_out, n, S_OUT0);
   hipLaunchKernelGGL(k_sel_predict, dim3((n + 255) / 256), dim3(256), 0, st.s, P, dx,
-                     uint32_t(n), c);
+                     uint32_t(n), lowest, c);
   NFH_LAUNCH_CHECK();
   st.finish();
   return OK;
   NFH_GUARD_END(int32_t)
+}
+
+// Mean cross-entropy gradient of the selector at params[4420] over
+// (X[n][64], y[n]) by the training kernels (grad_out[4420]).
+NF_EXPORT int32_t nfh_selector_grad(const float* params, const float* X, const int32_t* y,
+                                    size_t n, float* grad_out, float* loss_out) {
+  return sel_grad(params, X, y, nullptr, n, grad_out, loss_out);
+}
+
+// Train the selector in place: `steps` full-batch Adam steps (learning
+// rate lr) from params_inout[4420] on (X[n][64], y[n] in 0..3).
+// *loss_out: the loss before the last step.
+NF_EXPORT int32_t nfh_selector_train(const float* X, const int32_t* y, size_t n, uint32_t steps,
+                                     float lr, float* params_inout, float* loss_out) {
+  return sel_train(X, y, nullptr, n, steps, lr, params_inout, loss_out);
+}
+
+// Predicted class (0..3) per sample of X[n][64].
+NF_EXPORT int32_t nfh_selector_predict(const float* params, const float* X, size_t n,
+                                       int32_t* cls_out) {
+  return sel_predict(params, X, n, false, cls_out);
+}
+
+// Cost model: gradient of the mean 1/2 sum_c (z_c - T_c)^2 over
+// (X[n][64], T[n][4]), T = each strategy's centred log time.
+NF_EXPORT int32_t nfh_selector_grad_cost(const float* params, const float* X, const float* T,
+                                         size_t n, float* grad_out, float* loss_out) {
+  return sel_grad(params, X, nullptr, T, n, grad_out, loss_out);
+}
+
+// Cost model: train in place (as nfh_selector_train) on T[n][4].
+NF_EXPORT int32_t nfh_selector_train_cost(const float* X, const float* T, size_t n,
+                                          uint32_t steps, float lr, float* params_inout,
+                                          float* loss_out) {
+  return sel_train(X, nullptr, T, n, steps, lr, params_inout, loss_out);
+}
+
+// Cost model: the strategy class with the smallest predicted cost.
+NF_EXPORT int32_t nfh_selector_predict_cost(const float* params, const float* X, size_t n,
+                                            int32_t* cls_out) {
+  return sel_predict(params, X, n, true, cls_out);
 }

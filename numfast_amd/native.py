@@ -111,6 +111,9 @@ GROUPS = {
         ("nfh_selector_grad", [_vp, _vp, _vp, _sz, _vp, _vp], _i32),
         ("nfh_selector_train", [_vp, _vp, _sz, _u32, ctypes.c_float, _vp, _vp], _i32),
         ("nfh_selector_predict", [_vp, _vp, _sz, _vp], _i32),
+        ("nfh_selector_grad_cost", [_vp, _vp, _vp, _sz, _vp, _vp], _i32),
+        ("nfh_selector_train_cost", [_vp, _vp, _sz, _u32, ctypes.c_float, _vp, _vp], _i32),
+        ("nfh_selector_predict_cost", [_vp, _vp, _sz, _vp], _i32),
         ("nfh_set_groupby_strategy", [_i32], _i32),
         ("nfh_groupby_last_strategy", [], _i32),
     ],
@@ -792,7 +795,8 @@ class GroupBatch:
 
 
 # ------------------------------------------------------------ strategy selector
-SEL_P = 64 * 64 + 64 + 3 * 64 + 3
+SEL_CLASSES = 4  # dense, partitioned, sort-based, hash-scatter (strategy = class + 1)
+SEL_P = 64 * 64 + 64 + SEL_CLASSES * 64 + SEL_CLASSES
 
 
 def groupby_features(keys):
@@ -807,11 +811,12 @@ def groupby_features(keys):
 
 
 def selector_init(seed=0):
-    """He-style random start of the 64 -> 64 -> 3 classifier."""
+    """He-style random start of the 64 -> 64 -> 4 classifier."""
     rng = np.random.default_rng(seed)
     p = np.zeros(SEL_P, np.float32)
     p[:64 * 64] = rng.normal(0, np.sqrt(2 / 64), 64 * 64)
-    p[64 * 64 + 64:64 * 64 + 64 + 192] = rng.normal(0, np.sqrt(1 / 64), 192)
+    w2 = SEL_CLASSES * 64
+    p[64 * 64 + 64:64 * 64 + 64 + w2] = rng.normal(0, np.sqrt(1 / 64), w2)
     return p
 
 
@@ -842,4 +847,35 @@ def selector_predict(params, X):
                                               _p(out))
     if rc != 0:
         raise RuntimeError("nfh_selector_predict rc=%d" % rc)
+    return out
+
+
+def selector_grad_cost(params, X, T):
+    """Cost-model gradient and loss at params over (X, T[n][4])."""
+    g = np.zeros(SEL_P, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib("selector").nfh_selector_grad_cost(_p(_c(params, np.float32)), _p(_c(X, np.float32)),
+                                                _p(_c(T, np.float32)), len(T), _p(g), _p(loss))
+    if rc != 0:
+        raise RuntimeError("nfh_selector_grad_cost rc=%d" % rc)
+    return g, float(loss[0])
+
+
+def selector_train_cost(X, T, steps=300, lr=1e-2, params=None):
+    p = selector_init() if params is None else _c(params, np.float32).copy()
+    loss = np.zeros(1, np.float32)
+    rc = lib("selector").nfh_selector_train_cost(_p(_c(X, np.float32)), _p(_c(T, np.float32)),
+                                                 len(T), steps, lr, _p(p), _p(loss))
+    if rc != 0:
+        raise RuntimeError("nfh_selector_train_cost rc=%d" % rc)
+    return p, float(loss[0])
+
+
+def selector_predict_cost(params, X):
+    X = _c(X, np.float32)
+    out = np.zeros(X.shape[0], np.int32)
+    rc = lib("selector").nfh_selector_predict_cost(_p(_c(params, np.float32)), _p(X), X.shape[0],
+                                                   _p(out))
+    if rc != 0:
+        raise RuntimeError("nfh_selector_predict_cost rc=%d" % rc)
     return out

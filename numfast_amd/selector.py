@@ -1,16 +1,21 @@
 """Learned group-by strategy selector: data set, training, evaluation.
 
 Key columns are generated on the GPU from a seeded mixture -- uniform keys,
-Zipf(s) with s in [0.5, 2], clustered runs (run length 1..4096) and
-presorted keys -- with 2^lo .. 2^hi rows (log-uniform) and a log-uniform
-key cardinality.  Each column is labelled by timing the three plain
-group-by strategies (nfh_group_agg_i32 forced to dense / partitioned /
-sort-based; a strategy that cannot run on the column counts at the time
-of the one that runs instead).  The selector (64 features -> 64 -> 3 MLP)
+Zipf(s) with s in [0.5, 2], clustered runs (run length 1..4096), presorted
+keys and keys spread over the whole int32 range -- with 2^lo .. 2^hi rows
+(log-uniform) and a log-uniform key cardinality.  Each column is labelled
+by timing the four plain group-by strategies (nfh_group_agg_i32 forced to
+dense / partitioned / sort-based / hash-scatter; a strategy that cannot
+run on the column counts at the time of the one that runs instead, and a
+hash table that overflows counts its wasted pass plus the sort-based
+path).  The selector (64 features -> 64 -> 4 MLP)
 is trained on the device on 80% of the columns; on the held-out 20% the
 report compares the total time of the selected strategies with the best
 fixed plan (one strategy for every column, chosen on the training split),
-with the built-in rule and with the per-column best.
+with the built-in rule and with the per-column best.  Two models share the
+network: a cost model (the selector: it predicts each strategy's centred
+log time and picks the smallest) and a classifier of the fastest strategy
+(reported beside it).
 
     python -m numfast_amd.selector --columns 10000 --out report.json
 """
@@ -22,12 +27,18 @@ import time
 
 import numpy as np
 
-KINDS = ("uniform", "zipf", "runs", "presorted")
+KINDS = ("uniform", "zipf", "runs", "presorted", "spread")
+STRATEGIES = ("dense", "partitioned", "sort", "hash")
 
 
 def gen_column(torch, g, n, kind, card, s=1.0, device="cuda"):
     """One int32 key column of n rows over [0, card)."""
     u = lambda m: torch.rand(m, generator=g, device=device, dtype=torch.float64)
+    if kind == "spread":
+        # card distinct ids anywhere in int32, uniform over them
+        ids = torch.randint(-(1 << 31), (1 << 31) - 1, (card,), generator=g, device=device,
+                            dtype=torch.int64)
+        return ids[torch.randint(0, card, (n,), generator=g, device=device)].to(torch.int32)
     if kind in ("uniform", "presorted"):
         k = torch.randint(0, card, (n,), generator=g, device=device, dtype=torch.int64)
         if kind == "presorted":
@@ -69,8 +80,8 @@ def label_columns(torch, native, columns, lo=20, hi=26, seed=0, log=None):
     outs = [torch.empty(nmax, dtype=dt, device="cuda") for dt in
             (torch.int64, torch.int64, torch.int64, torch.int32, torch.int32, torch.float64)]
     X = np.zeros((columns, 64), np.float32)
-    T = np.zeros((columns, 3), np.float64)
-    ran = np.zeros((columns, 3), np.int32)
+    T = np.zeros((columns, len(STRATEGIES)), np.float64)
+    ran = np.zeros((columns, len(STRATEGIES)), np.int32)
     meta = []
     t_start = time.time()
     for c in range(columns):
@@ -80,7 +91,7 @@ def label_columns(torch, native, columns, lo=20, hi=26, seed=0, log=None):
         s = float(rng.uniform(0.5, 2.0))
         k = gen_column(torch, g, n, kind, card, s)
         X[c] = native.groupby_features(k)
-        for st in (1, 2, 3):
+        for st in range(1, len(STRATEGIES) + 1):
             S.nfh_set_groupby_strategy(st)
             args = (k.data_ptr(), vals.data_ptr(), None, None, n,
                     *[o.data_ptr() for o in outs], nmax)
@@ -98,9 +109,10 @@ def label_columns(torch, native, columns, lo=20, hi=26, seed=0, log=None):
         del k
         if log and (c + 1) % 500 == 0:
             log("labelled %d / %d columns (%.0f s)" % (c + 1, columns, time.time() - t_start))
-    # a strategy that could not run costs the time of the one that ran
+    # a strategy that could not run costs the time of the one that ran (the
+    # hash-scatter fallback after an overflow keeps its own, longer time)
     for c in range(columns):
-        for st in range(3):
+        for st in range(2):
             if ran[c, st] != st + 1:
                 T[c, st] = T[c, ran[c, st] - 1]
     return X, T, ran, meta
@@ -113,32 +125,45 @@ def rule_choice(ran):
 
 
 def evaluate(native, X, T, ran, train_frac=0.8, steps=400, lr=1e-2, seed=0):
+    """Train on train_frac of the columns, report on the rest.  The
+    selector is the cost model (per strategy, the centred log time; the
+    pick is the smallest prediction); the classifier trained on the
+    fastest strategy's label is reported beside it."""
     n = X.shape[0]
     order = np.random.default_rng(seed + 1).permutation(n)
     ntr = int(train_frac * n)
     tr, te = order[:ntr], order[ntr:]
     y = np.argmin(T, axis=1).astype(np.int32)
+    lt = np.log(T)
+    cost = (lt - lt.mean(1, keepdims=True)).astype(np.float32)
     t0 = time.perf_counter()
-    params, loss = native.selector_train(X[tr], y[tr], steps=steps, lr=lr)
+    params, loss = native.selector_train_cost(X[tr], cost[tr], steps=steps, lr=lr)
     train_ms = (time.perf_counter() - t0) * 1e3
-    pred = native.selector_predict(params, X[te])
+    pred = native.selector_predict_cost(params, X[te])
+    cparams, closs = native.selector_train(X[tr], y[tr], steps=steps, lr=lr)
+    cpred = native.selector_predict(cparams, X[te])
     fixed = int(np.argmin(T[tr].sum(0)))
     rule = rule_choice(ran)
     t_sel = float(T[te, pred].sum())
+    t_cls = float(T[te, cpred].sum())
     t_fixed = float(T[te, fixed].sum())
     t_rule = float(T[te, rule[te]].sum())
     t_best = float(T[te].min(1).sum())
     return {
         "columns": n, "train": int(ntr), "held_out": int(n - ntr),
         "train_steps": steps, "train_loss": loss, "train_ms": round(train_ms, 1),
-        "label_share": [round(float(np.mean(y == c)), 3) for c in range(3)],
+        "label_share": [round(float(np.mean(y == c)), 3) for c in range(len(STRATEGIES))],
         "held_out_accuracy": round(float(np.mean(pred == y[te])), 4),
-        "fixed_plan": ["dense", "partitioned", "sort"][fixed],
+        "fixed_plan": STRATEGIES[fixed],
         "held_out_ms": {"selected": round(t_sel, 2), "fixed_plan": round(t_fixed, 2),
-                        "rule": round(t_rule, 2), "per_column_best": round(t_best, 2)},
+                        "rule": round(t_rule, 2), "per_column_best": round(t_best, 2),
+                        "classifier": round(t_cls, 2)},
         "speedup_vs_fixed_plan": round(t_fixed / t_sel, 4),
         "speedup_vs_rule": round(t_rule / t_sel, 4),
         "selected_over_best": round(t_sel / t_best, 4),
+        "classifier": {"train_loss": closs,
+                       "held_out_accuracy": round(float(np.mean(cpred == y[te])), 4),
+                       "speedup_vs_fixed_plan": round(t_fixed / t_cls, 4)},
     }, params
 
 

@@ -6,14 +6,14 @@ then log2(n)/32, log2(span)/32, distinct fraction, largest key share, share
 of rows whose key occurs >= 4 times, ascending-neighbour fraction,
 log2(distinct)/16, 1).
 
-forward / loss / grad: the 64 -> 64 (ReLU) -> 3 softmax classifier, float64,
-parameter layout W1[64][64] (row = hidden unit), b1[64], W2[3][64], b2[3];
+forward / loss / grad: the 64 -> 64 (ReLU) -> 4 softmax classifier, float64,
+parameter layout W1[64][64] (row = hidden unit), b1[64], W2[4][64], b2[4];
 pinned by finite differences in tests/test_oracle_golden.py.
 """
 
 import numpy as np
 
-F, H, C = 64, 64, 3
+F, H, C = 64, 64, 4
 W1, B1 = 0, F * H
 W2 = B1 + H
 B2 = W2 + C * H
@@ -92,4 +92,35 @@ def finite_diff(p, X, y, idx, eps=1e-6):
     a = loss(p, X, y)
     p[idx] -= 2 * eps
     b = loss(p, X, y)
+    return (a - b) / (2 * eps)
+
+
+def loss_cost(p, X, T):
+    """Cost model: mean over samples of 1/2 sum_c (z_c - T_c)^2."""
+    d = forward(p, X) - np.asarray(T, np.float64)
+    return float(0.5 * (d * d).sum() / d.shape[0])
+
+
+def grad_cost(p, X, T):
+    w1, b1, w2, b2 = unpack(p)
+    X = np.asarray(X, np.float64)
+    N = X.shape[0]
+    a = X @ w1.T + b1
+    h = np.maximum(a, 0.0)
+    dz = (h @ w2.T + b2 - np.asarray(T, np.float64)) / N
+    g = np.zeros(P)
+    g[W2:B2] = (dz.T @ h).reshape(-1)
+    g[B2:P] = dz.sum(0)
+    dh = (dz @ w2) * (a > 0)
+    g[W1:B1] = (dh.T @ X).reshape(-1)
+    g[B1:W2] = dh.sum(0)
+    return g
+
+
+def finite_diff_cost(p, X, T, idx, eps=1e-6):
+    p = np.asarray(p, np.float64).copy()
+    p[idx] += eps
+    a = loss_cost(p, X, T)
+    p[idx] -= 2 * eps
+    b = loss_cost(p, X, T)
     return (a - b) / (2 * eps)

@@ -19,28 +19,59 @@ def _check(got, ref):
     assert got["means"].tobytes() == ref["means"].tobytes()
 
 
-@pytest.mark.parametrize("kind", ["uniform", "zipf", "runs", "wide"])
+@pytest.mark.parametrize("kind", ["uniform", "zipf", "runs", "wide", "spread", "spread_zipf",
+                                  "one", "big_values"])
 def test_every_strategy_gives_the_same_groups(hip, kind):
     S = native.lib("selector")
     rng = np.random.default_rng(3)
     n = 1 << 18
+    vals = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
     if kind == "uniform":
         keys = rng.integers(0, 50000, n).astype(np.int32)
     elif kind == "zipf":
         keys = (rng.zipf(1.3, n) % 100000).astype(np.int32)
     elif kind == "runs":
         keys = np.repeat(rng.integers(-5000, 5000, n // 100 + 1), 100)[:n].astype(np.int32)
-    else:
+    elif kind == "wide":
         keys = rng.integers(-(2 ** 31), 2 ** 31 - 1, n, dtype=np.int64).astype(np.int32)
-    vals = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
+    elif kind == "spread":  # 3000 distinct keys over the whole int32 range
+        ids = rng.integers(-(2 ** 31), 2 ** 31 - 1, 3000, dtype=np.int64).astype(np.int32)
+        keys = ids[rng.integers(0, 3000, n)]
+    elif kind == "spread_zipf":  # hot keys in LDS, the tail in the hash tables
+        ids = rng.integers(-(2 ** 31), 2 ** 31 - 1, 200000, dtype=np.int64).astype(np.int32)
+        keys = ids[(rng.zipf(1.2, n) - 1) % 200000]
+    elif kind == "one":
+        keys = np.full(n, -7, np.int32)
+    else:  # values outside +-2^15 and int32 extremes
+        keys = rng.integers(0, 2000, n).astype(np.int32) * 1000003
+        vals = rng.integers(-(2 ** 31), 2 ** 31 - 1, n, dtype=np.int64).astype(np.int32)
     valid = rng.random(n) > 0.05
     ref = ox.group_agg(keys, vals, valid)
     try:
-        for st in (1, 2, 3):
+        for st in (1, 2, 3, 4):
             S.nfh_set_groupby_strategy(st)
             _check(native.group_agg(keys, vals, valid), ref)
             ran = S.nfh_groupby_last_strategy()
-            assert ran == st or (kind == "wide" and st in (1, 2) and ran == 3)
+            # a choice that cannot run on the column falls back to the rule
+            assert ran == st or (st in (1, 2) and ran in (1, 2, 3))
+    finally:
+        S.nfh_set_groupby_strategy(0)
+
+
+def test_hash_strategy_overflow_falls_back(hip):
+    """Distinct keys beyond the hash table's room: the overflow is detected
+    and the sort-based path gives the result."""
+    S = native.lib("selector")
+    rng = np.random.default_rng(8)
+    n = (1 << 22) + 3
+    keys = rng.permutation(np.arange(n, dtype=np.int64) * 977 - (1 << 31)).astype(np.int32)
+    vals = rng.integers(-100, 100, n).astype(np.int32)
+    valid = rng.random(n) > 0.01
+    ref = ox.group_agg(keys, vals, valid)
+    try:
+        S.nfh_set_groupby_strategy(4)
+        _check(native.group_agg(keys, vals, valid), ref)
+        assert S.nfh_groupby_last_strategy() in (3, 4)
     finally:
         S.nfh_set_groupby_strategy(0)
 
@@ -62,7 +93,7 @@ def test_selector_gradient_matches_oracle(hip, N):
     rng = np.random.default_rng(N)
     p = rng.normal(0, 0.3, osel.P).astype(np.float32)
     X = rng.random((N, 64)).astype(np.float32)
-    y = rng.integers(0, 3, N).astype(np.int32)
+    y = rng.integers(0, osel.C, N).astype(np.int32)
     g, loss = native.selector_grad(p, X, y)
     ref = osel.grad(p, X, y)
     scale = max(1e-12, np.abs(ref).max())
@@ -70,11 +101,38 @@ def test_selector_gradient_matches_oracle(hip, N):
     assert abs(loss - osel.loss(p, X, y)) <= 1e-4 * max(1.0, osel.loss(p, X, y))
 
 
+@pytest.mark.parametrize("N", [1, 700, 5000])
+def test_selector_cost_gradient_matches_oracle(hip, N):
+    rng = np.random.default_rng(N + 11)
+    p = rng.normal(0, 0.3, osel.P).astype(np.float32)
+    X = rng.random((N, 64)).astype(np.float32)
+    T = rng.normal(0, 1, (N, osel.C)).astype(np.float32)
+    g, loss = native.selector_grad_cost(p, X, T)
+    ref = osel.grad_cost(p, X, T)
+    scale = max(1e-12, np.abs(ref).max())
+    assert np.max(np.abs(g - ref)) <= 2e-4 * scale
+    assert abs(loss - osel.loss_cost(p, X, T)) <= 1e-4 * max(1.0, osel.loss_cost(p, X, T))
+
+
+def test_selector_cost_model_learns_costs():
+    """Costs from a rule on two features; the trained cost model picks the
+    cheapest strategy on held-out samples."""
+    rng = np.random.default_rng(6)
+    X = rng.random((4000, 64)).astype(np.float32)
+    T = np.stack([X[:, 0], 1 - X[:, 0], 0.5 + 0 * X[:, 1], X[:, 1] + 0.3], 1).astype(np.float32)
+    p, _ = native.selector_train_cost(X[:3000], T[:3000], steps=600, lr=1e-2)
+    pick = native.selector_predict_cost(p, X[3000:])
+    regret = T[3000:][np.arange(1000), pick] - T[3000:].min(1)
+    print("mean regret %.4f" % regret.mean())
+    assert regret.mean() < 0.02
+
+
 def test_selector_training_learns_a_rule():
     """Labels from a rule on two features; device training fits it."""
     rng = np.random.default_rng(5)
     X = rng.random((4000, 64)).astype(np.float32)
-    y = np.where(X[:, 0] > 0.6, 2, np.where(X[:, 1] > 0.5, 1, 0)).astype(np.int32)
+    y = np.where(X[:, 0] > 0.6, 2, np.where(X[:, 1] > 0.5, 1, np.where(X[:, 2] > 0.8, 3, 0)))
+    y = y.astype(np.int32)
     p0 = native.selector_init(1)
     _, l0 = native.selector_grad(p0, X, y)
     p, _ = native.selector_train(X[:3000], y[:3000], steps=600, lr=1e-2, params=p0)

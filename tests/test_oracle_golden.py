@@ -405,11 +405,29 @@ def test_selector_oracle_gradient_vs_finite_differences():
     rng = np.random.default_rng(8)
     p = rng.normal(0, 0.3, sel.P)
     X = rng.random((200, 64))
-    y = rng.integers(0, 3, 200)
+    y = rng.integers(0, sel.C, 200)
     g = sel.grad(p, X, y)
-    for idx in list(range(0, sel.P, 97)) + [sel.B2, sel.B2 + 2, sel.W2 + 5, sel.B1 + 7]:
+    for idx in list(range(0, sel.P, 97)) + [sel.B2, sel.B2 + 3, sel.W2 + 5, sel.B1 + 7]:
         fd = sel.finite_diff(p, X, y, idx)
         assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), idx
+
+
+def test_selector_cost_model_gradient_vs_finite_differences():
+    from oracle import selector as sel
+    rng = np.random.default_rng(9)
+    p = rng.normal(0, 0.3, sel.P)
+    X = rng.random((150, 64))
+    T = rng.normal(0, 1, (150, sel.C))
+    g = sel.grad_cost(p, X, T)
+    for idx in list(range(0, sel.P, 89)) + [sel.B2, sel.B2 + 3, sel.W2 + 7, sel.B1 + 2]:
+        fd = sel.finite_diff_cost(p, X, T, idx)
+        assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), idx
+    # closed form: with zero weights the outputs are the biases, and the
+    # bias gradient is the mean residual
+    q = np.zeros(sel.P)
+    q[sel.B2:sel.P] = [0.5, -1.0, 2.0, 0.0]
+    gq = sel.grad_cost(q, X, T)
+    assert np.allclose(gq[sel.B2:sel.P], (q[sel.B2:sel.P] - T).mean(0))
 
 
 def test_selector_features_restatement():
