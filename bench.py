@@ -36,8 +36,11 @@ def parse():
     ap.add_argument("--rows-log2", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="only the group-by + unique step")
-    ap.add_argument("--legs", default="step,argsort,learned,batch,selector,uniform,ghash",
-                    help="comma list of legs to run (for profiling one workload)")
+    # the selector leg (400 labelled columns, every strategy forced on each)
+    # runs only when named: --legs ...,selector
+    ap.add_argument("--legs", default="step,argsort,learned,batch,uniform,ghash",
+                    help="comma list of legs to run (for profiling one workload); "
+                         "add 'selector' for the 400-column strategy-selector leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scaling)")
