@@ -227,17 +227,18 @@ int64_t nfh_filter_derive_sort_limit_i32(const int32_t *v, size_t n, int32_t mul
 
 /* Group-by strategy for this thread's following nfh_group_agg_i32 calls:
  * 0 the built-in rule, 1 dense (LDS hot keys + global spill), 2 partitioned
- * (bucket passes), 3 sort-based, 4 hash-scatter (LDS hash tables + one
- * global hash table; sort-based when that table overflows); an impossible
- * choice falls back to the rule.  Returns the previous setting.  Results
- * are identical for all. */
+ * (bucket passes, 2048-key buckets), 3 sort-based, 4 hash-scatter (LDS hash
+ * tables + one global hash table; sort-based when that table overflows),
+ * 5 partitioned with 4096-key buckets (half the partition count); an
+ * impossible choice falls back to the rule.  Returns the previous setting.
+ * Results are identical for all. */
 int32_t nfh_set_groupby_strategy(int32_t strategy);
 /* The strategy the last group-by on this thread ran. */
 int32_t nfh_groupby_last_strategy(void);
 
 /* Learned strategy selector: 64 features of a key column (56-bin sampled
- * histogram + 8 summaries), and a 64 -> 64 -> 4 fp32 classifier
- * (params[4420] = W1[64][64] b1[64] W2[4][64] b2[4]) trained on the device
+ * histogram + 8 summaries), and a 64 -> 64 -> 5 fp32 classifier
+ * (params[4485] = W1[64][64] b1[64] W2[5][64] b2[5]) trained on the device
  * (full-batch Adam, softmax cross-entropy); class c = strategy c + 1. */
 int32_t nfh_groupby_features_i32(const int32_t *keys, size_t n, float *out64);
 int32_t nfh_selector_grad(const float *params, const float *X, const int32_t *y, size_t n,
@@ -246,7 +247,7 @@ int32_t nfh_selector_train(const float *X, const int32_t *y, size_t n, uint32_t 
                            float *params_inout, float *loss_out);
 int32_t nfh_selector_predict(const float *params, const float *X, size_t n, int32_t *cls_out);
 /* The same network as a cost model: z_c predicts strategy c's centred log
- * time (T[n][4]), trained on the mean of 1/2 sum_c (z_c - T_c)^2; the pick
+ * time (T[n][5]), trained on the mean of 1/2 sum_c (z_c - T_c)^2; the pick
  * is the smallest predicted cost. */
 int32_t nfh_selector_grad_cost(const float *params, const float *X, const float *T, size_t n,
                                float *grad_out, float *loss_out);

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n, int32_t kmin, uint32_t nb,
     const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc,
-    uint32_t* __restrict__ chist, uint32_t ptiles, bool vec) {
+    uint32_t* __restrict__ chist, uint32_t ptiles, bool vec, int bbits) {
   extern __shared__ uint32_t s_h[];  // nb counters
   __shared__ uint32_t s_c0[NCHAIN][128];  // digit-0 counts per chain of the first pass
   __shared__ int32_t t_key[HOT_TBL], t_idx[HOT_TBL];
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
       atomicMin(&h_mn[hi], x);
       atomicMax(&h_mx[hi], x);
     } else {
-      const uint32_t b = uint32_t(k - kmin) >> GP_BUCKET_BITS;
+      const uint32_t b = uint32_t(k - kmin) >> bbits;
       atomicAdd(&s_h[b], 1u);
       atomicAdd(&s_c0[ch][b & 127u], 1u);
     }
@@ -375,18 +375,27 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
 }
 
 // ------------------------------------------------------------ bucket aggregation
-// One workgroup per 2048-key bucket: LDS aggregation of its records, then
-// plain stores of all of the bucket's slots (empty ones included).
-__global__ __launch_bounds__(GP_BLOCK) void k_gp_bucket(const uint32_t* __restrict__ o,
-                                                        const int32_t* __restrict__ v,
-                                                        const uint32_t* __restrict__ bstart,
-                                                        uint64_t span, GpSlot* __restrict__ slots) {
-  constexpr int W = 1 << GP_BUCKET_BITS;
-  __shared__ u64 a_sum[W];
-  __shared__ uint32_t a_cnt[W];
-  __shared__ int32_t a_mn[W], a_mx[W];
+// One workgroup per 2^BITS-key bucket: LDS aggregation of its records, then
+// plain stores of all of the bucket's slots (empty ones included).  LDS:
+// 20 B per key of the bucket (dynamic: 40 KB for 2048 keys, 80 KB for 4096).
+template <int BITS>
+constexpr size_t gp_bucket_lds() {
+  return size_t(1 << BITS) * (sizeof(u64) + 3 * sizeof(uint32_t));
+}
+
+template <int BITS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_gp_bucket(const uint32_t* __restrict__ o,
+                                                     const int32_t* __restrict__ v,
+                                                     const uint32_t* __restrict__ bstart,
+                                                     uint64_t span, GpSlot* __restrict__ slots) {
+  constexpr int W = 1 << BITS;
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  u64* a_sum = reinterpret_cast<u64*>(s_raw);
+  uint32_t* a_cnt = reinterpret_cast<uint32_t*>(a_sum + W);
+  int32_t* a_mn = reinterpret_cast<int32_t*>(a_cnt + W);
+  int32_t* a_mx = a_mn + W;
   const uint32_t b = blockIdx.x;
-  for (int i = threadIdx.x; i < W; i += GP_BLOCK) {
+  for (int i = threadIdx.x; i < W; i += BLOCK) {
     a_sum[i] = 0;
     a_cnt[i] = 0;
     a_mn[i] = INT32_MAX;
@@ -397,12 +406,12 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_bucket(const uint32_t* __restri
   // RB records per thread per step, all loaded before any is added (every
   // lane loads an in-range record), so their latency overlaps
   constexpr int RB = 8;
-  for (uint32_t i0 = lo; i0 < hi; i0 += RB * GP_BLOCK) {
+  for (uint32_t i0 = lo; i0 < hi; i0 += RB * BLOCK) {
     uint32_t os[RB];
     int32_t xs[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-      const uint32_t i = i0 + uint32_t(r * GP_BLOCK) + threadIdx.x;
+      const uint32_t i = i0 + uint32_t(r * BLOCK) + threadIdx.x;
       const uint32_t ic = i < hi ? i : hi - 1u;
       os[r] = o[ic];
       xs[r] = v[ic];
@@ -410,7 +419,7 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_bucket(const uint32_t* __restri
     asm volatile("" ::: "memory");  // keep every load above the adds
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-      if (i0 + uint32_t(r * GP_BLOCK) + threadIdx.x >= hi) continue;
+      if (i0 + uint32_t(r * BLOCK) + threadIdx.x >= hi) continue;
       const uint32_t s = os[r] & (W - 1);
       const int32_t x = xs[r];
       atomicAdd(&a_sum[s], (u64)(long long)x);
@@ -420,8 +429,8 @@ __global__ __launch_bounds__(GP_BLOCK) void k_gp_bucket(const uint32_t* __restri
     }
   }
   __syncthreads();
-  const uint64_t first = uint64_t(b) << GP_BUCKET_BITS;
-  for (int i = threadIdx.x; i < W; i += GP_BLOCK) {
+  const uint64_t first = uint64_t(b) << BITS;
+  for (int i = threadIdx.x; i < W; i += BLOCK) {
     const uint64_t g = first + uint64_t(i);
     if (g >= span) break;
     GpSlot sl;
@@ -462,13 +471,17 @@ __global__ void k_gp_hotinit(HotAcc* __restrict__ h, uint32_t m) {
   }
 }
 
-bool gp_supported(uint64_t span) { return span <= (uint64_t(1) << (GP_BUCKET_BITS + 13)); }
+bool gp_supported(uint64_t span, int bucket_bits) {
+  return (bucket_bits == GP_BUCKET_BITS || bucket_bits == GP_BUCKET_BITS_WIDE) &&
+         span <= (uint64_t(1) << (bucket_bits + 13));
+}
 
 void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, size_t n_,
                   int32_t kmin, uint64_t span, const int2* table, const int32_t* hot_keys,
-                  const uint32_t* hot_n, GpSlot* slots, hipStream_t s) {
+                  const uint32_t* hot_n, GpSlot* slots, hipStream_t s, int bucket_bits) {
   const uint32_t n = uint32_t(n_);
-  const uint32_t nb = uint32_t((span + (1u << GP_BUCKET_BITS) - 1) >> GP_BUCKET_BITS);
+  const int bb = bucket_bits == GP_BUCKET_BITS_WIDE ? GP_BUCKET_BITS_WIDE : GP_BUCKET_BITS;
+  const uint32_t nb = uint32_t((span + (1u << bb) - 1) >> bb);
   const int passes = nb <= 1 ? 0 : (nb <= 128 ? 1 : 2);
   constexpr size_t CH = size_t(NCHAIN) * 128;
   // scratch: hist[nb] | bstart[nb+1] | per-chain digit counts [2][CH] |
@@ -495,7 +508,7 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
                      (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
     hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_HBLOCK, 2 * num_cus())),
                        dim3(GP_HBLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
-                       chist, tiles, vec);
+                       chist, tiles, vec, bb);
     NFH_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart);
@@ -526,6 +539,9 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
     for (const void* f : fns)
       NFH_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(GP_PART_LDS)));
+    NFH_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_gp_bucket<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(gp_bucket_lds<GP_BUCKET_BITS_WIDE>())));
   });
   const bool lr = lds_rank_supported();
   const bool two = passes == 2 && m > 0;
@@ -542,7 +558,7 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
     const auto fn = two ? (lr ? &k_gp_part<true, true, true> : &k_gp_part<true, false, true>)
                         : (lr ? &k_gp_part<true, true, false> : &k_gp_part<true, false, false>);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n, kmin,
-                       nullptr, nullptr, 0u, o0, v0, GP_BUCKET_BITS, dstart, lb, ctr, epoch, table,
+                       nullptr, nullptr, 0u, o0, v0, bb, dstart, lb, ctr, epoch, table,
                        t2, chist + CH);
     NFH_LAUNCH_CHECK();
   }
@@ -556,7 +572,7 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
       const uint32_t epoch = lookback_epoch(s);
       const auto fn = lr ? &k_gp_part<false, true, false> : &k_gp_part<false, false, false>;
       hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
-                         nullptr, 0u, kmin, o0, v0, m, o1, v1, GP_BUCKET_BITS + 7, dstart + CH,
+                         nullptr, 0u, kmin, o0, v0, m, o1, v1, bb + 7, dstart + CH,
                          lb, ctr + NCHAIN, epoch, nullptr, 0u, nullptr);
       NFH_LAUNCH_CHECK();
     }
@@ -565,7 +581,13 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   }
   {
     ProfScope prof("gp_bucket", s, double(m) * 8.0 + double(span) * sizeof(GpSlot));
-    hipLaunchKernelGGL(k_gp_bucket, dim3(nb), dim3(GP_BLOCK), 0, s, fo, fv, bstart, span, slots);
+    if (bb == GP_BUCKET_BITS_WIDE)
+      hipLaunchKernelGGL((k_gp_bucket<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK>), dim3(nb),
+                         dim3(2 * GP_BLOCK), gp_bucket_lds<GP_BUCKET_BITS_WIDE>(), s, fo, fv,
+                         bstart, span, slots);
+    else
+      hipLaunchKernelGGL((k_gp_bucket<GP_BUCKET_BITS, GP_BLOCK>), dim3(nb), dim3(GP_BLOCK),
+                         gp_bucket_lds<GP_BUCKET_BITS>(), s, fo, fv, bstart, span, slots);
     NFH_LAUNCH_CHECK();
   }
   if (table) {

@@ -498,18 +498,19 @@ static bool build_hot_table(const int32_t* keys, size_t n, int2** table, int32_t
 
 // Strategy of the next group-by on this thread (0: the rule below) and the
 // one the last group-by used: 1 dense (LDS hot keys + global spill), 2
-// partitioned (bucket passes, no per-row global atomics), 3 sort-based
-// (radix unique + aggregation by code), 4 hash-scatter (LDS hash tables +
-// one global hash table).  A forced strategy that cannot run on the input
-// (span too large, key mask for the partitioned path) falls back to the
-// rule; a hash table that overflows falls back to the sort-based path.
+// partitioned (bucket passes, no per-row global atomics; 2048-key buckets),
+// 3 sort-based (radix unique + aggregation by code), 4 hash-scatter (LDS
+// hash tables + one global hash table), 5 partitioned with 4096-key buckets
+// (half the partition count).  A forced strategy that cannot run on the
+// input (span too large, key mask for the partitioned path) falls back to
+// the rule; a hash table that overflows falls back to the sort-based path.
 namespace {
 thread_local int t_strategy = 0;
 thread_local int t_last_strategy = 0;
 }  // namespace
 int set_groupby_strategy(int s) {
   const int old = t_strategy;
-  t_strategy = (s >= 0 && s <= 4) ? s : 0;
+  t_strategy = (s >= 0 && s <= 5) ? s : 0;
   return old;
 }
 int last_groupby_strategy() { return t_last_strategy; }
@@ -529,18 +530,21 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
   const u64 span = u64(int64_t(kmax) - int64_t(kmin) + 1);
   const bool dense_ok = span <= dense_span_limit(n);
   const bool part_ok = dense_ok && gp_supported(span) && kv == nullptr;
+  const bool wide_ok = dense_ok && gp_supported(span, GP_BUCKET_BITS_WIDE) && kv == nullptr;
   int strat = t_strategy;
-  if ((strat == 1 && !dense_ok) || (strat == 2 && !part_ok)) strat = 0;
+  if ((strat == 1 && !dense_ok) || (strat == 2 && !part_ok) || (strat == 5 && !wide_ok))
+    strat = 0;
   if (strat == 0) strat = part_ok ? 2 : dense_ok ? 1 : 3;
   t_last_strategy = strat;
-  if (strat == 2) {
+  if (strat == 2 || strat == 5) {
     // Partitioned path: no per-row global atomics.
     GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
     int2* table = nullptr;
     int32_t* hot_keys = nullptr;
     uint32_t* hot_n = nullptr;
     if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
-    gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, slots, s);
+    gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, slots, s,
+                 strat == 5 ? GP_BUCKET_BITS_WIDE : GP_BUCKET_BITS);
     return compact_slots(slots, span, nullptr, kmin, o, s);
   }
   if (strat == 1) {
@@ -924,7 +928,8 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
 using namespace nfh;
 
 // Strategy override for this thread's following group-bys (0 = rule,
-// 1 dense, 2 partitioned, 3 sort-based); returns the previous setting.
+// 1 dense, 2 partitioned, 3 sort-based, 4 hash-scatter, 5 partitioned with
+// 4096-key buckets); returns the previous setting.
 NF_EXPORT int32_t nfh_set_groupby_strategy(int32_t strategy) {
   return set_groupby_strategy(strategy);
 }

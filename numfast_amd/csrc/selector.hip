@@ -7,15 +7,16 @@
 // at least 4 times, fraction of ascending neighbour pairs, log2(distinct) /
 // 16, and a constant 1.
 //
-// Classifier: a 64 -> 64 (ReLU) -> 4 fp32 MLP with softmax cross-entropy,
+// Classifier: a 64 -> 64 (ReLU) -> 5 fp32 MLP with softmax cross-entropy,
 // trained on the device: k_sel_grad (forward + backward, one sample per
 // thread, per-workgroup partial gradients in a fixed order) and k_sel_adam
 // (reduction + Adam).  Class c is group-by strategy c + 1 (dense,
-// partitioned, sort-based, hash-scatter; groupagg.hip).  A wrong pick costs time only:
-// every strategy returns the same result.
+// partitioned, sort-based, hash-scatter, partitioned with half the
+// partitions; groupagg.hip).  A wrong pick costs time only: every strategy
+// returns the same result.
 //
 // Parameter layout (floats): W1[64][64] (row = hidden unit), b1[64],
-// W2[4][64], b2[4].
+// W2[5][64], b2[5].
 #include <cmath>
 
 #include "device.hpp"
@@ -25,7 +26,7 @@ namespace nfh {
 
 constexpr int SF = 64;  // features
 constexpr int SH = 64;  // hidden
-constexpr int SC = 4;   // classes
+constexpr int SC = 5;   // classes
 constexpr int S_W1 = 0, S_B1 = SF * SH, S_W2 = S_B1 + SH, S_B2 = S_W2 + SC * SH,
               S_P = S_B2 + SC;
 constexpr int SEL_SAMPLE = 4096;

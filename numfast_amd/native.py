@@ -795,7 +795,9 @@ class GroupBatch:
 
 
 # ------------------------------------------------------------ strategy selector
-SEL_CLASSES = 4  # dense, partitioned, sort-based, hash-scatter (strategy = class + 1)
+# dense, partitioned, sort-based, hash-scatter, partitioned with 4096-key
+# buckets (strategy = class + 1)
+SEL_CLASSES = 5
 SEL_P = 64 * 64 + 64 + SEL_CLASSES * 64 + SEL_CLASSES
 
 
@@ -811,7 +813,7 @@ def groupby_features(keys):
 
 
 def selector_init(seed=0):
-    """He-style random start of the 64 -> 64 -> 4 classifier."""
+    """He-style random start of the 64 -> 64 -> 5 classifier."""
     rng = np.random.default_rng(seed)
     p = np.zeros(SEL_P, np.float32)
     p[:64 * 64] = rng.normal(0, np.sqrt(2 / 64), 64 * 64)

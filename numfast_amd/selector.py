@@ -4,11 +4,13 @@ Key columns are generated on the GPU from a seeded mixture -- uniform keys,
 Zipf(s) with s in [0.5, 2], clustered runs (run length 1..4096), presorted
 keys and keys spread over the whole int32 range -- with 2^lo .. 2^hi rows
 (log-uniform) and a log-uniform key cardinality.  Each column is labelled
-by timing the four plain group-by strategies (nfh_group_agg_i32 forced to
-dense / partitioned / sort-based / hash-scatter; a strategy that cannot
-run on the column counts at the time of the one that runs instead, and a
-hash table that overflows counts its wasted pass plus the sort-based
-path).  The selector (64 features -> 64 -> 4 MLP)
+by timing the five plain group-by strategies (nfh_group_agg_i32 forced to
+dense / partitioned with 2048-key buckets / sort-based / hash-scatter /
+partitioned with 4096-key buckets, i.e. the path and, for the partitioned
+path, its partition count; a strategy that cannot run on the column counts
+at the time of the one that runs instead, and a hash table that overflows
+counts its wasted pass plus the sort-based path).  The selector (64
+features -> 64 -> 5 MLP)
 is trained on the device on 80% of the columns; on the held-out 20% the
 report compares the total time of the selected strategies with the best
 fixed plan (one strategy for every column, chosen on the training split),
@@ -28,7 +30,7 @@ import time
 import numpy as np
 
 KINDS = ("uniform", "zipf", "runs", "presorted", "spread")
-STRATEGIES = ("dense", "partitioned", "sort", "hash")
+STRATEGIES = ("dense", "partitioned", "sort", "hash", "partitioned_4096")
 
 
 def gen_column(torch, g, n, kind, card, s=1.0, device="cuda"):
@@ -112,7 +114,7 @@ def label_columns(torch, native, columns, lo=20, hi=26, seed=0, log=None):
     # a strategy that could not run costs the time of the one that ran (the
     # hash-scatter fallback after an overflow keeps its own, longer time)
     for c in range(columns):
-        for st in range(2):
+        for st in (0, 1, 4):
             if ran[c, st] != st + 1:
                 T[c, st] = T[c, ran[c, st] - 1]
     return X, T, ran, meta

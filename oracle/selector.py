@@ -6,14 +6,14 @@ then log2(n)/32, log2(span)/32, distinct fraction, largest key share, share
 of rows whose key occurs >= 4 times, ascending-neighbour fraction,
 log2(distinct)/16, 1).
 
-forward / loss / grad: the 64 -> 64 (ReLU) -> 4 softmax classifier, float64,
-parameter layout W1[64][64] (row = hidden unit), b1[64], W2[4][64], b2[4];
+forward / loss / grad: the 64 -> 64 (ReLU) -> 5 softmax classifier, float64,
+parameter layout W1[64][64] (row = hidden unit), b1[64], W2[5][64], b2[5];
 pinned by finite differences in tests/test_oracle_golden.py.
 """
 
 import numpy as np
 
-F, H, C = 64, 64, 4
+F, H, C = 64, 64, 5
 W1, B1 = 0, F * H
 W2 = B1 + H
 B2 = W2 + C * H

@@ -48,12 +48,40 @@ def test_every_strategy_gives_the_same_groups(hip, kind):
     valid = rng.random(n) > 0.05
     ref = ox.group_agg(keys, vals, valid)
     try:
-        for st in (1, 2, 3, 4):
+        for st in (1, 2, 3, 4, 5):
             S.nfh_set_groupby_strategy(st)
             _check(native.group_agg(keys, vals, valid), ref)
             ran = S.nfh_groupby_last_strategy()
             # a choice that cannot run on the column falls back to the rule
-            assert ran == st or (st in (1, 2) and ran in (1, 2, 3))
+            assert ran == st or (st in (1, 2, 5) and ran in (1, 2, 3))
+    finally:
+        S.nfh_set_groupby_strategy(0)
+
+
+@pytest.mark.parametrize("span", [1, 4096, 4097, 1 << 19, (1 << 19) + 4097, (1 << 24) + 5,
+                                  1 << 25])
+def test_partition_counts_agree(hip, span):
+    """The partitioned path with 2048- and 4096-key buckets (one or two
+    partition passes, a last partial bucket, spans only the wider buckets
+    take) against the oracle."""
+    S = native.lib("selector")
+    rng = np.random.default_rng(span % 1000)
+    n = 1 << 20
+    base = -(1 << 30) + 17
+    keys = (base + rng.integers(0, span, n)).astype(np.int32)
+    keys[:3] = [base, base + span - 1, base]  # both ends of the span present
+    vals = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
+    vals[::97] = rng.integers(-(2 ** 31), 2 ** 31 - 1, vals[::97].size, dtype=np.int64)
+    valid = rng.random(n) > 0.05
+    valid[:3] = True
+    ref = ox.group_agg(keys, vals, valid)
+    try:
+        for st in (2, 5):
+            S.nfh_set_groupby_strategy(st)
+            _check(native.group_agg(keys, vals, valid), ref)
+            ran = S.nfh_groupby_last_strategy()
+            if st == 5 or span <= (1 << 24):
+                assert ran == st
     finally:
         S.nfh_set_groupby_strategy(0)
 
@@ -119,7 +147,8 @@ def test_selector_cost_model_learns_costs():
     cheapest strategy on held-out samples."""
     rng = np.random.default_rng(6)
     X = rng.random((4000, 64)).astype(np.float32)
-    T = np.stack([X[:, 0], 1 - X[:, 0], 0.5 + 0 * X[:, 1], X[:, 1] + 0.3], 1).astype(np.float32)
+    T = np.stack([X[:, 0], 1 - X[:, 0], 0.5 + 0 * X[:, 1], X[:, 1] + 0.3, 0.8 - X[:, 2]],
+                 1).astype(np.float32)
     p, _ = native.selector_train_cost(X[:3000], T[:3000], steps=600, lr=1e-2)
     pick = native.selector_predict_cost(p, X[3000:])
     regret = T[3000:][np.arange(1000), pick] - T[3000:].min(1)

@@ -425,7 +425,7 @@ def test_selector_cost_model_gradient_vs_finite_differences():
     # closed form: with zero weights the outputs are the biases, and the
     # bias gradient is the mean residual
     q = np.zeros(sel.P)
-    q[sel.B2:sel.P] = [0.5, -1.0, 2.0, 0.0]
+    q[sel.B2:sel.P] = [0.5, -1.0, 2.0, 0.0, -0.25]
     gq = sel.grad_cost(q, X, T)
     assert np.allclose(gq[sel.B2:sel.P], (q[sel.B2:sel.P] - T).mean(0))
 
