@@ -117,13 +117,52 @@ __global__ __launch_bounds__(256) void k_key_range(const int32_t* __restrict__ k
                                                    const uint8_t* __restrict__ kv, uint32_t n,
                                                    int* __restrict__ out /* min, max */) {
   int mn = INT32_MAX, mx = INT32_MIN;
-  // key-only pass: eight quads in flight per thread
-  for_each_row<VEC, 8>(keys, nullptr, v, kv, n, [&](int32_t k, int32_t, bool ok) {
+  // key-only pass: eight quads in flight per thread, streaming loads
+  // (0.33 -> 0.31 ms at 2^28 rows)
+  auto f = [&](int32_t k, bool ok) {
     if (ok) {
       mn = k < mn ? k : mn;
       mx = k > mx ? k : mx;
     }
-  });
+  };
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t tail = 0;
+  if constexpr (VEC) {
+    constexpr int R = 8;
+    const uint32_t nq = n / 4;
+    uint32_t q = tid;
+    for (; q + (R - 1) * stride < nq; q += R * stride) {
+      int4 k4[R];
+      uint32_t a[R], b[R];
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        const uint32_t qq = q + uint32_t(u) * stride;
+        k4[u] = ld_stream4(keys, qq);
+        a[u] = v ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(v) + qq) : 0x01010101u;
+        b[u] = kv ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(kv) + qq) : 0x01010101u;
+      }
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        f(k4[u].x, (a[u] & 0xFFu) && (b[u] & 0xFFu));
+        f(k4[u].y, (a[u] & 0xFF00u) && (b[u] & 0xFF00u));
+        f(k4[u].z, (a[u] & 0xFF0000u) && (b[u] & 0xFF0000u));
+        f(k4[u].w, (a[u] & 0xFF000000u) && (b[u] & 0xFF000000u));
+      }
+    }
+    for (; q < nq; q += stride) {
+      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
+      const uint32_t a = v ? reinterpret_cast<const uint32_t*>(v)[q] : 0x01010101u;
+      const uint32_t b = kv ? reinterpret_cast<const uint32_t*>(kv)[q] : 0x01010101u;
+      f(k4.x, (a & 0xFFu) && (b & 0xFFu));
+      f(k4.y, (a & 0xFF00u) && (b & 0xFF00u));
+      f(k4.z, (a & 0xFF0000u) && (b & 0xFF0000u));
+      f(k4.w, (a & 0xFF000000u) && (b & 0xFF000000u));
+    }
+    tail = nq * 4;
+  }
+  for (uint32_t i = tail + tid; i < n; i += stride)
+    f(keys[i], (v == nullptr || v[i] != 0) && (kv == nullptr || kv[i] != 0));
   for (int off = 32; off > 0; off >>= 1) {
     const int a = __shfl_down(mn, off), b = __shfl_down(mx, off);
     mn = a < mn ? a : mn;
@@ -334,9 +373,12 @@ struct SlotCompactOp {
 
 // Compaction of the present slots in key order with coalesced output: each
 // wave takes 64 consecutive slots per step (striped), ballots the present
-// ones and writes them to consecutive positions; tiles of 4096 slots chain
-// their counts with a decoupled look-back.
-constexpr int SC2_BLOCK = 256, SC2_IPT = 16, SC2_TILE = SC2_BLOCK * SC2_IPT;
+// ones and writes them to consecutive positions; tiles of 2048 slots chain
+// their counts with a decoupled look-back.  Each slot is read once, whole,
+// and kept in registers for the write (reading the counts first and the
+// present slots again fetched the slot array about twice: 0.33 -> 0.26 ms
+// at 2^24 slots).
+constexpr int SC2_BLOCK = 256, SC2_IPT = 8, SC2_TILE = SC2_BLOCK * SC2_IPT;
 
 __global__ __launch_bounds__(SC2_BLOCK) void k_slot_compact(
     const GroupSlot* __restrict__ slots, uint64_t m, const int32_t* __restrict__ ukey_of,
@@ -351,11 +393,16 @@ __global__ __launch_bounds__(SC2_BLOCK) void k_slot_compact(
   const uint64_t wbase = uint64_t(tile) * SC2_TILE + uint64_t(wave) * (WAVE * SC2_IPT);
   uint64_t bal[SC2_IPT];
   uint32_t wtot = 0;
+  GroupSlot sv[SC2_IPT];
 #pragma unroll
   for (int j = 0; j < SC2_IPT; ++j) {
     const uint64_t s = wbase + uint64_t(j * WAVE + lane);
-    // every lane loads (past the end: the last slot), so the loads overlap
-    const bool present = slots[s < m ? s : m - 1].cnt != 0 && s < m;
+    sv[j] = slots[s < m ? s : m - 1];
+  }
+#pragma unroll
+  for (int j = 0; j < SC2_IPT; ++j) {
+    const uint64_t s = wbase + uint64_t(j * WAVE + lane);
+    const bool present = sv[j].cnt != 0 && s < m;
     bal[j] = __ballot(present);
     wtot += uint32_t(__popcll(bal[j]));
   }
@@ -386,7 +433,7 @@ __global__ __launch_bounds__(SC2_BLOCK) void k_slot_compact(
     const uint64_t s = wbase + uint64_t(j * WAVE + lane);
     if ((bal[j] >> lane) & 1ull) {
       const uint32_t ex = pos + uint32_t(__popcll(bal[j] & lanemask_lt()));
-      const GroupSlot sl = slots[s];
+      const GroupSlot sl = sv[j];
       const int64_t sum = (int64_t)sl.sum;
       if (o.ukeys) o.ukeys[ex] = ukey_of ? int64_t(ukey_of[s]) : int64_t(kmin) + int64_t(s);
       if (o.counts) o.counts[ex] = int64_t(sl.cnt);
