@@ -60,7 +60,8 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n, int32_t kmin, uint32_t nb,
     const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc,
-    uint32_t* __restrict__ chist, uint32_t ptiles, bool vec, int bbits) {
+    uint32_t* __restrict__ chist, uint32_t ptiles, bool vec, int bbits,
+    uint32_t* __restrict__ mode) {
   extern __shared__ uint32_t s_h[];  // nb counters
   __shared__ uint32_t s_c0[NCHAIN][128];  // digit-0 counts per chain of the first pass
   __shared__ int32_t t_key[HOT_TBL], t_idx[HOT_TBL];
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     h_mx[b] = INT32_MIN;
   }
   __syncthreads();
+  bool oor = false;  // a record value outside [-2^15, 2^15) (mode bit 0)
   // chain of row i's tile in the first partition pass
   auto chain_of_row = [&](uint32_t i) { return chain_of_tile(ptiles, i / uint32_t(GP_TILE)); };
   auto row = [&](int32_t k, int32_t x, bool ok, uint32_t ch) {
@@ -98,6 +100,7 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
       atomicMax(&h_mx[hi], x);
     } else {
       const uint32_t b = uint32_t(k - kmin) >> bbits;
+      oor |= x < -32768 || x >= 32768;
       atomicAdd(&s_h[b], 1u);
       atomicAdd(&s_c0[ch][b & 127u], 1u);
     }
@@ -145,6 +148,7 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   }
   for (uint32_t i = tail + tid; i < n; i += stride)
     row(keys[i], vals[i], !vm || vm[i], chain_of_row(i));
+  if (__ballot(oor) != 0 && lane_id() == 0) atomicOr(mode, 1u);
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
@@ -169,8 +173,10 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
 }
 
 // Bucket starts: exclusive scan of hist (bstart[nb] = number of records).
+// mode |= 2 when a bucket holds 2^24 records or more (no packed counts).
 __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ hist, uint32_t nb,
-                                                  uint32_t* __restrict__ bstart) {
+                                                  uint32_t* __restrict__ bstart,
+                                                  uint32_t* __restrict__ mode) {
   __shared__ uint32_t s_tmp[16];
   __shared__ uint32_t s_carry;
   if (threadIdx.x == 0) s_carry = 0;
@@ -178,6 +184,7 @@ __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ h
   for (uint32_t base = 0; base < nb; base += blockDim.x) {
     const uint32_t b = base + threadIdx.x;
     const uint32_t c = b < nb ? hist[b] : 0u;
+    if (c >= (1u << 24)) atomicOr(mode, 2u);
     uint32_t tot;
     const uint32_t ex = block_excl_scan<16>(c, s_tmp, &tot);
     if (b < nb) bstart[b] = s_carry + ex;
@@ -376,28 +383,33 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
 
 // ------------------------------------------------------------ bucket aggregation
 // One workgroup per 2^BITS-key bucket: LDS aggregation of its records, then
-// plain stores of all of the bucket's slots (empty ones included).  LDS:
-// 20 B per key of the bucket (dynamic: 40 KB for 2048 keys, 80 KB for 4096).
-template <int BITS>
+// plain stores of all of the bucket's slots (empty ones included).
+// PACKED (every record value in [-2^15, 2^15) and every bucket under 2^24
+// records, known from the histogram pass): one 64-bit LDS add per record
+// carries the count (bits 40..63) and the biased sum (x + 2^15, below
+// 2^40), so a record costs three LDS atomics instead of four and the
+// bucket 16 B of LDS per key instead of 20.
+template <int BITS, bool PACKED>
 constexpr size_t gp_bucket_lds() {
-  return size_t(1 << BITS) * (sizeof(u64) + 3 * sizeof(uint32_t));
+  return size_t(1 << BITS) * (PACKED ? 16 : 20);
 }
 
-template <int BITS, int BLOCK>
+template <int BITS, int BLOCK, bool PACKED>
 __global__ __launch_bounds__(BLOCK) void k_gp_bucket(const uint32_t* __restrict__ o,
                                                      const int32_t* __restrict__ v,
                                                      const uint32_t* __restrict__ bstart,
                                                      uint64_t span, GpSlot* __restrict__ slots) {
   constexpr int W = 1 << BITS;
   extern __shared__ __align__(16) unsigned char s_raw[];
-  u64* a_sum = reinterpret_cast<u64*>(s_raw);
-  uint32_t* a_cnt = reinterpret_cast<uint32_t*>(a_sum + W);
-  int32_t* a_mn = reinterpret_cast<int32_t*>(a_cnt + W);
+  u64* a_sum = reinterpret_cast<u64*>(s_raw);  // PACKED: count << 40 | biased sum
+  uint32_t* a_cnt = reinterpret_cast<uint32_t*>(a_sum + W);  // unused when PACKED
+  int32_t* a_mn = PACKED ? reinterpret_cast<int32_t*>(a_sum + W)
+                         : reinterpret_cast<int32_t*>(a_cnt + W);
   int32_t* a_mx = a_mn + W;
   const uint32_t b = blockIdx.x;
   for (int i = threadIdx.x; i < W; i += BLOCK) {
     a_sum[i] = 0;
-    a_cnt[i] = 0;
+    if constexpr (!PACKED) a_cnt[i] = 0;
     a_mn[i] = INT32_MAX;
     a_mx[i] = INT32_MIN;
   }
@@ -422,8 +434,12 @@ __global__ __launch_bounds__(BLOCK) void k_gp_bucket(const uint32_t* __restrict_
       if (i0 + uint32_t(r * BLOCK) + threadIdx.x >= hi) continue;
       const uint32_t s = os[r] & (W - 1);
       const int32_t x = xs[r];
-      atomicAdd(&a_sum[s], (u64)(long long)x);
-      atomicAdd(&a_cnt[s], 1u);
+      if constexpr (PACKED) {
+        atomicAdd(&a_sum[s], (u64(1) << 40) | u64(uint32_t(x + 32768)));
+      } else {
+        atomicAdd(&a_sum[s], (u64)(long long)x);
+        atomicAdd(&a_cnt[s], 1u);
+      }
       atomicMin(&a_mn[s], x);
       atomicMax(&a_mx[s], x);
     }
@@ -434,8 +450,15 @@ __global__ __launch_bounds__(BLOCK) void k_gp_bucket(const uint32_t* __restrict_
     const uint64_t g = first + uint64_t(i);
     if (g >= span) break;
     GpSlot sl;
-    sl.sum = a_sum[i];
-    sl.cnt = a_cnt[i];
+    if constexpr (PACKED) {
+      const u64 pk = a_sum[i];
+      const uint32_t c = uint32_t(pk >> 40);
+      sl.sum = (pk & ((u64(1) << 40) - 1)) - u64(c) * 32768u;  // wrapping u64
+      sl.cnt = c;
+    } else {
+      sl.sum = a_sum[i];
+      sl.cnt = a_cnt[i];
+    }
     sl.mn = a_mn[i];
     sl.mx = a_mx[i];
     sl.pad[0] = sl.pad[1] = sl.pad[2] = 0;
@@ -484,13 +507,15 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   const uint32_t nb = uint32_t((span + (1u << bb) - 1) >> bb);
   const int passes = nb <= 1 ? 0 : (nb <= 128 ? 1 : 2);
   constexpr size_t CH = size_t(NCHAIN) * 128;
-  // scratch: hist[nb] | bstart[nb+1] | per-chain digit counts [2][CH] |
-  // per-chain digit starts [2][CH] | tile counters [2][NCHAIN] | hot_acc
-  const size_t meta_words = size_t(nb) * 2 + 1 + 4 * CH + 2 * NCHAIN;
+  // scratch: hist[nb] | bstart[nb+1] | mode | per-chain digit counts
+  // [2][CH] | per-chain digit starts [2][CH] | tile counters [2][NCHAIN] |
+  // hot_acc
+  const size_t meta_words = size_t(nb) * 2 + 2 + 4 * CH + 2 * NCHAIN;
   uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + 4 + HOT_MAX * 6);
   uint32_t* hist = meta;
   uint32_t* bstart = hist + nb;
-  uint32_t* chist = bstart + nb + 1;
+  uint32_t* mode = bstart + nb + 1;
+  uint32_t* chist = mode + 1;
   uint32_t* dstart = chist + 2 * CH;
   uint32_t* ctr = dstart + 2 * CH;
   HotAcc* hot_acc = reinterpret_cast<HotAcc*>(meta + ((meta_words + 3) & ~size_t(3)));
@@ -508,18 +533,19 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
                      (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
     hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_HBLOCK, 2 * num_cus())),
                        dim3(GP_HBLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
-                       chist, tiles, vec, bb);
+                       chist, tiles, vec, bb, mode);
     NFH_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart);
+  hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart, mode);
   NFH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_chain_bins<128>, dim3(1), dim3(128), 0, s, chist, dstart);
   NFH_LAUNCH_CHECK();
-  // records = valid non-hot rows (bstart[nb])
-  uint32_t* h_m = static_cast<uint32_t*>(pinned(4));
-  NFH_CHECK(hipMemcpyAsync(h_m, bstart + nb, 4, hipMemcpyDeviceToHost, s));
+  // records = valid non-hot rows (bstart[nb]), then the mode word
+  uint32_t* h_m = static_cast<uint32_t*>(pinned(8));
+  NFH_CHECK(hipMemcpyAsync(h_m, bstart + nb, 8, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipStreamSynchronize(s));
-  const uint32_t m = *h_m;
+  const uint32_t m = h_m[0];
+  const bool packed = h_m[1] == 0u;
   // record buffers (ping-pong): at most n records
   uint32_t* o0 = scratch_t<uint32_t>(S_T0, n);
   int32_t* v0 = scratch_t<int32_t>(S_T1, n);
@@ -540,8 +566,13 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
       NFH_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(GP_PART_LDS)));
     NFH_CHECK(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_gp_bucket<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, int(gp_bucket_lds<GP_BUCKET_BITS_WIDE>())));
+        reinterpret_cast<const void*>(&k_gp_bucket<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK, false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize,
+        int(gp_bucket_lds<GP_BUCKET_BITS_WIDE, false>())));
+    NFH_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_gp_bucket<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK, true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize,
+        int(gp_bucket_lds<GP_BUCKET_BITS_WIDE, true>())));
   });
   const bool lr = lds_rank_supported();
   const bool two = passes == 2 && m > 0;
@@ -581,13 +612,21 @@ void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, s
   }
   {
     ProfScope prof("gp_bucket", s, double(m) * 8.0 + double(span) * sizeof(GpSlot));
-    if (bb == GP_BUCKET_BITS_WIDE)
-      hipLaunchKernelGGL((k_gp_bucket<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK>), dim3(nb),
-                         dim3(2 * GP_BLOCK), gp_bucket_lds<GP_BUCKET_BITS_WIDE>(), s, fo, fv,
-                         bstart, span, slots);
-    else
-      hipLaunchKernelGGL((k_gp_bucket<GP_BUCKET_BITS, GP_BLOCK>), dim3(nb), dim3(GP_BLOCK),
-                         gp_bucket_lds<GP_BUCKET_BITS>(), s, fo, fv, bstart, span, slots);
+#define NFH_GP_BUCKET(BITS, BLOCK, PK)                                                  \
+  hipLaunchKernelGGL((k_gp_bucket<BITS, BLOCK, PK>), dim3(nb), dim3(BLOCK),               \
+                     (gp_bucket_lds<BITS, PK>()), s, fo, fv, bstart, span, slots)
+    if (bb == GP_BUCKET_BITS_WIDE) {
+      if (packed)
+        NFH_GP_BUCKET(GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK, true);
+      else
+        NFH_GP_BUCKET(GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK, false);
+    } else {
+      if (packed)
+        NFH_GP_BUCKET(GP_BUCKET_BITS, GP_BLOCK, true);
+      else
+        NFH_GP_BUCKET(GP_BUCKET_BITS, GP_BLOCK, false);
+    }
+#undef NFH_GP_BUCKET
     NFH_LAUNCH_CHECK();
   }
   if (table) {

@@ -86,6 +86,25 @@ def test_partition_counts_agree(hip, span):
         S.nfh_set_groupby_strategy(0)
 
 
+def test_partitioned_bucket_over_2p24_records(hip):
+    """One 2048-key bucket receives more than 2^24 records (2^25 rows over
+    2048 keys, a quarter of them hot): the bucket pass must not use its
+    packed 24-bit counts there."""
+    S = native.lib("selector")
+    rng = np.random.default_rng(12)
+    n = (1 << 25) + 5
+    keys = (1000 + rng.integers(0, 2048, n)).astype(np.int32)
+    vals = rng.integers(-100, 100, n).astype(np.int32)
+    valid = rng.random(n) > 0.01
+    ref = ox.group_agg(keys, vals, valid)
+    try:
+        S.nfh_set_groupby_strategy(2)
+        _check(native.group_agg(keys, vals, valid), ref)
+        assert S.nfh_groupby_last_strategy() == 2
+    finally:
+        S.nfh_set_groupby_strategy(0)
+
+
 def test_hash_strategy_overflow_falls_back(hip):
     """Distinct keys beyond the hash table's room: the overflow is detected
     and the sort-based path gives the result."""
