@@ -140,8 +140,8 @@ __global__ __launch_bounds__(HA_BLOCK) void k_hash_acc(
       if (idx >= 0) {
         atomicAdd(&a_sum[idx], (u64)(long long)x);
         atomicAdd(&a_cnt[idx], 1u);
-        atomicMin(&a_mn[idx], x);
-        atomicMax(&a_mx[idx], x);
+        if (x < a_mn[idx]) atomicMin(&a_mn[idx], x);
+        if (x > a_mx[idx]) atomicMax(&a_mx[idx], x);
         return;
       }
     }
@@ -153,8 +153,8 @@ __global__ __launch_bounds__(HA_BLOCK) void k_hash_acc(
       if (cur == 0 || cur == kw) {
         atomicAdd(&l_sum[h], (u64)(long long)x);
         atomicAdd(&l_cnt[h], 1u);
-        atomicMin(&l_mn[h], x);
-        atomicMax(&l_mx[h], x);
+        if (x < l_mn[h]) atomicMin(&l_mn[h], x);
+        if (x > l_mx[h]) atomicMax(&l_mx[h], x);
         return;
       }
       h = (h + 1) & uint32_t(HL_SLOTS - 1);

@@ -96,8 +96,9 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
         atomicAdd(&h_sum[hi], (u64)(long long)x);
         atomicAdd(&h_cnt[hi], 1u);
       }
-      atomicMin(&h_mn[hi], x);
-      atomicMax(&h_mx[hi], x);
+      // min / max only when x would change them (see k_gp_bucket)
+      if (x < h_mn[hi]) atomicMin(&h_mn[hi], x);
+      if (x > h_mx[hi]) atomicMax(&h_mx[hi], x);
     } else {
       const uint32_t b = uint32_t(k - kmin) >> bbits;
       oor |= x < -32768 || x >= 32768;
@@ -440,8 +441,12 @@ __global__ __launch_bounds__(BLOCK) void k_gp_bucket(const uint32_t* __restrict_
         atomicAdd(&a_sum[s], (u64)(long long)x);
         atomicAdd(&a_cnt[s], 1u);
       }
-      atomicMin(&a_mn[s], x);
-      atomicMax(&a_mx[s], x);
+      // min / max only when x would change them: they move one way, so a
+      // read that misses a concurrent update only adds a redundant atomic
+      // (0.55 -> 0.35 ms at 2^28 Zipf rows; the hot-key aggregates of the
+      // histogram pass, 0.65 -> 0.59 ms)
+      if (x < a_mn[s]) atomicMin(&a_mn[s], x);
+      if (x > a_mx[s]) atomicMax(&a_mx[s], x);
     }
   }
   __syncthreads();

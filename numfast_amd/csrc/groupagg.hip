@@ -321,8 +321,8 @@ __global__ __launch_bounds__(ACC_BLOCK) void k_group_acc(
       if (idx >= 0) {
         atomicAdd(&a_sum[idx], (u64)(long long)x);
         atomicAdd(&a_cnt[idx], 1u);
-        atomicMin(&a_mn[idx], x);
-        atomicMax(&a_mx[idx], x);
+        if (x < a_mn[idx]) atomicMin(&a_mn[idx], x);
+        if (x > a_mx[idx]) atomicMax(&a_mx[idx], x);
         return;
       }
     }
