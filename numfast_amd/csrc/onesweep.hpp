@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_chain_hist(const K* __restrict__ keys,
                                                     const word_t<K>* __restrict__ win, uint32_t n,
                                                     uint32_t T, uint32_t tile_items, DG dg,
                                                     bool desc, uint32_t* __restrict__ hist,
-                                                    word_t<K>* __restrict__ orand) {
+                                                    word_t<K>* __restrict__ orand, bool vec) {
   using W = word_t<K>;
   extern __shared__ __align__(16) unsigned char s_raw[];
   uint32_t* s_h = reinterpret_cast<uint32_t*>(s_raw);
@@ -184,6 +184,48 @@ __global__ __launch_bounds__(256) void k_chain_hist(const K* __restrict__ keys,
     }
     const uint32_t lo = t * tile_items;
     const uint32_t hi = lo + tile_items < n ? lo + tile_items : n;
+    if constexpr (FIRST && sizeof(K) == 4) {
+      if (vec) {
+        // 16-byte loads of four keys: full quads of the tile (tile_items is
+        // a multiple of 4, so lo / 4 is exact), then the last partial quad
+        // key by key (the learned code histogram 0.33 -> 0.30 ms at 2^28)
+        constexpr int BQ = 4;  // quads in flight per thread
+        const uint32_t q0 = lo / 4, qf = hi / 4;
+        for (uint32_t b = q0; b < qf; b += BQ * blockDim.x) {
+          int4 x[BQ];
+#pragma unroll
+          for (int u = 0; u < BQ; ++u) {
+            const uint32_t qi = b + uint32_t(u) * blockDim.x + threadIdx.x;
+            x[u] = qi < qf ? reinterpret_cast<const int4*>(keys)[qi] : make_int4(0, 0, 0, 0);
+          }
+#pragma unroll
+          for (int u = 0; u < BQ; ++u) {
+            const bool valid = b + uint32_t(u) * blockDim.x + threadIdx.x < qf;
+            const int32_t kk[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const W w = Key<K>::word(K(kk[c]), desc);
+              if (valid) {
+                o |= w;
+                a &= w;
+              }
+              hist_add<DG::PEEL>(s_h, uint32_t(dg.code(w, s_dg) >> dg.shift) & 255u, valid);
+            }
+          }
+        }
+        for (uint32_t b = qf * 4; b < hi; b += blockDim.x) {
+          const uint32_t i = b + threadIdx.x;
+          const bool valid = i < hi;
+          const W w = valid ? Key<K>::word(keys[i], desc) : W(0);
+          if (valid) {
+            o |= w;
+            a &= w;
+          }
+          hist_add<DG::PEEL>(s_h, uint32_t(dg.code(w, s_dg) >> dg.shift) & 255u, valid);
+        }
+        continue;
+      }
+    }
     constexpr int B = 16;  // loads in flight per thread
     for (uint32_t b = lo; b < hi; b += B * blockDim.x) {
       W w[B];
@@ -480,9 +522,10 @@ void chain_hist_first(const K* keys, uint32_t n, const DG& dg, bool desc, uint32
   const unsigned grid = std::max(1u, std::min<unsigned>(T, 4u * unsigned(num_cus())));
   const size_t lds = 256 * 4 + DG::EXTRA_LDS;
   ProfScope prof(prof_name, s, double(n) * sizeof(K));
+  const bool vec = sizeof(K) == 4 && (reinterpret_cast<uintptr_t>(keys) % 16) == 0;
   hipLaunchKernelGGL((k_chain_hist<K, true, DG>), dim3(grid), dim3(256), lds, s, keys,
                      static_cast<const word_t<K>*>(nullptr), n, T, uint32_t(G::TILE), dg, desc,
-                     hist, orand);
+                     hist, orand, vec);
   NFH_LAUNCH_CHECK();
 }
 
