@@ -106,6 +106,14 @@ __global__ __launch_bounds__(R) void k_chain_bins(const uint32_t* __restrict__ h
 // lane of the wave must call it.
 template <bool PEEL = false>
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) {
+  // the first and last lanes differ: no whole-wave run, skip the ballots
+  // (see rank_lds)
+  if constexpr (!PEEL) {
+    if (__builtin_amdgcn_readfirstlane(bin) != __builtin_amdgcn_readlane(bin, WAVE - 1)) {
+      if (valid) atomicAdd(&h[bin], 1u);
+      return;
+    }
+  }
   const uint64_t act = __ballot(valid);
   if (act == 0) return;
   const int first = __ffsll((unsigned long long)act) - 1;
