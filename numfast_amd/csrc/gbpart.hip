@@ -41,7 +41,7 @@ constexpr int GP_WAVES = GP_PBLOCK / WAVE;
 constexpr int GP_IPT = 16;
 constexpr int GP_TILE = GP_PBLOCK * GP_IPT;
 constexpr size_t GP_PART_LDS = size_t(GP_TILE) * 8 + GP_WAVES * 128 * 4 + 128 * 4 * 2 +
-                               GP_WAVES * 4 + 16 + size_t(NCHAIN) * 128 * 4;
+                               GP_WAVES * 4 + 16 + size_t(NCHAIN) * 128 * 4 + 128;
 static_assert(HOT_MAX == 512 && HOT_TBL == 1024, "LDS budget of the passes below");
 
 // ------------------------------------------------------------ histogram
@@ -224,6 +224,9 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
   uint32_t* s_tmp = s_gbase + 128;
   uint32_t* s_tile = s_tmp + GP_WAVES;
   uint32_t* s_nh = s_tile + 4;  // [NCHAIN][128]
+  // per digit: the next pass's chain of this tile's whole run, or 0xFF
+  // when the run crosses a chain boundary
+  unsigned char* s_cc = reinterpret_cast<unsigned char*>(s_nh + NCHAIN * 128);
   __shared__ int32_t t_key[FIRST ? HOT_TBL : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
@@ -356,6 +359,17 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
         lb_store(lookback + size_t(tile) * 128 + tid, lb_pack(epoch, LB_PREFIX, excl + cnt));
       }
       s_gbase[tid] = bins[tid] + excl - tstart;
+      if constexpr (NEXT) {
+        const uint32_t g0 = bins[tid] + excl;
+        const uint32_t g1 = cnt ? g0 + cnt - 1u : g0;
+        uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+        for (int k = 1; k < NCHAIN; ++k) {
+          c0 += g0 >= nb[k] ? 1u : 0u;
+          c1 += g1 >= nb[k] ? 1u : 0u;
+        }
+        s_cc[tid] = (unsigned char)(c0 == c1 ? c0 : 0xFFu);
+      }
     }
     __syncthreads();
     for (uint32_t i0 = 0; i0 < tile_total; i0 += GP_PBLOCK) {  // uniform trip count
@@ -368,9 +382,12 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
         vout[g] = s_v[i];
       }
       if constexpr (NEXT) {
-        uint32_t c = 0;
+        uint32_t c = s_cc[(o >> shift) & 127u];
+        if (c == 0xFFu) {
+          c = 0;
 #pragma unroll
-        for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+          for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+        }
         hist_add(s_nh, c * 128u + ((o >> (shift + 7)) & 127u), valid);
       }
     }

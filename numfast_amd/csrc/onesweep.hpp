@@ -138,7 +138,7 @@ struct SortGeom {
   // tile (word + row id), per-wave digit counters, digit starts and bases,
   // scan scratch and tile index, the next pass's per-chain counts
   static constexpr size_t LDS = size_t(TILE) * (sizeof(K) + 4) + size_t(WAVES) * 256 * 4 +
-                                256 * 4 * 2 + WAVES * 4 + 16 + size_t(NCHAIN) * 256 * 4;
+                                256 * 4 * 2 + WAVES * 4 + 16 + size_t(NCHAIN) * 256 * 4 + 256;
 };
 
 // The plain radix digit: byte `shift / 8` of the key word; `next_shift`
@@ -311,6 +311,9 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
   uint32_t* s_tmp = s_gbase + 256;
   uint32_t* s_tile = s_tmp + SORT_WAVES;
   uint32_t(*s_nh)[256] = reinterpret_cast<uint32_t(*)[256]>(s_tile + 4);
+  // per digit: the next-pass chain of this tile's whole run, or 0xFF when
+  // the run crosses a chain boundary
+  unsigned char* s_cc = reinterpret_cast<unsigned char*>(s_nh + NCHAIN);
   unsigned char* s_dg = s_raw + G::LDS;
 
   const int tid = threadIdx.x;
@@ -432,6 +435,19 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
         lb_store(my_lb, lb_pack(epoch, LB_PREFIX, excl + cnt));
       }
       s_gbase[tid] = bins[tid] + excl - tstart;
+      if constexpr (NEXT) {
+        // chain of the run's first and last output position (monotone in
+        // the position, so equal ends mean one chain for the whole run)
+        const uint32_t g0 = bins[tid] + excl;
+        const uint32_t g1 = cnt ? g0 + cnt - 1u : g0;
+        uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+        for (int k = 1; k < NCHAIN; ++k) {
+          c0 += g0 >= nb[k] ? 1u : 0u;
+          c1 += g1 >= nb[k] ? 1u : 0u;
+        }
+        s_cc[tid] = (unsigned char)(c0 == c1 ? c0 : 0xFFu);
+      }
     }
     __syncthreads();
 
@@ -453,9 +469,14 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
         }
       }
       if constexpr (NEXT) {
-        uint32_t c = 0;
+        // the run's chain from LDS; item by item only in the few runs that
+        // cross a chain boundary
+        uint32_t c = s_cc[d];
+        if (c == 0xFFu) {
+          c = 0;
 #pragma unroll
-        for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+          for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+        }
         hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + (uint32_t(code >> dg.next_shift) & 255u), valid);
       }
     }
