@@ -497,8 +497,8 @@ def main():
         # vs the same kernels launched query by query
         extra["batched_groupby_4096q"] = batch_leg(torch, W, native, n, max(3, args.steps // 2))
     if "selector" in legs:
-        # learned strategy selector on a reduced column set (the full
-        # 10,000-column run: python -m numfast_amd.selector)
+        # learned strategy selector on a reduced column set (larger runs:
+        # python -m numfast_amd.selector)
         from numfast_amd import selector as SEL
         X, T, ran, _ = SEL.label_columns(torch, native, 400, lo=18, hi=22, seed=7)
         extra["strategy_selector_400col"], _ = SEL.evaluate(native, X, T, ran, steps=400)
