@@ -183,3 +183,29 @@ def _unique_device_offset(keys, valid):
     torch.cuda.synchronize()
     assert ng >= 0
     return uniq[1:1 + ng].cpu().numpy(), inv[1:].cpu().numpy()
+
+
+@pytest.mark.parametrize("dist,masked,vec", [("uniform", False, True), ("uniform", True, True),
+                                              ("zipf", True, True), ("edges", True, True),
+                                              ("uniform", True, False), ("zipf", False, False)])
+def test_unique_valid_large_span(dist, masked, vec):
+    """Dense-span unique with flags larger than an XCD's L2 share (spans
+    over 2^21 ids at 2^22+ rows): rows not a multiple of 4, keys at both
+    ends of the span, hot keys, NULL rows, and the row-at-a-time path."""
+    rng = np.random.default_rng(21)
+    n = (1 << 22) + 7
+    if dist == "uniform":
+        keys = rng.integers(-(1 << 23), 1 << 23, n).astype(np.int32)
+    elif dist == "zipf":
+        keys = zipf_keys(rng, n, 1 << 24, 1.1) - 12345
+    else:  # span exactly 2^21 + 1, both ends present
+        keys = rng.integers(0, (1 << 21) + 1, n).astype(np.int32)
+        keys[:2] = [0, 1 << 21]
+        keys[-1] = 1 << 21
+    valid = rng.random(n) > 0.1 if masked else None
+    if valid is not None:
+        valid[:2] = True
+        valid[-1] = True
+    ru, ri = ox.unique_inverse(keys, valid)
+    u, inv = native.unique_valid(keys, valid) if vec else _unique_device_offset(keys, valid)
+    assert np.array_equal(u, ru) and np.array_equal(inv, ri)
