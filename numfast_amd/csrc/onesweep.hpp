@@ -452,32 +452,55 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     __syncthreads();
 
     const uint32_t nvalid = (n - base) < uint32_t(SORT_TILE) ? (n - base) : uint32_t(SORT_TILE);
-    for (uint32_t i0 = 0; i0 < nvalid; i0 += SORT_BLOCK) {  // uniform trip count
-      const uint32_t i = i0 + uint32_t(tid);
-      const bool valid = i < nvalid;
-      const W w = valid ? s_w[i] : W(0);
-      const auto code = dg.code(w, s_dg);
-      const uint32_t d = uint32_t(code >> dg.shift) & 255u;
-      const uint32_t g = s_gbase[d] + i;
-      if (valid) {
-        if constexpr (LAST) {
-          iout[g] = int32_t(s_i[i]);
-          if (kout != nullptr) kout[g] = Tk::key(w, desc);
-        } else {
-          wout[g] = w;
-          iout[g] = int32_t(s_i[i]);
-        }
-      }
-      if constexpr (NEXT) {
-        // the run's chain from LDS; item by item only in the few runs that
-        // cross a chain boundary
-        uint32_t c = s_cc[d];
-        if (c == 0xFFu) {
-          c = 0;
+    // WU items per thread per step, their LDS reads issued together so one
+    // wait covers them (radix passes 1.05 -> 1.02 ms at 2^28 with two; the
+    // learned passes, whose code map reads LDS knots, measured slower)
+    constexpr int WU = DG::PEEL ? 1 : 2;
+    for (uint32_t i0 = 0; i0 < nvalid; i0 += WU * SORT_BLOCK) {  // uniform trip count
+      W w[WU];
+      uint32_t xi[WU], gb[WU], cc[WU];
+      bool ok[WU];
 #pragma unroll
-          for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+      for (int u = 0; u < WU; ++u) {
+        const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
+        ok[u] = i < nvalid;
+        const uint32_t ic = ok[u] ? i : 0u;
+        w[u] = s_w[ic];
+        xi[u] = s_i[ic];
+      }
+#pragma unroll
+      for (int u = 0; u < WU; ++u) {
+        if (!ok[u]) w[u] = W(0);
+        const uint32_t d = uint32_t(dg.code(w[u], s_dg) >> dg.shift) & 255u;
+        gb[u] = s_gbase[d];
+        if constexpr (NEXT) cc[u] = s_cc[d];
+      }
+#pragma unroll
+      for (int u = 0; u < WU; ++u) {
+        const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
+        const uint32_t g = gb[u] + i;
+        if (ok[u]) {
+          if constexpr (LAST) {
+            iout[g] = int32_t(xi[u]);
+            if (kout != nullptr) kout[g] = Tk::key(w[u], desc);
+          } else {
+            wout[g] = w[u];
+            iout[g] = int32_t(xi[u]);
+          }
         }
-        hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + (uint32_t(code >> dg.next_shift) & 255u), valid);
+        if constexpr (NEXT) {
+          // the run's chain from LDS; item by item only in the few runs
+          // that cross a chain boundary
+          uint32_t c = cc[u];
+          if (c == 0xFFu) {
+            c = 0;
+#pragma unroll
+            for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+          }
+          const auto code = dg.code(w[u], s_dg);
+          hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + (uint32_t(code >> dg.next_shift) & 255u),
+                             ok[u]);
+        }
       }
     }
     __syncthreads();  // the tile buffers are reused by the next tile
