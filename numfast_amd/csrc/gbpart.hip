@@ -372,23 +372,45 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
       }
     }
     __syncthreads();
-    for (uint32_t i0 = 0; i0 < tile_total; i0 += GP_PBLOCK) {  // uniform trip count
-      const uint32_t i = i0 + uint32_t(tid);
-      const bool valid = i < tile_total;
-      const uint32_t o = valid ? s_o[i] : 0u;
-      const uint32_t g = s_gbase[(o >> shift) & 127u] + i;
-      if (valid) {
-        oout[g] = o;
-        vout[g] = s_v[i];
-      }
-      if constexpr (NEXT) {
-        uint32_t c = s_cc[(o >> shift) & 127u];
-        if (c == 0xFFu) {
-          c = 0;
+    // GWU records per thread per step, their LDS reads issued together
+    // (0.89 -> 0.88 ms for the first pass at 2^28 rows; four measured no
+    // better)
+    constexpr int GWU = 2;
+    for (uint32_t i0 = 0; i0 < tile_total; i0 += GWU * GP_PBLOCK) {  // uniform trip count
+      uint32_t ob[GWU], xv[GWU], gb[GWU], cc[GWU];
+      bool ok[GWU];
 #pragma unroll
-          for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+      for (int u = 0; u < GWU; ++u) {
+        const uint32_t i = i0 + uint32_t(u * GP_PBLOCK + tid);
+        ok[u] = i < tile_total;
+        const uint32_t ic = ok[u] ? i : 0u;
+        ob[u] = s_o[ic];
+        xv[u] = uint32_t(s_v[ic]);
+      }
+#pragma unroll
+      for (int u = 0; u < GWU; ++u) {
+        if (!ok[u]) ob[u] = 0u;
+        const uint32_t d = (ob[u] >> shift) & 127u;
+        gb[u] = s_gbase[d];
+        if constexpr (NEXT) cc[u] = s_cc[d];
+      }
+#pragma unroll
+      for (int u = 0; u < GWU; ++u) {
+        const uint32_t i = i0 + uint32_t(u * GP_PBLOCK + tid);
+        const uint32_t g = gb[u] + i;
+        if (ok[u]) {
+          oout[g] = ob[u];
+          vout[g] = int32_t(xv[u]);
         }
-        hist_add(s_nh, c * 128u + ((o >> (shift + 7)) & 127u), valid);
+        if constexpr (NEXT) {
+          uint32_t c = cc[u];
+          if (c == 0xFFu) {
+            c = 0;
+#pragma unroll
+            for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+          }
+          hist_add(s_nh, c * 128u + ((ob[u] >> (shift + 7)) & 127u), ok[u]);
+        }
       }
     }
     __syncthreads();  // the tile buffers are reused by the next tile
