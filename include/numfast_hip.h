@@ -179,6 +179,23 @@ int64_t nfh_group_agg_i32(const int32_t *keys, const int32_t *vals, const uint8_
                           int64_t *sums, int32_t *mins, int32_t *maxs, double *means,
                           size_t cap);
 
+/* The same group-by with a float64 value column: counts int64; sums,
+ * mins, maxs, means float64.  Sums are the np.bincount row-order sums of
+ * the reference within 1e-12 relative (exactly rounded, or bit-equal where
+ * redone in row order); min / max are np.minimum.at / np.maximum.at over a
+ * +-inf fill (cpu.py:2570-2601): the group's first NaN when it has one, a
+ * zero extreme with the sign of the group's last zero.  Returns ng or -7. */
+int64_t nfh_group_agg_f64(const int32_t *keys, const double *vals, const uint8_t *valid,
+                          const uint8_t *key_valid, size_t n, int64_t *ukeys, int64_t *counts,
+                          double *sums, double *mins, double *maxs, double *means, size_t cap);
+/* int64 value column: sums int64 (wrapping; exact where the executor's 2^53
+ * gate admits the column, cpu.py:424-446), mins / maxs int64 (iinfo fill),
+ * means float64.  Returns ng or -7. */
+int64_t nfh_group_agg_i64(const int32_t *keys, const int64_t *vals, const uint8_t *valid,
+                          const uint8_t *key_valid, size_t n, int64_t *ukeys, int64_t *counts,
+                          int64_t *sums, int64_t *mins, int64_t *maxs, double *means,
+                          size_t cap);
+
 /* np.unique over valid rows; inv = -1 on NULL rows
  * (src/Drivers/CPU/_lib/cpu.py:4032-4036).  Returns ng. */
 int64_t nfh_unique_valid_i32(const int32_t *keys, const uint8_t *valid, size_t n,
@@ -231,7 +248,8 @@ int64_t nfh_filter_derive_sort_limit_i32(const int32_t *v, size_t n, int32_t mul
  * tables + one global hash table; sort-based when that table overflows),
  * 5 partitioned with 4096-key buckets (half the partition count); an
  * impossible choice falls back to the rule.  Returns the previous setting.
- * Results are identical for all. */
+ * Results are identical for all.  With 4 set, nfh_group_agg_i32 refuses
+ * inputs of more than 2^22 rows with -3. */
 int32_t nfh_set_groupby_strategy(int32_t strategy);
 /* The strategy the last group-by on this thread ran. */
 int32_t nfh_groupby_last_strategy(void);

@@ -562,6 +562,14 @@ int set_groupby_strategy(int s) {
 }
 int last_groupby_strategy() { return t_last_strategy; }
 
+// Forced hash-scatter is refused above this many rows: a labelling run that
+// forced every strategy on columns of 2^20..2^26 rows ended in a GPU memory
+// fault whose kernel and column are unknown, and hash-scatter (with its
+// sort-based fallback) is the one strategy that only a forced call reaches.
+// The built-in rule never picks it, so unforced calls are not affected.
+constexpr size_t HASH_FORCE_MAX_ROWS = size_t(1) << 22;
+bool forced_strategy_refused(size_t n) { return t_strategy == 4 && n > HASH_FORCE_MAX_ROWS; }
+
 int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t* v,
                          const uint8_t* kv, size_t n, const GroupAggOut& o, hipStream_t s,
                          const int32_t* known_range) {
@@ -985,8 +993,9 @@ NF_EXPORT int32_t nfh_groupby_last_strategy(void) { return last_groupby_strategy
 
 // Group-by over raw int32 keys with NULL masks (valid / key_valid may be
 // null = all valid).  Outputs hold `cap` lanes; any output pointer may be
-// null.  Returns ng >= 0, -1 null input, -3 n > i32::MAX, -6 device error,
-// -7 cap < ng (outputs unwritten).
+// null.  Returns ng >= 0, -1 null input, -3 n > i32::MAX or hash-scatter
+// forced on more than 2^22 rows, -6 device error, -7 cap < ng (outputs
+// unwritten).
 NF_EXPORT int64_t nfh_group_agg_i32(const int32_t* keys, const int32_t* vals,
                                     const uint8_t* valid, const uint8_t* key_valid, size_t n,
                                     int64_t* ukeys, int64_t* counts, int64_t* sums,
@@ -994,6 +1003,7 @@ NF_EXPORT int64_t nfh_group_agg_i32(const int32_t* keys, const int32_t* vals,
   if (n == 0) return 0;
   if (!keys || !vals) return NULL_OR_ABORT;
   if (n > size_t(INT32_MAX)) return MALFORMED;
+  if (forced_strategy_refused(n)) return MALFORMED;
   NFH_GUARD_BEGIN
   Stage st;
   const int32_t* k = st.in(keys, n, S_IN0);

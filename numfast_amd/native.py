@@ -130,6 +130,8 @@ GROUPS = {
         ("nfh_device_ok", [], _i32),
         ("nfh_abi_version", [], _i32),
         ("nfh_group_agg_i32", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
+        ("nfh_group_agg_f64", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
+        ("nfh_group_agg_i64", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
         ("nfh_unique_valid_i32", [_vp, _vp, _sz, _vp, _vp], _i64),
         ("nfh_profile_enable", [_i32], _i32),
         ("nfh_profile_reset", [], _i32),
@@ -608,6 +610,31 @@ def group_agg(keys, vals, valid=None, key_valid=None):
                                  _p(out["maxs"]), _p(out["means"]), n))
     if ng < 0:
         raise RuntimeError("nfh_group_agg_i32 rc=%d" % ng)
+    return {k: v[:ng] for k, v in out.items()}
+
+
+def group_agg64(keys, vals, valid=None, key_valid=None):
+    """Group-by of a float64 or int64 value column with sum/count/min/max/
+    mean and NULL masks (nfh_group_agg_f64 / nfh_group_agg_i64): the
+    executor's min/max branch for 64-bit values (cpu.py:3507-3563,
+    :2570-2601).  Returns a ColumnCarry-shaped dict (carry.py:21-62):
+    ukeys / counts int64, sums / mins / maxs in the value dtype, means
+    float64."""
+    vals = np.asarray(vals)
+    dt = np.float64 if vals.dtype.kind == "f" else np.int64
+    keys, vals = _c(keys, np.int32), _c(vals, dt)
+    n = keys.size
+    L = lib("hip")
+    vm = None if valid is None else _c(valid, np.uint8)
+    km = None if key_valid is None else _c(key_valid, np.uint8)
+    out = {"ukeys": np.zeros(n, np.int64), "counts": np.zeros(n, np.int64),
+           "sums": np.zeros(n, dt), "mins": np.zeros(n, dt), "maxs": np.zeros(n, dt),
+           "means": np.zeros(n, np.float64)}
+    fn = L.nfh_group_agg_f64 if dt is np.float64 else L.nfh_group_agg_i64
+    ng = int(fn(_p(keys), _p(vals), _p(vm), _p(km), n, _p(out["ukeys"]), _p(out["counts"]),
+                _p(out["sums"]), _p(out["mins"]), _p(out["maxs"]), _p(out["means"]), n))
+    if ng < 0:
+        raise RuntimeError("%s rc=%d" % (fn.__name__, ng))
     return {k: v[:ng] for k, v in out.items()}
 
 

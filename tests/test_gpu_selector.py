@@ -110,7 +110,7 @@ def test_hash_strategy_overflow_falls_back(hip):
     and the sort-based path gives the result."""
     S = native.lib("selector")
     rng = np.random.default_rng(8)
-    n = (1 << 22) + 3
+    n = 1 << 22  # the most rows a forced hash-scatter accepts
     keys = rng.permutation(np.arange(n, dtype=np.int64) * 977 - (1 << 31)).astype(np.int32)
     vals = rng.integers(-100, 100, n).astype(np.int32)
     valid = rng.random(n) > 0.01
@@ -119,6 +119,29 @@ def test_hash_strategy_overflow_falls_back(hip):
         S.nfh_set_groupby_strategy(4)
         _check(native.group_agg(keys, vals, valid), ref)
         assert S.nfh_groupby_last_strategy() in (3, 4)
+    finally:
+        S.nfh_set_groupby_strategy(0)
+
+
+def test_forced_hash_refused_above_2p22_rows(hip):
+    """A forced hash-scatter on more than 2^22 rows returns -3 without
+    running; the built-in rule on the same input still runs."""
+    S = native.lib("selector")
+    L = native.lib("hip")
+    n = (1 << 22) + 1
+    keys = np.zeros(n, np.int32)
+    vals = np.ones(n, np.int32)
+    out = [np.zeros(8, dt) for dt in (np.int64, np.int64, np.int64, np.int32, np.int32,
+                                      np.float64)]
+    try:
+        S.nfh_set_groupby_strategy(4)
+        rc = L.nfh_group_agg_i32(keys.ctypes.data, vals.ctypes.data, None, None, n,
+                                 *[o.ctypes.data for o in out], 8)
+        assert rc == -3
+        S.nfh_set_groupby_strategy(0)
+        rc = L.nfh_group_agg_i32(keys.ctypes.data, vals.ctypes.data, None, None, n,
+                                 *[o.ctypes.data for o in out], 8)
+        assert rc == 1 and out[1][0] == n and out[2][0] == n
     finally:
         S.nfh_set_groupby_strategy(0)
 
