@@ -443,3 +443,90 @@ def test_sorted_run_check_separates_summation_orders():
     row = np.bincount(keys.astype(np.int64) + 300, weights=vals)[ru.astype(np.int64) + 300]
     pair = np.add.reduceat(vals, starts)
     assert not np.array_equal(_bits(row), _bits(pair))
+
+
+# ------------------------------------------------------------------ executor min / max
+def _fold_minmax(keys, vals, valid):
+    """Brute force of _group_minmax (cpu.py:2570-2601) by its definition:
+    per valid row, in row order, acc = np.minimum(acc, v) / np.maximum from
+    a +-inf (float) or iinfo (int) fill."""
+    lo, hi = {}, {}
+    fill_lo = np.inf if vals.dtype.kind == "f" else np.iinfo(vals.dtype).max
+    fill_hi = -np.inf if vals.dtype.kind == "f" else np.iinfo(vals.dtype).min
+    with np.errstate(invalid="ignore"):
+        for k, v, ok in zip(keys.tolist(), vals, valid):
+            if not ok:
+                continue
+            lo[k] = np.minimum(lo.get(k, vals.dtype.type(fill_lo)), v)
+            hi[k] = np.maximum(hi.get(k, vals.dtype.type(fill_hi)), v)
+    ks = sorted(lo)
+    return (np.array(ks, np.int64), np.array([lo[k] for k in ks], vals.dtype),
+            np.array([hi[k] for k in ks], vals.dtype))
+
+
+def group_minmax_ok(group_agg):
+    rng = np.random.default_rng(21)
+    n = 3000
+    keys = rng.integers(-5, 40, n).astype(np.int32)
+    f = rng.uniform(-10, 10, n)
+    pick = rng.integers(0, 9, n)
+    f[pick == 0], f[pick == 1], f[pick == 2] = np.nan, 0.0, -0.0
+    f[pick == 3], f[pick == 4] = np.inf, -np.inf
+    i = rng.integers(-(2 ** 63), 2 ** 63 - 1, n, dtype=np.int64)
+    valid = rng.random(n) > 0.1
+    keys[valid & (keys == 7)] = 8  # key 7 only on NULL rows: no group
+    for vals in (f, i):
+        with np.errstate(invalid="ignore"):
+            r = group_agg(keys, vals, valid)
+        uk, mn, mx = _fold_minmax(keys, vals, valid)
+        if not np.array_equal(r["ukeys"], uk):
+            return False
+        if r["mins"].tobytes() != mn.tobytes() or r["maxs"].tobytes() != mx.tobytes():
+            return False
+    # tie and NaN order on a hand case: first NaN, last zero's sign
+    # and one-signed groups (the fill must never win)
+    k = np.array([0, 0, 1, 1, 2, 2, 3, 3, 4, 4], np.int32)
+    v = np.array([0.0, -0.0, -0.0, 0.0, np.nan, 1.0, 3.0, 4.0, -3.0, -4.0])
+    with np.errstate(invalid="ignore"):
+        r = group_agg(k, v)
+    return (np.signbit(r["mins"][:2]).tolist() == [True, False]
+            and np.signbit(r["maxs"][:2]).tolist() == [True, False] and np.isnan(r["mins"][2])
+            and r["mins"][3:].tolist() == [3.0, -4.0] and r["maxs"][3:].tolist() == [4.0, -3.0])
+
+
+def _mut_group_agg(fmin=False, zero_fill=False, first_zero=False):
+    """Copies of ox.group_agg's min / max lanes with one mistake planted:
+    NaN-skipping fmin/fmax, a zero fill, or first-zero-wins ties."""
+    from oracle import executor as ox
+
+    def fn(keys, vals, valid=None, key_valid=None):
+        r = ox.group_agg(keys, vals, valid, key_valid)
+        m = np.ones(keys.shape, bool) if valid is None else np.asarray(valid, bool)
+        kk, vv = keys[m], vals[m]
+        _, inv = np.unique(kk, return_inverse=True)
+        ng = r["ukeys"].size
+        fl = vv.dtype.kind == "f"
+        lo = np.full(ng, 0 if zero_fill else (np.inf if fl else np.iinfo(vv.dtype).max), vv.dtype)
+        hi = np.full(ng, 0 if zero_fill else (-np.inf if fl else np.iinfo(vv.dtype).min),
+                     vv.dtype)
+        if first_zero:
+            order = np.arange(vv.size)[::-1]
+            np.minimum.at(lo, inv[order], vv[order])
+            np.maximum.at(hi, inv[order], vv[order])
+        else:
+            (np.fmin if fmin else np.minimum).at(lo, inv, vv)
+            (np.fmax if fmin else np.maximum).at(hi, inv, vv)
+        r["mins"], r["maxs"] = lo, hi
+        return r
+    return fn
+
+
+def test_executor_minmax_pinned():
+    from oracle import executor as ox
+    assert group_minmax_ok(ox.group_agg)
+
+
+@pytest.mark.parametrize("flags", [dict(fmin=True), dict(zero_fill=True),
+                                   dict(first_zero=True)])
+def test_executor_minmax_check_rejects_mutants(flags):
+    assert not group_minmax_ok(_mut_group_agg(**flags))
