@@ -201,6 +201,17 @@ int64_t nfh_group_agg_i64(const int32_t *keys, const int64_t *vals, const uint8_
 int64_t nfh_unique_valid_i32(const int32_t *keys, const uint8_t *valid, size_t n,
                              int32_t *uniq_out, int32_t *inv_out);
 
+/* No counterpart in the reference: nf_sort_perm_i32 (lib.rs:1326-1357) for
+ * callers that cannot allocate device memory (HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Device buffers only; the sort's
+ * own scratch is a fixed 2.1 MiB device array per GPU, so n <= 2^24.  Runs
+ * on the stream set by nfh_set_stream (else the null stream) and returns
+ * when it is done.  Calls that overlap on one GPU share that array and
+ * race.  0, -1 null pointer, -3 host buffer or n > 2^24, -6 device error. */
+int32_t nfh_sort_perm_i32_noalloc(const int32_t *keys, size_t n, uint8_t descending,
+                                  int32_t *perm_out, uint32_t *tmp0, uint32_t *tmp1,
+                                  int32_t *tmp_p);
+
 /* Learned-CDF model: a 1 -> 32 -> 32 -> 1 fp32 ReLU network over the
  * normalised key word (w - wmin) / span, w = (uint32)key ^ 0x80000000.
  * p = w1[32] b1[32] w2[32][32] b2[32] w3[32] b3. */

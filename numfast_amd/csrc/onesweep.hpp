@@ -521,7 +521,7 @@ template <class K, bool FIRST, bool LAST, class DG>
 void onesweep_pass(const K* keys, const word_t<K>* win, const int32_t* iin, word_t<K>* wout,
                    int32_t* iout, K* kout, const uint32_t* bins, uint64_t* lb, uint32_t* ctr,
                    uint32_t n, const DG& dg, uint32_t* next_hist, bool desc, hipStream_t s,
-                   const char* prof_name = "radix_onesweep") {
+                   const char* prof_name = "radix_onesweep", uint32_t epoch_in = 0) {
   using G = SortGeom<K>;
   constexpr size_t LDS = G::LDS + DG::EXTRA_LDS;
   static_assert(LDS <= 160 * 1024, "LDS budget of the pass kernel");
@@ -538,7 +538,8 @@ void onesweep_pass(const K* keys, const word_t<K>* win, const int32_t* iin, word
   const bool lr = lds_rank_supported();
   const bool next = next_hist != nullptr;
   const uint32_t T = (n + G::TILE - 1) / G::TILE;
-  const uint32_t epoch = lookback_epoch(s);
+  // epoch_in != 0: the caller's epoch, newer than every word in lb
+  const uint32_t epoch = epoch_in != 0 ? epoch_in : lookback_epoch(s);
   // persistent: one workgroup per CU (the tile buffers take most of the
   // LDS), NCHAIN-aligned so every chain has the same number of workers
   const unsigned grid =

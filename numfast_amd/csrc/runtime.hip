@@ -87,24 +87,26 @@ __global__ __launch_bounds__(1024) void k_lds_rank_check(uint32_t trials,
   if (v) atomicAdd(bad, v);
 }
 
+// Mismatch counter of the check (a static device word: no allocation).
+static __device__ unsigned long long g_rank_bad;
+
 bool lds_rank_supported() {
   static std::once_flag once;
   static bool ok = false;
   std::call_once(once, [] {
     ensure_device();
     unsigned long long* d = nullptr;
-    if (hipMalloc(&d, sizeof(unsigned long long)) != hipSuccess) return;
+    if (hipGetSymbolAddress(reinterpret_cast<void**>(&d), HIP_SYMBOL(g_rank_bad)) != hipSuccess)
+      return;
     unsigned long long h = 1;
-    hipStream_t s = stream();
+    // the null stream: no stream is created here
+    hipStream_t s = nullptr;
     fill_u64(d, 0, 1, s);
-    {
-      hipLaunchKernelGGL(k_lds_rank_check, dim3(64), dim3(1024), 0, s, 256u, d);
-      if (hipGetLastError() == hipSuccess &&
-          hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
-          hipStreamSynchronize(s) == hipSuccess)
-        ok = (h == 0);
-    }
-    (void)hipFree(d);
+    hipLaunchKernelGGL(k_lds_rank_check, dim3(64), dim3(1024), 0, s, 256u, d);
+    if (hipGetLastError() == hipSuccess &&
+        hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess)
+      ok = (h == 0);
   });
   return ok;
 }
@@ -231,9 +233,16 @@ struct Pending {
 thread_local std::vector<Pending> t_pending;
 }  // namespace
 
+namespace {
+thread_local bool t_prof_suspended = false;
+}  // namespace
+
+ProfSuspend::ProfSuspend() : prev_(t_prof_suspended) { t_prof_suspended = true; }
+ProfSuspend::~ProfSuspend() { t_prof_suspended = prev_; }
+
 ProfScope::ProfScope(const char* name, hipStream_t s, double bytes)
     : name_(name), s_(s), bytes_(bytes) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || t_prof_suspended) return;
   NFH_CHECK(hipEventCreate(&a_));
   NFH_CHECK(hipEventCreate(&b_));
   NFH_CHECK(hipEventRecord(a_, s_));

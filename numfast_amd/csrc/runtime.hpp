@@ -67,12 +67,11 @@ enum Slot : int {
 struct Ctx {
   hipStream_t stream = nullptr;
   bool stream_set_by_user = false;
-  std::vector<void*> slot;
-  std::vector<size_t> cap;
+  void* slot[S_COUNT] = {};
+  size_t cap[S_COUNT] = {};
   void* pinned = nullptr;  // small pinned buffer for scalar readbacks
   size_t pinned_cap = 0;
   uint32_t epoch = 0;  // 30-bit look-back epoch (see sort.hip)
-  Ctx() : slot(S_COUNT, nullptr), cap(S_COUNT, 0) {}
   ~Ctx();
 };
 
@@ -175,6 +174,16 @@ class ProfScope {
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 void prof_collect();
+// While one lives on this thread, ProfScope creates no events (entry points
+// that must not allocate run under it).
+class ProfSuspend {
+ public:
+  ProfSuspend();
+  ~ProfSuspend();
+
+ private:
+  bool prev_;
+};
 
 }  // namespace nfh
 

@@ -77,9 +77,20 @@ uint32_t lookback_epoch(hipStream_t s) {
   return e;
 }
 
+static_assert(SORT_NCHAIN == NCHAIN, "chains of the sort workspace");
+static_assert(SORT_TILE_I32 == uint32_t(SortGeom<int32_t>::TILE), "int32 sort tile");
+
 template <class K>
-void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>* t0,
-                   word_t<K>* t1, int32_t* tp, K* sorted_out, hipStream_t s) {
+size_t sort_lookback_words(size_t n) {
+  return (n + SortGeom<K>::TILE - 1) / SortGeom<K>::TILE * 256;
+}
+template size_t sort_lookback_words<int32_t>(size_t);
+template size_t sort_lookback_words<int64_t>(size_t);
+
+template <class K>
+void radix_argsort_ws(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>* t0,
+                      word_t<K>* t1, int32_t* tp, K* sorted_out, const SortWs& ws,
+                      hipStream_t s) {
   using T = Key<K>;
   using W = typename T::W;
   constexpr int P = T::PASSES;
@@ -88,24 +99,32 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
   if (n == 0) return;
   // meta: or / and words | per pass: per-chain hist | per pass: bins | tile counters
   constexpr size_t OW = 2 * sizeof(W) / 4;
-  uint32_t* meta = scratch_t<uint32_t>(S_META, OW + 2 * P * CH + P * NCHAIN);
+  uint32_t* meta = ws.meta;
   W* orand = reinterpret_cast<W*>(meta);
   uint32_t* hist = meta + OW;
   uint32_t* bins = hist + P * CH;
   uint32_t* ctrs = bins + P * CH;
-  fill_u32(meta, 0, OW / 2, s);
-  fill_u32(meta + OW / 2, 0xFFFFFFFFu, OW / 2, s);
   fill_u32(hist, 0, 2 * P * CH + P * NCHAIN, s);
-  // per-chain histogram of digit 0 and the OR / AND of all key words
-  chain_hist_first<K>(keys, n, RadixDigit{0, 0}, desc, hist, orand, s, "radix_hist");
-  W* h_oa = static_cast<W*>(pinned(2 * sizeof(W)));
-  NFH_CHECK(hipMemcpyAsync(h_oa, orand, 2 * sizeof(W), hipMemcpyDeviceToHost, s));
-  NFH_CHECK(hipStreamSynchronize(s));
-  const W differ = h_oa[0] ^ h_oa[1];
   int active[P];
   int na = 0;
-  for (int p = 0; p < P; ++p)  // a digit every key shares is an identity pass
-    if ((differ >> (8 * p)) & W(255)) active[na++] = p;
+  if (ws.all_passes) {
+    // every digit pass runs (a digit all keys share is an identity pass):
+    // no OR / AND readback, so no host synchronisation
+    chain_hist_first<K>(keys, n, RadixDigit{0, 0}, desc, hist, static_cast<W*>(nullptr), s,
+                        "radix_hist");
+    for (int p = 0; p < P; ++p) active[na++] = p;
+  } else {
+    fill_u32(meta, 0, OW / 2, s);
+    fill_u32(meta + OW / 2, 0xFFFFFFFFu, OW / 2, s);
+    // per-chain histogram of digit 0 and the OR / AND of all key words
+    chain_hist_first<K>(keys, n, RadixDigit{0, 0}, desc, hist, orand, s, "radix_hist");
+    W* h_oa = static_cast<W*>(pinned(2 * sizeof(W)));
+    NFH_CHECK(hipMemcpyAsync(h_oa, orand, 2 * sizeof(W), hipMemcpyDeviceToHost, s));
+    NFH_CHECK(hipStreamSynchronize(s));
+    const W differ = h_oa[0] ^ h_oa[1];
+    for (int p = 0; p < P; ++p)  // a digit every key shares is an identity pass
+      if ((differ >> (8 * p)) & W(255)) active[na++] = p;
+  }
   if (na == 0) {
     hipLaunchKernelGGL((k_iota<K>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, perm,
                        sorted_out, keys, n);
@@ -115,8 +134,7 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
   if (active[0] != 0)  // rare: digit 0 is shared, count the first active one
     chain_hist_first<K>(keys, n, RadixDigit{8 * active[0], 0}, desc, hist + CH,
                         static_cast<W*>(nullptr), s, "radix_hist");
-  const size_t tiles = (n + SortGeom<K>::TILE - 1) / SortGeom<K>::TILE;
-  uint64_t* lb = lookback_words(tiles * 256, s);
+  uint64_t* lb = ws.lb;
   // Intermediate pairs: A = (t1, tp), B = (t0, perm).  The last pass must
   // read A so that it can write perm; k = na-1 intermediate outputs.
   // Pass a's per-chain counts are hist + CH * slot(a): slot 0 holds digit
@@ -132,30 +150,41 @@ void radix_argsort(const K* keys, size_t n_, bool desc, int32_t* perm, word_t<K>
     uint32_t* nh = last ? nullptr : hist + CH * size_t(a + 1 + off);
     uint32_t* bs = bins + CH * size_t(a);
     uint32_t* ctr = ctrs + NCHAIN * a;
+    const uint32_t ep = ws.epoch0 != 0 ? ws.epoch0 + uint32_t(a) : 0u;
     const RadixDigit dg{8 * p, last ? 0 : 8 * active[a + 1]};
     hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, h, bs);
     NFH_LAUNCH_CHECK();
     if (last) {
       if (first)
         onesweep_pass<K, true, true>(keys, nullptr, nullptr, nullptr, perm, sorted_out, bs, lb,
-                                     ctr, n, dg, nh, desc, s);
+                                     ctr, n, dg, nh, desc, s, "radix_onesweep", ep);
       else
         onesweep_pass<K, false, true>(nullptr, cw, ci, nullptr, perm, sorted_out, bs, lb, ctr,
-                                      n, dg, nh, desc, s);
+                                      n, dg, nh, desc, s, "radix_onesweep", ep);
     } else {
       const bool use_a = ((k - 1 - a) % 2) == 0;
       W* ow = use_a ? t1 : t0;
       int32_t* oi = use_a ? tp : perm;
       if (first)
         onesweep_pass<K, true, false>(keys, nullptr, nullptr, ow, oi, nullptr, bs, lb, ctr, n,
-                                      dg, nh, desc, s);
+                                      dg, nh, desc, s, "radix_onesweep", ep);
       else
         onesweep_pass<K, false, false>(nullptr, cw, ci, ow, oi, nullptr, bs, lb, ctr, n, dg, nh,
-                                       desc, s);
+                                       desc, s, "radix_onesweep", ep);
       cw = ow;
       ci = oi;
     }
   }
+}
+
+template <class K>
+void radix_argsort(const K* keys, size_t n, bool desc, int32_t* perm, word_t<K>* t0,
+                   word_t<K>* t1, int32_t* tp, K* sorted_out, hipStream_t s) {
+  if (n == 0) return;
+  SortWs ws;
+  ws.meta = scratch_t<uint32_t>(S_META, sort_meta_words<K>());
+  ws.lb = lookback_words(sort_lookback_words<K>(n), s);
+  radix_argsort_ws<K>(keys, n, desc, perm, t0, t1, tp, sorted_out, ws, s);
 }
 
 void radix_histograms_i32(const int32_t* keys, uint32_t n, uint32_t* hist, hipStream_t s) {
@@ -169,6 +198,9 @@ void radix_histograms_i32(const int32_t* keys, uint32_t n, uint32_t* hist, hipSt
 
 template void radix_argsort<int32_t>(const int32_t*, size_t, bool, int32_t*, uint32_t*,
                                      uint32_t*, int32_t*, int32_t*, hipStream_t);
+template void radix_argsort_ws<int32_t>(const int32_t*, size_t, bool, int32_t*, uint32_t*,
+                                        uint32_t*, int32_t*, int32_t*, const SortWs&,
+                                        hipStream_t);
 template void radix_argsort<int64_t>(const int64_t*, size_t, bool, int32_t*, uint64_t*,
                                      uint64_t*, int32_t*, int64_t*, hipStream_t);
 

@@ -133,6 +133,7 @@ GROUPS = {
         ("nfh_group_agg_f64", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
         ("nfh_group_agg_i64", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
         ("nfh_unique_valid_i32", [_vp, _vp, _sz, _vp, _vp], _i64),
+        ("nfh_sort_perm_i32_noalloc", [_vp, _sz, _u8] + [_vp] * 4, _i32),
         ("nfh_profile_enable", [_i32], _i32),
         ("nfh_profile_reset", [], _i32),
         ("nfh_profile_get", [ctypes.c_char_p, _vp, _vp], _i32),
