@@ -256,6 +256,70 @@ def i32_gate_ok(vals, n=None, _b=None):
     return nn * max(abs(mn), abs(mx)) < 2 ** 31 - 1
 
 
+def _bounds(v):
+    """(min, max) as Python ints, or None for a float / empty column."""
+    v = np.asarray(v)
+    if v.dtype.kind not in "iu" or v.size == 0:
+        return None
+    return int(v.min()), int(v.max())
+
+
+def int_exact_ok(vcols):
+    """Executor gate for the float64 lanes (cpu.py:424-446): integer
+    columns summed through nf_group_sum_count / nf_group_multi_sum_count
+    stay exact while n * max|v| < 2**53 (every partial sum is then a
+    float64 integer)."""
+    for v in vcols:
+        b = _bounds(v)
+        if b is not None and int(np.asarray(v).size) * max(abs(b[0]), abs(b[1])) >= 2 ** 53:
+            return False
+    return True
+
+
+def intlane_ok(vcols):
+    """Executor gate for the integer lanes (cpu.py:353-378): every integer
+    column has n * max|v| < 2**63 - 1, so the int32 / checked int64 lanes
+    serve it exactly."""
+    for v in vcols:
+        b = _bounds(v)
+        if b is not None and int(np.asarray(v).size) * max(abs(b[0]), abs(b[1])) >= 2 ** 63 - 1:
+            return False
+    return True
+
+
+def native_st(keys, vcols, m):
+    """The executor's single-thread native dense aggregate (cpu.py:461-518)
+    over int32 codes in [0, m): integer-bearing columns take the mixed lanes
+    (int32 when the i32 gate holds, checked int64, float64), anything else
+    or a lane failure takes the float64 fused lanes when int_exact_ok holds;
+    then one carry compaction per column (nf_carry_build_*).  Returns
+    (ukeys, counts, sums_list) or None when the executor would fall back."""
+    try:
+        keys = _c(keys, np.int32)
+        cols = [np.ascontiguousarray(v) for v in vcols]
+        res = None
+        if any(c.dtype.kind in "iu" for c in cols):
+            res = mixed_sum_count(keys, cols, m)
+        if res is None:
+            if not int_exact_ok(cols):
+                return None
+            if len(cols) == 1:
+                s1, counts_m = fused_sum_count(keys, _c(cols[0], np.float64), m)
+                res = ([s1], counts_m)
+            else:
+                res = multi_sum_count(keys, [_c(c, np.float64) for c in cols], m)
+        sums_list, counts_m = res
+        ukeys, counts, out = None, None, []
+        for s_m in sums_list:
+            uk, cnt, sm = carry_build(counts_m, s_m)
+            if ukeys is None:
+                ukeys, counts = uk, cnt
+            out.append(sm)
+        return ukeys, counts, out
+    except (OverflowError, RuntimeError, AssertionError, ValueError):
+        return None
+
+
 def fused_sum_count(keys, vals, g):
     """native_cpu.py:791-802 -> nf_group_sum_count."""
     keys, vals = _c(keys, np.int32), _c(vals, np.float64)
@@ -589,6 +653,35 @@ def sort_perm_step(keys, descending):
         if ok_d and st_d:
             return np.arange(n - 1, -1, -1, dtype=np.int32), "rev"
     return sort_perm_raw(k, descending), "native"
+
+
+def sort_perm(cols, descending, valid=None):
+    """Multi-key stable sort permutation (cpu.py:1174-1225, _sort_perm):
+    successive stable key steps from the last key to the first, each from
+    sort_perm_step (the HIP radix argsort or a monotone shortcut); rows
+    with valid False go last in input order.  int32 positions."""
+    cols = [np.asarray(c) for c in cols]
+    n = cols[0].size
+    for c in cols:
+        if c.size != n:
+            raise ValueError("sort keys size mismatch %d != %d" % (c.size, n))
+    if valid is not None:
+        msk = np.asarray(valid, dtype=bool).reshape(-1)
+        if msk.size != n:
+            raise ValueError("sort validity size %d != keys %d" % (msk.size, n))
+        vpos, ipos = np.flatnonzero(msk), np.flatnonzero(~msk)
+    else:
+        vpos, ipos = np.arange(n), np.zeros(0, dtype=np.int64)
+    idx = np.arange(vpos.size)
+    for c, desc in zip(reversed(cols), reversed(list(descending))):
+        k = c[vpos][idx] if vpos.size else c[:0]
+        step, backend = sort_perm_step(k, bool(desc))
+        if backend != "ident":
+            idx = idx[step]
+    perm = np.empty(n, dtype=np.int32)
+    perm[:vpos.size] = vpos[idx]
+    perm[vpos.size:] = ipos
+    return perm
 
 
 # ------------------------------------------------------------ additive surface

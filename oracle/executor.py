@@ -141,6 +141,39 @@ def group_agg(keys, vals, valid=None, key_valid=None):
             "mins": mins, "maxs": maxs, "means": means}
 
 
+def i32_gate_ok(vals, n=None):
+    """src/Drivers/CPU/_lib/native_cpu.py:360-379: the int32-direct lane is
+    safe when n * max|v| < 2**31 - 1 (all rows in one group)."""
+    vals = np.asarray(vals)
+    if vals.size == 0:
+        return True
+    nn = vals.size if n is None else int(n)
+    return nn * max(abs(int(vals.min())), abs(int(vals.max()))) < 2 ** 31 - 1
+
+
+def int_exact_ok(vcols):
+    """src/Drivers/CPU/_lib/cpu.py:424-446 (_int_exact_ok): integer columns
+    stay exact through the float64 lanes iff n * max|v| < 2**53; float and
+    empty columns do not count."""
+    for v in vcols:
+        v = np.asarray(v)
+        if v.dtype.kind in "iu" and v.size:
+            if v.size * max(abs(int(v.min())), abs(int(v.max()))) >= 2 ** 53:
+                return False
+    return True
+
+
+def intlane_ok(vcols):
+    """src/Drivers/CPU/_lib/cpu.py:353-378 (_intlane_ok): every integer
+    column has n * max|v| < 2**63 - 1."""
+    for v in vcols:
+        v = np.asarray(v)
+        if v.dtype.kind in "iu" and v.size:
+            if v.size * max(abs(int(v.min())), abs(int(v.max()))) >= 2 ** 63 - 1:
+                return False
+    return True
+
+
 def exact_sums_f64(keys, vals, g):
     """Exact-rational per-group sums (Fraction), for error reporting only.
     Pure Python: small inputs."""
