@@ -196,6 +196,16 @@ int64_t nfh_group_agg_i64(const int32_t *keys, const int64_t *vals, const uint8_
                           int64_t *sums, int64_t *mins, int64_t *maxs, double *means,
                           size_t cap);
 
+/* Positional take (the IR gather, src/Drivers/CPU/_lib/cpu.py:1110-1138,
+ * :3170-3176): out[i] = values[idx[i]] for k indices of idx_bytes (4 or 8),
+ * elements of elem_size bytes (1, 4 or 8); valid_in / valid_out (both or
+ * neither) take the validity bytes alike.  0, -1 null pointer, -2 an index
+ * outside [0, n) (the reference raises; outputs unspecified), -3 bad sizes,
+ * -6 device error. */
+int32_t nfh_gather(const void *values, size_t n, size_t elem_size, const void *idx,
+                   size_t idx_bytes, size_t k, void *out, const uint8_t *valid_in,
+                   uint8_t *valid_out);
+
 /* np.unique over valid rows; inv = -1 on NULL rows
  * (src/Drivers/CPU/_lib/cpu.py:4032-4036).  Returns ng. */
 int64_t nfh_unique_valid_i32(const int32_t *keys, const uint8_t *valid, size_t n,

@@ -134,6 +134,7 @@ GROUPS = {
         ("nfh_group_agg_i64", [_vp] * 4 + [_sz] + [_vp] * 6 + [_sz], _i64),
         ("nfh_unique_valid_i32", [_vp, _vp, _sz, _vp, _vp], _i64),
         ("nfh_sort_perm_i32_noalloc", [_vp, _sz, _u8] + [_vp] * 4, _i32),
+        ("nfh_gather", [_vp, _sz, _sz, _vp, _sz, _sz, _vp, _vp, _vp], _i32),
         ("nfh_profile_enable", [_i32], _i32),
         ("nfh_profile_reset", [], _i32),
         ("nfh_profile_get", [ctypes.c_char_p, _vp, _vp], _i32),
@@ -311,6 +312,58 @@ def native_st(keys, vcols, m):
         sums_list, counts_m = res
         ukeys, counts, out = None, None, []
         for s_m in sums_list:
+            uk, cnt, sm = carry_build(counts_m, s_m)
+            if ukeys is None:
+                ukeys, counts = uk, cnt
+            out.append(sm)
+        return ukeys, counts, out
+    except (OverflowError, RuntimeError, AssertionError, ValueError):
+        return None
+
+
+def native_mt(keys, vcols, m, t):
+    """The executor's host multi-thread dense aggregate (cpu.py:521-636,
+    the S-state path): a contiguous row partition into t slices, each
+    aggregated by one pool worker through the lanes (mixed lanes when a
+    column is an integer, the float64 lanes otherwise; ctypes releases the
+    GIL, and each thread has its own library stream and scratch), then a
+    merge of the slice states in slice order (integer states widened to
+    int64) and one carry compaction per column.  Returns (ukeys, counts,
+    sums_list) or None where the executor would fall back."""
+    from concurrent.futures import ThreadPoolExecutor
+    keys = _c(keys, np.int32)
+    cols = [np.ascontiguousarray(v) for v in vcols]
+    mix = any(c.dtype.kind in "iu" for c in cols)
+    if (not mix and not int_exact_ok(cols)) or (mix and not intlane_ok(cols)):
+        return None
+    n = int(keys.size)
+    t = max(1, min(int(t), n))
+    bounds = np.linspace(0, n, t + 1).astype(np.int64)
+
+    def work(w):
+        a, b = int(bounds[w]), int(bounds[w + 1])
+        kk, vv = keys[a:b], [c[a:b] for c in cols]
+        if mix:
+            r = mixed_sum_count(kk, vv, m)
+            if r is not None:
+                return r
+        if len(vv) == 1:
+            s1, cc = fused_sum_count(kk, _c(vv[0], np.float64), m)
+            return [s1], cc
+        return multi_sum_count(kk, [_c(x, np.float64) for x in vv], m)
+
+    try:
+        with ThreadPoolExecutor(max_workers=t) as ex:
+            parts = list(ex.map(work, range(t)))
+        sums_m = [np.zeros(m, dtype=np.int64 if s_.dtype.kind in "iu" else s_.dtype)
+                  for s_ in parts[0][0]]
+        counts_m = np.zeros(m, dtype=np.int64)
+        for ss, cc in parts:  # slice order: the sequential merge
+            for j, s_ in enumerate(ss):
+                sums_m[j] += s_
+            counts_m += cc
+        ukeys, counts, out = None, None, []
+        for s_m in sums_m:
             uk, cnt, sm = carry_build(counts_m, s_m)
             if ukeys is None:
                 ukeys, counts = uk, cnt
@@ -682,6 +735,31 @@ def sort_perm(cols, descending, valid=None):
     perm[:vpos.size] = vpos[idx]
     perm[vpos.size:] = ipos
     return perm
+
+
+def gather(values, indices, valid=None):
+    """Positional take (cpu.py:1110-1138 _gather_ref, the IR gather at
+    :3170-3176): values[indices] in index order, validity taken alike.
+    Raises IndexError for an index outside [0, n) and TypeError for
+    non-integer indices, as the reference refuses them."""
+    v = np.ascontiguousarray(np.asarray(values))
+    idx = np.asarray(indices)
+    if idx.dtype.kind not in "iu":
+        raise TypeError("gather needs integer indices, got %s" % idx.dtype)
+    idx = _c(idx, np.int64 if idx.dtype.itemsize > 4 or idx.dtype == np.uint32 else np.int32)
+    if v.dtype.itemsize not in (1, 4, 8):
+        raise TypeError("gather: 1-, 4- or 8-byte elements only, got %s" % v.dtype)
+    k = idx.size
+    out = np.empty(k, dtype=v.dtype)
+    vin = None if valid is None else _c(valid, np.uint8)
+    vout = None if valid is None else np.empty(k, dtype=np.uint8)
+    rc = int(lib("hip").nfh_gather(_p(v), v.size, v.dtype.itemsize, _p(idx), idx.dtype.itemsize,
+                                   k, _p(out), _p(vin), _p(vout)))
+    if rc == -2:
+        raise IndexError("gather index out of range")
+    if rc != 0:
+        raise RuntimeError("nfh_gather rc=%d" % rc)
+    return out, (None if vout is None else vout.astype(bool))
 
 
 # ------------------------------------------------------------ additive surface
