@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="only the group-by + unique step")
     # the selector leg (400 labelled columns, every strategy forced on each)
     # runs only when named: --legs ...,selector
-    ap.add_argument("--legs", default="step,argsort,learned,batch,uniform,ghash",
+    ap.add_argument("--legs", default="step,host,argsort,learned,batch,uniform,ghash,cpu",
                     help="comma list of legs to run (for profiling one workload); "
                          "add 'selector' for the 400-column strategy-selector leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -148,26 +148,31 @@ def timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(n_sample=1 << 22):
-    """The oracle's C restatement of the reference route for this workload
-    (NULL compaction, radix unique on the surviving keys -> codes, then
-    per-code sums/counts (np.bincount) and min/max (np.minimum/maximum.at),
-    mean = sum/count; unique of the key column over its valid rows),
-    single-threaded, on a bounded sample of the same distribution."""
+def _zipf_host(rng, n, ids=1 << 24, s=1.1):
+    """Bounded Zipf(s) keys over 2^24 ids (the workload generator's rule,
+    on the host)."""
     import numpy as np
+    ranks = np.arange(1, ids + 1, dtype=np.float64)
+    cdf = np.cumsum(ranks ** -s)
+    cdf /= cdf[-1]
+    r = np.minimum(np.searchsorted(cdf, rng.random(n), side="right"), ids - 1)
+    return rng.permutation(ids).astype(np.int32)[r]
+
+
+def _oracle():
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle.executor import load_c_oracle
     lib, _ = load_c_oracle()
-    rng = np.random.default_rng(0)
-    ids = 1 << 24
-    ranks = np.arange(1, ids + 1, dtype=np.float64)
-    cdf = np.cumsum(ranks ** -1.1)
-    cdf /= cdf[-1]
-    r = np.minimum(np.searchsorted(cdf, rng.random(n_sample), side="right"), ids - 1)
-    keys = rng.permutation(ids).astype(np.int32)[r]
-    vals = rng.integers(-(1 << 15), 1 << 15, n_sample).astype(np.int32)
-    valid = rng.random(n_sample) >= 0.05
-    kvalid = rng.random(n_sample) >= 0.05
+    return lib
+
+
+def _reference_route_step(lib, keys, vals, valid, kvalid):
+    """The reference's route for the group-by + unique step (hash lane ->
+    the native radix unique, cpu.py:3508-3563, groupindex.py:178-211):
+    NULL compaction, the C restatement of nf_unique_inverse_i32 for codes,
+    np.bincount sums / counts, np.minimum.at / np.maximum.at, mean = sum /
+    count; then unique of the key column over its valid rows."""
+    import numpy as np
 
     def unique_c(k):
         n = k.size
@@ -178,33 +183,141 @@ def cpu_baseline(n_sample=1 << 22):
                                        tp.ctypes.data)
         return u[:ng], inv
 
-    def step():
-        kk = np.ascontiguousarray(keys[valid])
-        vv = vals[valid]
-        u, inv = unique_c(kk)
-        ng = u.size
-        sums = np.bincount(inv, weights=vv.astype(np.int64), minlength=ng).astype(np.int64)
-        counts = np.bincount(inv, minlength=ng).astype(np.int64)
-        mins = np.full(ng, np.iinfo(np.int32).max, np.int32)
-        maxs = np.full(ng, np.iinfo(np.int32).min, np.int32)
-        np.minimum.at(mins, inv, vv)
-        np.maximum.at(maxs, inv, vv)
-        _ = sums.astype(np.float64) / counts
-        uk, _ = unique_c(np.ascontiguousarray(keys[kvalid]))
-        return ng, uk.size
+    kk = np.ascontiguousarray(keys[valid])
+    vv = vals[valid]
+    u, inv = unique_c(kk)
+    ng = u.size
+    sums = np.bincount(inv, weights=vv.astype(np.int64), minlength=ng).astype(np.int64)
+    counts = np.bincount(inv, minlength=ng).astype(np.int64)
+    mins = np.full(ng, np.iinfo(np.int32).max, np.int32)
+    maxs = np.full(ng, np.iinfo(np.int32).min, np.int32)
+    np.minimum.at(mins, inv, vv)
+    np.maximum.at(maxs, inv, vv)
+    _ = sums.astype(np.float64) / counts
+    del kk, vv, inv
+    uk, _ = unique_c(np.ascontiguousarray(keys[kvalid]))
+    return ng, uk.size
 
-    step()
+
+def cpu_baseline_full(host):
+    """The headline workload's reference route at its full size (2^28 rows
+    copied from the GPU's seeded inputs), one run, one thread."""
+    lib = _oracle()
+    t0 = time.perf_counter()
+    ng, nu = _reference_route_step(lib, *host)
+    t = time.perf_counter() - t0
+    n = host[0].size
+    return {"value": n / t, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": ("the full step: group-by sum/count/min/max/mean + unique, %d rows "
+                       "(the same seeded Zipf(1.1) / 2^24-id / 5%%-NULL inputs, copied to the "
+                       "host); C restatement of the reference radix unique (gcc -O3, one "
+                       "thread) + NumPy bincount / minimum.at (reference route), one run of "
+                       "%.1f s; ng %d / %d" % (n, t, ng, nu))}
+
+
+def _median_time(fn, reps):
+    fn()
     ts = []
-    for _ in range(3):
+    for _ in range(reps):
         t0 = time.perf_counter()
-        step()
+        fn()
         ts.append(time.perf_counter() - t0)
-    t = sorted(ts)[len(ts) // 2]
-    return {"value": n_sample / t, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": ("group-by sum/count/min/max/mean + unique, %d rows of the same Zipf(1.1) "
-                       "/ 2^24-id / 5%%-NULL distribution; C restatement of the reference "
-                       "radix unique + NumPy bincount/minimum.at (reference route), median "
-                       "of 3" % n_sample)}
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_baselines_extra(torch, L, native, threads):
+    """Reported CPU baselines beside the headline one (median of 5 unless
+    stated): the NumPy executor on a config-2 sample, config 1 (the CPU
+    configuration: NumPy min/max branch, and the native sum/count lanes in
+    the executor's thread-pool slice-and-merge form), and the stable
+    argsort, each beside this GPU's figure for the same call."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import executor as ox
+    lib = _oracle()
+    out = {"host_threads": threads}
+    rng = np.random.default_rng(0)
+    # config-2 sample through the NumPy executor restatement (np.unique +
+    # bincount + minimum.at, cpu.py:2373-2601)
+    n2 = 1 << 22
+    k2 = _zipf_host(rng, n2)
+    v2 = rng.integers(-(1 << 15), 1 << 15, n2).astype(np.int32)
+    m2 = rng.random(n2) >= 0.05
+    t = _median_time(lambda: ox.group_agg(k2, v2, m2), 3)
+    out["config2_numpy_executor_2^22"] = {"rows_per_s": n2 / t, "ms": round(t * 1e3, 1),
+                                          "cores": 1}
+    # config 1: 2^20 rows, key U[0,1000), v_i32 U[-1000,1000], v_f64 U[0,1000)
+    rng = np.random.default_rng(0)
+    n1 = 1_048_576
+    key = rng.integers(0, 1000, n1, dtype=np.int32)
+    vi = rng.integers(-1000, 1001, n1, dtype=np.int32)
+    vf = rng.uniform(0, 1000, n1)
+
+    def numpy_query():
+        ox.group_agg(key, vi)
+        ox.group_agg(key, vf)
+    t_np = _median_time(numpy_query, 5)
+
+    def mt_lanes():
+        # cpu.py:521-636: contiguous slices, one worker each, ordered merge
+        bounds = np.linspace(0, n1, threads + 1).astype(np.int64)
+
+        def work(w):
+            a, b = int(bounds[w]), int(bounds[w + 1])
+            kk, ff = np.ascontiguousarray(key[a:b]), np.ascontiguousarray(vf[a:b])
+            s, c = np.zeros(1000), np.zeros(1000, np.int64)
+            lib.nf_group_sum_count(kk.ctypes.data, ff.ctypes.data, kk.size, s.ctypes.data,
+                                   c.ctypes.data, 1000)
+            si, ci = np.zeros(1000, np.int32), np.zeros(1000, np.int64)
+            ii = np.ascontiguousarray(vi[a:b])
+            lib.nf_group_variant_i32(kk.ctypes.data, ii.ctypes.data, kk.size, 0, si.ctypes.data,
+                                     ci.ctypes.data, 1000)
+            return s, c, si
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            parts = list(ex.map(work, range(threads)))
+        sf, cnt, si = np.zeros(1000), np.zeros(1000, np.int64), np.zeros(1000, np.int64)
+        for s, c, i in parts:
+            sf += s
+            cnt += c
+            si += i
+    t_mt = _median_time(mt_lanes, 5)
+    gi = lambda: (native.group_agg(key, vi), native.group_agg64(key, vf))
+    t_gh = _median_time(gi, 5)
+    kd, vid, vfd = (torch.from_numpy(x).cuda() for x in (key, vi, vf))
+    outs = [torch.empty(1000, dtype=dt, device="cuda") for dt in
+            (torch.int64, torch.int64, torch.int64, torch.int32, torch.int32, torch.float64)]
+    outf = [torch.empty(1000, dtype=dt, device="cuda") for dt in
+            (torch.int64, torch.int64, torch.float64, torch.float64, torch.float64,
+             torch.float64)]
+
+    def gpu_dev():
+        L.nfh_group_agg_i32(kd.data_ptr(), vid.data_ptr(), None, None, n1,
+                            *[o.data_ptr() for o in outs], 1000)
+        L.nfh_group_agg_f64(kd.data_ptr(), vfd.data_ptr(), None, None, n1,
+                            *[o.data_ptr() for o in outf], 1000)
+    t_gd = timed(torch, gpu_dev, 20, 3)
+    out["config1_2^20"] = {
+        "numpy_executor_ms": round(t_np * 1e3, 2),
+        "native_lanes_mt_ms": round(t_mt * 1e3, 2),
+        "native_lanes_mt_note": "sum/count lanes only (the executor's min/max branch is NumPy)",
+        "gpu_device_buffers_ms": round(t_gd * 1e3, 3),
+        "gpu_host_buffers_ms": round(t_gh * 1e3, 2),
+        "rows_per_s": {"numpy_executor": n1 / t_np, "native_lanes_mt": n1 / t_mt,
+                       "gpu_device": n1 / t_gd, "gpu_host_buffers": n1 / t_gh}}
+    # stable argsort: C restatement (one thread) and np.argsort(stable)
+    na = 1 << 24
+    ka = np.random.default_rng(0).integers(-(2 ** 31), 2 ** 31, na).astype(np.int32)
+    perm = np.empty(na, np.int32)
+    t0b, t1b, tpb = np.empty(na, np.uint32), np.empty(na, np.uint32), np.empty(na, np.int32)
+    t_c = _median_time(lambda: lib.nf_sort_perm_i32(ka.ctypes.data, na, 0, perm.ctypes.data,
+                                                    t0b.ctypes.data, t1b.ctypes.data,
+                                                    tpb.ctypes.data), 3)
+    t_a = _median_time(lambda: np.argsort(ka, kind="stable"), 3)
+    out["argsort_2^24_uniform"] = {"c_restatement_rows_per_s": na / t_c,
+                                   "numpy_stable_rows_per_s": na / t_a,
+                                   "c_ms": round(t_c * 1e3, 1), "numpy_ms": round(t_a * 1e3, 1),
+                                   "cores": 1}
+    return out
 
 
 def ghash_leg(torch, L, n, steps):
@@ -460,7 +573,33 @@ def main():
         extra.update({"groupby_ng": ng, "unique_ng": nu, "kernels": kernels,
                       "step_alg_GBps": round(step_bytes / t / 1e9, 1),
                       "step_alg_frac": round(step_bytes / t / 1e9 / HBM_PEAK_GBS, 4)})
+    host = None
+    if not args.no_cpu or "host" in legs:
+        host = [keys.cpu().numpy(), vals.cpu().numpy(), valid.cpu().numpy().astype(bool),
+                kvalid.cpu().numpy().astype(bool)]
     del keys, vals, valid, kvalid
+    if "host" in legs and host is not None:
+        # the boundary as NumFast calls it: NumPy host buffers, staged over
+        # PCIe by the library (never the reported value)
+        import numpy as np
+        kh, vh = host[0], host[1]
+        mh, kmh = host[2].astype(np.uint8), host[3].astype(np.uint8)
+        oh = [np.empty(ids, dt) for dt in (np.int64, np.int64, np.int64, np.int32, np.int32,
+                                           np.float64)]
+        uh, ih = np.empty(ids, np.int32), np.empty(n, np.int32)
+
+        def host_step():
+            a = L.nfh_group_agg_i32(kh.ctypes.data, vh.ctypes.data, mh.ctypes.data, None, n,
+                                    *[o.ctypes.data for o in oh], ids)
+            b = L.nfh_unique_valid_i32(kh.ctypes.data, kmh.ctypes.data, n, uh.ctypes.data,
+                                       ih.ctypes.data)
+            if a < 0 or b < 0:
+                raise RuntimeError("host-buffer step rc %d / %d" % (a, b))
+        th = _median_time(host_step, 3)
+        extra["step_host_buffers"] = {"ms": round(th * 1e3, 1), "rows_per_s": n / th,
+                                      "note": "PCIe-inclusive: NumPy host inputs and outputs "
+                                              "staged by the library"}
+        del oh, uh, ih, mh, kmh
 
     if "argsort" in legs:
         # stable argsort, 2^28 uniform int32 (pass model 60 B/row)
@@ -505,6 +644,9 @@ def main():
     if "ghash" in legs:
         # partitioned pair-hash (grouped COUNT DISTINCT), 2^26 int64 pairs
         extra["ghash_count_distinct_2^26"] = ghash_leg(torch, L, 1 << 26, max(3, args.steps // 2))
+    if "cpu" in legs and not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        extra["cpu_baselines"] = cpu_baselines_extra(torch, L, native, threads)
     if "uniform" in legs:
         # uniform-2^24 group-by variant
         keys, vals, valid, kvalid = W.groupby_config(n=n, ids=ids, dist="uniform")
@@ -531,7 +673,7 @@ def main():
                                "values, 5%% NULL keys for unique" % args.rows_log2,
                    "rows": n, "ids": ids, "parallelism": "single GPU"},
         "roofline": roofline,
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_full(host),
         "extra": extra,
     }
     print(json.dumps(out))

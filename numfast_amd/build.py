@@ -74,7 +74,7 @@ def build_oracle():
     src = ROOT / "oracle" / "nf_oracle.c"
     out = ROOT / "oracle" / "libnf_oracle.so"
     tmp = out.with_suffix(".so.tmp")
-    _run(["gcc", "-O3", "-march=x86-64-v2", "-fPIC", "-shared", "-fvisibility=hidden",
+    _run(["gcc", "-O3", "-march=x86-64-v3", "-fPIC", "-shared", "-fvisibility=hidden",
           "-o", str(tmp), str(src)])
     os.replace(tmp, out)
     return out

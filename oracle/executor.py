@@ -22,7 +22,7 @@ def build_if_missing():
         return ORACLE_LIB
     import subprocess
     tmp = str(ORACLE_LIB) + ".tmp"
-    subprocess.run(["gcc", "-O3", "-march=x86-64-v2", "-fPIC", "-shared",
+    subprocess.run(["gcc", "-O3", "-march=x86-64-v3", "-fPIC", "-shared",
                     "-fvisibility=hidden", "-o", tmp, str(src)], check=True)
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
