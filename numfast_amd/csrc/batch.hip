@@ -28,7 +28,10 @@
 #include <mutex>
 #include <vector>
 
+#include <memory>
+
 #include "device.hpp"
+#include "groupagg.hpp"
 #include "runtime.hpp"
 #include "sort.hpp"
 
@@ -237,6 +240,19 @@ __global__ void k_bt_next(BtMeta* meta) {
   meta->base = meta->total;
 }
 
+// Mode 2's range check: a group key outside [0, card_q) of its query marks
+// the run bad (the per-query group-by accepts any key).
+__global__ void k_bt_check_keys(const int64_t* __restrict__ group_off,
+                                const int64_t* __restrict__ ukeys,
+                                const BtQuery* __restrict__ qs, uint32_t nq,
+                                BtMeta* __restrict__ meta) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    const int64_t a = group_off[q], b = group_off[q + 1];
+    if (b > a && (ukeys[a] < 0 || ukeys[b - 1] >= int64_t(qs[q].card)))
+      atomicMin(&meta->bad_row, 0u);
+  }
+}
+
 // ------------------------------------------------------------ plan
 struct BatchPlan {
   const int32_t* keys = nullptr;
@@ -262,6 +278,8 @@ struct BatchPlan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<void*> owned;
+  std::vector<int64_t> roff;  // per query (+1): row offsets (host copy)
+  std::vector<int32_t> card;  // per query: key range (host copy)
   ~BatchPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -341,6 +359,39 @@ static void enqueue_per_query(BatchPlan& P) {
   NFH_LAUNCH_CHECK();
 }
 
+// The plain library path: each query through the library's own group-by
+// (nfh_group_agg_i32's device path, one call per query on this thread's
+// stream, each returning its group count), outputs at the running group
+// offset; then the key-range check of the batch contract.
+static int64_t run_library_per_query(BatchPlan& P) {
+  hipStream_t cs = stream();
+  std::vector<int64_t> goff(P.nq + 1, 0);
+  int64_t off = 0;
+  for (uint32_t q = 0; q < P.nq; ++q) {
+    goff[q] = off;
+    const int64_t r0 = P.roff[q], nrow = P.roff[q + 1] - P.roff[q];
+    if (nrow == 0) continue;
+    if (uint64_t(off) + uint64_t(std::min<int64_t>(nrow, P.card[q])) > P.cap) return TOO_SMALL;
+    GroupAggOut o{};
+    o.ukeys = P.ukeys + off;
+    o.counts = P.counts + off;
+    o.sums = P.sums + off;
+    off += group_agg_device(P.keys + r0, P.vals + r0, nullptr, nullptr, size_t(nrow), o, cs);
+  }
+  goff[P.nq] = off;
+  NFH_CHECK(hipMemcpyAsync(P.group_off, goff.data(), goff.size() * 8, hipMemcpyHostToDevice,
+                           cs));
+  hipLaunchKernelGGL(k_bt_reset, dim3(1), dim3(1), 0, cs, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_check_keys, dim3(grid_for(P.nq, 256, 256)), dim3(256), 0, cs,
+                     P.group_off, P.ukeys, P.qs, P.nq, P.meta);
+  NFH_LAUNCH_CHECK();
+  BtMeta m;
+  NFH_CHECK(hipMemcpyAsync(&m, P.meta, sizeof(m), hipMemcpyDeviceToHost, cs));
+  NFH_CHECK(hipStreamSynchronize(cs));
+  return m.bad_row != 0xFFFFFFFFu ? int64_t(BAD_RANGE) : off;
+}
+
 static float time_accumulate(BatchPlan& P) {
   hipEvent_t a, b;
   NFH_CHECK(hipEventCreate(&a));
@@ -400,7 +451,16 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
       if (roff[q + 1] < roff[q] || roff[q + 1] - roff[q] >= int64_t(BT_MAX_ROWS) || cd[q] < 0)
         return fail(MALFORMED);
     if (roff[nq] > int64_t(INT32_MAX)) return fail(MALFORMED);
-    auto* P = new BatchPlan();
+    // the dense slots (8 B per key of every query) and the overflow list
+    // (16 B per row) must fit the free device memory
+    uint64_t total_card = 0;
+    for (size_t q = 0; q < nq; ++q) total_card += uint64_t(cd[q]);
+    size_t free_b = 0, total_b = 0;
+    NFH_CHECK(hipMemGetInfo(&free_b, &total_b));
+    if (total_card * 8 + uint64_t(roff[nq]) * 16 > uint64_t(free_b)) return fail(MALFORMED);
+    std::unique_ptr<BatchPlan> P(new BatchPlan());
+    P->roff = roff;
+    P->card = cd;
     P->keys = keys;
     P->vals = vals;
     P->ukeys = ukeys;
@@ -458,13 +518,19 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     }
     build_items(*P, roff, bestR);
     P->grid = cus * bestG;
-    // capture the batched run
+    // capture the batched run (an error inside ends the capture first)
     NFH_CHECK(hipStreamBeginCapture(P->s, hipStreamCaptureModeThreadLocal));
-    enqueue_batched(*P);
+    try {
+      enqueue_batched(*P);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(P->s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      throw;
+    }
     NFH_CHECK(hipStreamEndCapture(P->s, &P->graph));
     NFH_CHECK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
     if (rc_out) *rc_out = OK;
-    return P;
+    return P.release();
   } catch (...) {
     return fail(DEVICE_ERROR);
   }
@@ -480,13 +546,20 @@ static int64_t batch_result(BatchPlan* P) {
 }
 
 // Run a plan: mode 0 replays the captured graph (one persistent launch of
-// the batch), mode 1 launches the kernels query by query (the plain form).
-// Returns the number of groups; -2 when a key is outside its query's range
-// (outputs undefined), -7 when cap is too small.  Synchronous.
+// the batch), mode 1 launches the same kernels query by query, mode 2
+// runs each query through the library's plain group-by (one call per
+// query, as a caller without the batch would).  Returns the number of
+// groups; -2 when a key is outside its query's range (outputs undefined),
+// -3 for an unknown mode, -7 when cap is too small.  Synchronous.
 NF_EXPORT int64_t nfh_group_batch_run(void* plan, int32_t mode) {
   if (!plan) return NULL_OR_ABORT;
+  if (mode < 0 || mode > 2) return MALFORMED;
   auto* P = static_cast<BatchPlan*>(plan);
   try {
+    if (mode == 2) {
+      NFH_CHECK(hipStreamSynchronize(P->s));
+      return run_library_per_query(*P);
+    }
     if (mode == 0)
       NFH_CHECK(hipGraphLaunch(P->exec, P->s));
     else

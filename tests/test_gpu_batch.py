@@ -38,6 +38,7 @@ def test_batch_parity_both_modes(hip, nq, total, card_hi):
     b, kt, vt = _plan(torch, k, v, row_off, card)
     _check(b.run(0), ref)
     _check(b.run(1), ref)
+    _check(b.run(2), ref)  # each query through the library's plain group-by
     b.close()
 
 
@@ -89,17 +90,19 @@ def test_batch_empty_queries_and_codes(hip):
     b, kt, vt = _plan(torch, k, v, row_off, card)
     _check(b.run(0), ref)
     _check(b.run(1), ref)
+    _check(b.run(2), ref)
     b.close()
     # a key outside its query's range -> -2
     kb = k.copy()
     kb[3] = 2  # query 2 has card 2
     assert ox.group_batch(kb, v, row_off, card) == -2
     b, _, _ = _plan(torch, kb, v, row_off, card)
-    assert b.run(0)[0] == -2 and b.run(1)[0] == -2
+    assert b.run(0)[0] == -2 and b.run(1)[0] == -2 and b.run(2)[0] == -2
     b.close()
     # capacity below the group count -> -7
     b, _, _ = _plan(torch, k, v, row_off, card, cap=2)
-    assert b.run(0)[0] == -7
+    assert b.run(0)[0] == -7 and b.run(2)[0] == -7
+    assert native.lib("batch").nfh_group_batch_run(b.plan, 3) == -3
     b.close()
 
 
