@@ -80,12 +80,25 @@ def build_oracle():
     return out
 
 
+def build_asan(jobs=8):
+    """The host-ASan build of the sources and the boundary driver
+    (tests/asan/Makefile: host code instrumented, device code as usual)."""
+    r = subprocess.run(["make", "-C", str(ROOT / "tests" / "asan"), "-j%d" % jobs],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout[-4000:] + r.stderr[-4000:])
+        raise RuntimeError("ASan build failed")
+    return ROOT / "tests" / "asan" / "boundary_asan"
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     lib = build_lib(verbose="-v" in argv)
     print("built", lib)
     if "--oracle" in argv:
         print("built", build_oracle())
+    if "--asan" in argv:
+        print("built", build_asan())
 
 
 if __name__ == "__main__":
