@@ -408,6 +408,14 @@ __global__ void k_gh_winstart(const int32_t* __restrict__ skey, uint64_t n, uint
   wstart[w] = uint32_t(lo);
 }
 
+// LDS address of window slot i.  A partition's reference homes are
+// congruent modulo P (a multiple of the 64 LDS banks), so unswizzled every
+// claim of a window starts in one bank; XOR-ing the low 6 bits with bits
+// 8..13 spreads homes 256 slots apart over the banks.  It permutes each
+// aligned 64-slot block, so [0, span) maps onto itself (span is a multiple
+// of 64) and the probe order, which follows slot numbers, is unchanged.
+__device__ __forceinline__ uint32_t gh_swz(uint32_t i) { return i ^ ((i >> 8) & 63u); }
+
 // Solve window w in LDS: its rows perm[wstart[w] .. wstart[w+1]) from
 // their homes, plus `nin` spilled ids from slot 0.  Writes the window's
 // slots to `hold` and its extension's ids (slot order) to spill[w];
@@ -458,7 +466,7 @@ __device__ void gh_solve_window(uint32_t w, const WinGeom& g, const uint32_t* s_
         over = true;
         break;
       }
-      const uint32_t old = atomicMin(&s_hold[c], carried);
+      const uint32_t old = atomicMin(&s_hold[gh_swz(c)], carried);
       if (old == GH_EMPTY) break;
       if (old > carried) carried = old;
       ++c;
@@ -467,7 +475,7 @@ __device__ void gh_solve_window(uint32_t w, const WinGeom& g, const uint32_t* s_
   if (over) atomicOr(err, 32u);
   __syncthreads();
   uint32_t* tab = hold + g.toff[q];
-  for (uint32_t i = threadIdx.x; i < wsz; i += blockDim.x) tab[(a + i) & mask] = s_hold[i];
+  for (uint32_t i = threadIdx.x; i < wsz; i += blockDim.x) tab[(a + i) & mask] = s_hold[gh_swz(i)];
   if (ext == 0) {
     if (threadIdx.x == 0) {
       if (nspill[w] != 0) s_misc[1] = 1;
@@ -484,7 +492,7 @@ __device__ void gh_solve_window(uint32_t w, const WinGeom& g, const uint32_t* s_
     uint32_t base = 0;
     bool diff = false;
     for (uint32_t s0 = 0; s0 < ext; s0 += WAVE) {
-      const uint32_t v = s_hold[wsz + s0 + threadIdx.x];
+      const uint32_t v = s_hold[gh_swz(wsz + s0 + threadIdx.x)];
       const bool occ = v != GH_EMPTY;
       const uint64_t bal = __ballot(occ);
       if (occ) {
