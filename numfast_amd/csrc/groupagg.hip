@@ -110,37 +110,45 @@ static bool vec_ok(const int32_t* k, const int32_t* v, const uint8_t* m1, const 
          (m2 == nullptr || aligned(m2, 4));
 }
 
-// min / max of the valid keys.
+// min / max of the valid keys.  Each workgroup reads one contiguous range
+// of quads, sixteen in flight per thread, with streaming loads; then one
+// min / max per workgroup reaches global memory, each atomic issued only
+// when it would move the value.  (At 2^28 rows: a grid-stride form with an
+// atomic pair per wave took 0.30 ms -- 8192 waves' atomics on two words --
+// and 0.44 ms at twice the grid; this form 0.24 ms, 5.6 TB/s.)
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_key_range(const int32_t* __restrict__ keys,
                                                    const uint8_t* __restrict__ v,
                                                    const uint8_t* __restrict__ kv, uint32_t n,
                                                    int* __restrict__ out /* min, max */) {
+  __shared__ int s_mn[256 / WAVE], s_mx[256 / WAVE];
   int mn = INT32_MAX, mx = INT32_MIN;
-  // key-only pass: eight quads in flight per thread, streaming loads
-  // (0.33 -> 0.31 ms at 2^28 rows)
   auto f = [&](int32_t k, bool ok) {
     if (ok) {
       mn = k < mn ? k : mn;
       mx = k > mx ? k : mx;
     }
   };
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t tail = 0;
   if constexpr (VEC) {
-    constexpr int R = 8;
+    constexpr int R = 16;
     const uint32_t nq = n / 4;
-    uint32_t q = tid;
-    for (; q + (R - 1) * stride < nq; q += R * stride) {
+    const uint32_t per = (nq + gridDim.x - 1) / gridDim.x;
+    const uint32_t q0 = blockIdx.x * per;
+    const uint32_t q1 = q0 + per < nq ? q0 + per : nq;
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += R * blockDim.x) {
       int4 k4[R];
       uint32_t a[R], b[R];
 #pragma unroll
       for (int u = 0; u < R; ++u) {
-        const uint32_t qq = q + uint32_t(u) * stride;
-        k4[u] = ld_stream4(keys, qq);
-        a[u] = v ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(v) + qq) : 0x01010101u;
-        b[u] = kv ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(kv) + qq) : 0x01010101u;
+        const uint32_t qq = q + uint32_t(u) * blockDim.x;
+        const uint32_t qc = qq < q1 ? qq : q0;  // in range; masked out below
+        k4[u] = ld_stream4(keys, qc);
+        a[u] = v ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(v) + qc)
+                 : 0x01010101u;
+        b[u] = kv ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(kv) + qc)
+                  : 0x01010101u;
+        if (qq >= q1) a[u] = 0u;
       }
 #pragma unroll
       for (int u = 0; u < R; ++u) {
@@ -150,17 +158,10 @@ __global__ __launch_bounds__(256) void k_key_range(const int32_t* __restrict__ k
         f(k4[u].w, (a[u] & 0xFF000000u) && (b[u] & 0xFF000000u));
       }
     }
-    for (; q < nq; q += stride) {
-      const int4 k4 = reinterpret_cast<const int4*>(keys)[q];
-      const uint32_t a = v ? reinterpret_cast<const uint32_t*>(v)[q] : 0x01010101u;
-      const uint32_t b = kv ? reinterpret_cast<const uint32_t*>(kv)[q] : 0x01010101u;
-      f(k4.x, (a & 0xFFu) && (b & 0xFFu));
-      f(k4.y, (a & 0xFF00u) && (b & 0xFF00u));
-      f(k4.z, (a & 0xFF0000u) && (b & 0xFF0000u));
-      f(k4.w, (a & 0xFF000000u) && (b & 0xFF000000u));
-    }
     tail = nq * 4;
   }
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = tail + tid; i < n; i += stride)
     f(keys[i], (v == nullptr || v[i] != 0) && (kv == nullptr || kv[i] != 0));
   for (int off = 32; off > 0; off >>= 1) {
@@ -169,8 +170,22 @@ __global__ __launch_bounds__(256) void k_key_range(const int32_t* __restrict__ k
     mx = b > mx ? b : mx;
   }
   if (lane_id() == 0) {
-    if (mn != INT32_MAX) atomicMin(&out[0], mn);
-    if (mx != INT32_MIN) atomicMax(&out[1], mx);
+    s_mn[wave_id()] = mn;
+    s_mx[wave_id()] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < int(blockDim.x / WAVE); ++w) {
+      mn = s_mn[w] < mn ? s_mn[w] : mn;
+      mx = s_mx[w] > mx ? s_mx[w] : mx;
+    }
+    // both only move one way: a stale read costs a redundant atomic only
+    if (mn != INT32_MAX && mn < __hip_atomic_load(&out[0], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(&out[0], mn);
+    if (mx != INT32_MIN && mx > __hip_atomic_load(&out[1], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&out[1], mx);
   }
 }
 
@@ -496,7 +511,7 @@ bool key_range(const int32_t* keys, const uint8_t* v, const uint8_t* kv, size_t 
   fill_u32(r + 1, uint32_t(INT32_MIN), 1, s);
   {
     ProfScope prof("key_range", s, double(n) * (4.0 + (v ? 1.0 : 0.0) + (kv ? 1.0 : 0.0)));
-    const unsigned grid = grid_for(n / 4 + 1, 256, 8 * num_cus());
+    const unsigned grid = grid_for(n / 64 + 1, 256, 4 * num_cus());
     if (vec_ok(keys, nullptr, v, kv))
       hipLaunchKernelGGL(k_key_range<true>, dim3(grid), dim3(256), 0, s, keys, v, kv,
                          uint32_t(n), r);
