@@ -187,15 +187,22 @@ def _unique_device_offset(keys, valid):
 
 @pytest.mark.parametrize("dist,masked,vec", [("uniform", False, True), ("uniform", True, True),
                                               ("zipf", True, True), ("edges", True, True),
-                                              ("uniform", True, False), ("zipf", False, False)])
+                                              ("uniform", True, False), ("zipf", False, False),
+                                              ("span25", True, True)])
 def test_unique_valid_large_span(dist, masked, vec):
     """Dense-span unique with flags larger than an XCD's L2 share (spans
     over 2^21 ids at 2^22+ rows): rows not a multiple of 4, keys at both
-    ends of the span, hot keys, NULL rows, and the row-at-a-time path."""
+    ends of the span, hot keys, NULL rows, and the row-at-a-time path.
+    From 2^22 rows the aligned path marks presence by key range (2^20
+    offsets per range): spans of 2^24 and 2^25 give 16 and 32 ranges,
+    "edges" a third range holding one offset."""
     rng = np.random.default_rng(21)
     n = (1 << 22) + 7
     if dist == "uniform":
         keys = rng.integers(-(1 << 23), 1 << 23, n).astype(np.int32)
+    elif dist == "span25":
+        keys = rng.integers(-(1 << 24), 1 << 24, n).astype(np.int32)
+        keys[:2] = [-(1 << 24), (1 << 24) - 1]
     elif dist == "zipf":
         keys = zipf_keys(rng, n, 1 << 24, 1.1) - 12345
     else:  # span exactly 2^21 + 1, both ends present
