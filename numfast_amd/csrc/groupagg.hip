@@ -677,12 +677,15 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
 // ---------------------------------------------------------------- unique
 // Dense unique.  k_mark sets one flag byte per present key offset
 // o = key - kmin with plain byte stores (set-once: most rows only read, and
-// a stale read only repeats a store of 1).  The scan packs the flags into
-// one 64-bit word per 32 offsets: presence bits low, and high the number of
-// present offsets in the words below, so the dense code of a present key is
-// hi(w) + popc(lo(w) & below(o)) -- one random 8-byte lookup per row over
-// span / 4 bytes (4 MiB at 2^24 ids).  Keys of the sampled hot-key table
-// are resolved in LDS in both row passes; they stay off the global tables.
+// a stale read only repeats a store of 1).  The scan packs the presence
+// bits into one 16-byte rank entry per 96 offsets: the number of present
+// offsets below the entry, then the entry's 96 presence bits, so the dense
+// code of a present key is base + popc(bits below o) -- one random 16-byte
+// lookup per row over span / 6 bytes (2.7 MiB at 2^24 ids, which stays in
+// an XCD's 4 MiB L2; an 8-byte word per 32 offsets made 4 MiB and took
+// 0.90 ms for the inverse at 2^28 rows against 0.75).  Keys of the sampled
+// hot-key table are resolved in LDS in both row passes; they stay off the
+// global tables.
 
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mark(const int32_t* __restrict__ keys,
@@ -777,25 +780,45 @@ __device__ __forceinline__ uint32_t flag_word(const uint8_t* __restrict__ flag, 
   return bits;
 }
 
-// Exclusive scan of the per-word flag counts; each word writes its packed
-// (bits, base) word and its present keys (ascending) to uniq.
-struct BitRankOp {
-  const uint8_t* flag;
-  u64* words;
+// Rank entries: {present offsets below, bits of offsets 96e .. 96e+31,
+// 96e+32 .. 96e+63, 96e+64 .. 96e+95}.
+typedef uint32_t rank_entry __attribute__((ext_vector_type(4)));
+constexpr uint32_t RANK_SPAN = 96;
+
+// Exclusive scan of the per-word presence counts (one item per 32 offsets,
+// so a scan tile covers 2^17 offsets); word w stores its bits into its
+// rank entry, the entry's base when it is the entry's first word, and its
+// present keys (ascending) to uniq.  Bits(w) gives the 32 presence bits of
+// word w.  Components of the last entry past the last word are zeroed.
+template <class Bits>
+struct RankOp {
+  Bits bits;
+  uint32_t* ent;  // rank entries as 4 words each
+  uint32_t nw;
   int32_t kmin;
   int32_t* uniq;
   int64_t* ng;
-  __device__ uint32_t value(uint32_t w) const { return uint32_t(__popc(flag_word(flag, w))); }
+  __device__ uint32_t value(uint32_t w) const { return uint32_t(__popc(bits(w))); }
   __device__ void emit(uint32_t w, uint32_t ex) const {
-    uint32_t b = flag_word(flag, w);
-    words[w] = (u64(ex) << 32) | b;
-    const int32_t k0 = kmin + int32_t(w << 5);
+    uint32_t b = bits(w);
+    const uint32_t e = w / 3, j = w - 3 * e;
+    ent[4 * e + 1 + j] = b;
+    if (j == 0) ent[4 * e] = ex;
+    if (w == nw - 1)
+      for (uint32_t k = j + 1; k < 3; ++k) ent[4 * e + 1 + k] = 0u;
+    const uint32_t k0 = uint32_t(kmin) + (w << 5);  // wraps only for absent keys
     while (b) {
-      uniq[ex++] = k0 + (__ffs(b) - 1);
+      uniq[ex++] = int32_t(k0 + uint32_t(__ffs(b) - 1));
       b &= b - 1u;
     }
   }
   __device__ void total(uint32_t t) const { *ng = int64_t(t); }
+};
+
+// The 32 flag bytes of word w (each 0 or 1) as a bit mask.
+struct FlagBits {
+  const uint8_t* flag;
+  __device__ uint32_t operator()(uint32_t w) const { return flag_word(flag, w); }
 };
 
 // Partitioned mark (inputs of 2^22 rows and more).  k_mark's random flag
@@ -967,48 +990,45 @@ __global__ __launch_bounds__(256) void k_umark_merge(const uint32_t* __restrict_
   }
 }
 
-// BitRankOp over a presence bitmap (one 32-bit word per 32 offsets).
-struct BitmapRankOp {
+// Words of a presence bitmap (one 32-bit word per 32 offsets).
+struct BitmapBits {
   const uint32_t* bits;
-  u64* words;
-  int32_t kmin;
-  int32_t* uniq;
-  int64_t* ng;
-  __device__ uint32_t value(uint32_t w) const { return uint32_t(__popc(bits[w])); }
-  __device__ void emit(uint32_t w, uint32_t ex) const {
-    uint32_t b = bits[w];
-    words[w] = (u64(ex) << 32) | b;
-    const int32_t k0 = kmin + int32_t(w << 5);
-    while (b) {
-      uniq[ex++] = k0 + (__ffs(b) - 1);
-      b &= b - 1u;
-    }
-  }
-  __device__ void total(uint32_t t) const { *ng = int64_t(t); }
+  __device__ uint32_t operator()(uint32_t w) const { return bits[w]; }
 };
 
-__device__ __forceinline__ int32_t bit_code(const u64* __restrict__ words, uint32_t o) {
-  const u64 w = words[o >> 5];
-  return int32_t(uint32_t(w >> 32) + uint32_t(__popc(uint32_t(w) & ((1u << (o & 31u)) - 1u))));
+// Entry of offset o (o / 96 as (o / 32) / 3 by multiply-high) and o's bit
+// position within it.
+__device__ __forceinline__ uint32_t rank_entry_of(uint32_t o) {
+  return __umulhi(o >> 5, 0xAAAAAAABu) >> 1;
+}
+__device__ __forceinline__ int32_t rank_in_entry(rank_entry en, uint32_t b) {
+  const uint32_t m0 = b >= 32 ? 0xFFFFFFFFu : ((1u << b) - 1u);
+  const uint32_t m1 = b >= 64 ? 0xFFFFFFFFu : (b <= 32 ? 0u : ((1u << (b - 32)) - 1u));
+  const uint32_t m2 = b <= 64 ? 0u : ((1u << (b - 64)) - 1u);
+  return int32_t(en.x + uint32_t(__popc(en.y & m0) + __popc(en.z & m1) + __popc(en.w & m2)));
+}
+__device__ __forceinline__ int32_t rank_code(const rank_entry* __restrict__ ent, uint32_t o) {
+  const uint32_t e = rank_entry_of(o);
+  return rank_in_entry(ent[e], o - e * RANK_SPAN);
 }
 
 // Dense codes of the hot keys (read by k_inv from LDS).  A hot key with no
 // valid row is never looked up; it may even lie outside [kmin, kmin + span)
-// (the sample ignores validity), so only in-range keys read the words.
+// (the sample ignores validity), so only in-range keys read the rank entries.
 __global__ void k_hot_codes(const int32_t* __restrict__ hot_keys,
                             const uint32_t* __restrict__ hot_n, int32_t kmin, uint32_t span,
-                            const u64* __restrict__ words, int32_t* __restrict__ codes) {
+                            const rank_entry* __restrict__ ent, int32_t* __restrict__ codes) {
   const uint32_t nh = *hot_n;
   for (uint32_t b = threadIdx.x; b < nh; b += blockDim.x) {
     const uint32_t o = uint32_t(hot_keys[b]) - uint32_t(kmin);
-    codes[b] = o < span ? bit_code(words, o) : -1;
+    codes[b] = o < span ? rank_code(ent, o) : -1;
   }
 }
 
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_inv(const int32_t* __restrict__ keys,
                                              const uint8_t* __restrict__ v, uint32_t n,
-                                             int32_t kmin, const u64* __restrict__ words,
+                                             int32_t kmin, const rank_entry* __restrict__ ent,
                                              const int2* __restrict__ table,
                                              const uint32_t* __restrict__ hot_n,
                                              const int32_t* __restrict__ hot_code,
@@ -1026,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t* __restrict__ keys,
       const int32_t idx = hot_index(t_key, t_idx, k);
       if (idx >= 0) return s_code[idx];
     }
-    return bit_code(words, uint32_t(k - kmin));
+    return rank_code(ent, uint32_t(k - kmin));
   };
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -1061,14 +1081,17 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t* __restrict__ keys,
           off[4 * u + e] = (ok && idx < 0) ? uint32_t(kk[e] - kmin) : 0u;
         }
       }
-      u64 wd[4 * RQ];
+      rank_entry en[4 * RQ];
+      uint32_t bi[4 * RQ];
 #pragma unroll
-      for (int j = 0; j < 4 * RQ; ++j) wd[j] = words[off[j] >> 5];
+      for (int j = 0; j < 4 * RQ; ++j) {
+        const uint32_t e = rank_entry_of(off[j]);
+        bi[j] = off[j] - e * RANK_SPAN;
+        en[j] = ent[e];
+      }
 #pragma unroll
       for (int j = 0; j < 4 * RQ; ++j)
-        if (c[j] == INT32_MIN)
-          c[j] = int32_t(uint32_t(wd[j] >> 32) +
-                         uint32_t(__popc(uint32_t(wd[j]) & ((1u << (off[j] & 31u)) - 1u))));
+        if (c[j] == INT32_MIN) c[j] = rank_in_entry(en[j], bi[j]);
       int4 o[RQ];
 #pragma unroll
       for (int u = 0; u < RQ; ++u) o[u] = make_int4(c[4 * u], c[4 * u + 1], c[4 * u + 2], c[4 * u + 3]);
@@ -1125,7 +1148,8 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
   int64_t* ng_d = scratch_t<int64_t>(S_WORK2, 1);
   if (span <= dense_span_limit(n)) {
     const size_t nw = size_t((span + 31) >> 5);
-    u64* words = scratch_t<u64>(S_WORK1, nw);
+    const size_t ne = (nw + 2) / 3;
+    rank_entry* ent = scratch_t<rank_entry>(S_WORK1, ne);
     int32_t* hot_code = scratch_t<int32_t>(S_T0, HOT_MAX);
     int2* table = nullptr;
     int32_t* hot_keys = nullptr;
@@ -1164,7 +1188,9 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
       hipLaunchKernelGGL(k_umark_merge, dim3(grid_for(nw, 256, 4 * num_cus())), dim3(256), 0, s,
                          staging, G, wr, uint32_t(nw), bits);
       NFH_LAUNCH_CHECK();
-      scan_tiles(BitmapRankOp{bits, words, kmin, uniq, ng_d}, nw, s);
+      scan_tiles(RankOp<BitmapBits>{BitmapBits{bits}, reinterpret_cast<uint32_t*>(ent),
+                                    uint32_t(nw), kmin, uniq, ng_d},
+                 nw, s);
     } else {
       uint8_t* flag = scratch_t<uint8_t>(S_WORK0, 32 * nw);
       fill_u32(flag, 0, 8 * nw, s);
@@ -1178,21 +1204,24 @@ int64_t unique_valid_device(const int32_t* keys, const uint8_t* v, size_t n, int
                              kmin, flag, table, hot_keys, hot_n);
         NFH_LAUNCH_CHECK();
       }
-      scan_tiles(BitRankOp{flag, words, kmin, uniq, ng_d}, nw, s);
+      scan_tiles(RankOp<FlagBits>{FlagBits{flag}, reinterpret_cast<uint32_t*>(ent), uint32_t(nw),
+                                  kmin, uniq, ng_d},
+                 nw, s);
     }
     if (table != nullptr) {
       hipLaunchKernelGGL(k_hot_codes, dim3(1), dim3(HOT_MAX), 0, s, hot_keys, hot_n, kmin,
-                         uint32_t(span), words, hot_code);
+                         uint32_t(span), ent, hot_code);
       NFH_LAUNCH_CHECK();
     }
     {
       ProfScope prof("unique_inv_dense", s, double(n) * (8.0 + (v ? 1.0 : 0.0)));
+      const unsigned grid_inv = grid_for(n / 4 + 1, 256, 16 * num_cus());
       if (vec)
-        hipLaunchKernelGGL(k_inv<true>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
-                           words, table, hot_n, hot_code, inv);
+        hipLaunchKernelGGL(k_inv<true>, dim3(grid_inv), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
+                           ent, table, hot_n, hot_code, inv);
       else
-        hipLaunchKernelGGL(k_inv<false>, dim3(grid), dim3(256), 0, s, keys, v, uint32_t(n), kmin,
-                           words, table, hot_n, hot_code, inv);
+        hipLaunchKernelGGL(k_inv<false>, dim3(grid_inv), dim3(256), 0, s, keys, v, uint32_t(n),
+                           kmin, ent, table, hot_n, hot_code, inv);
       NFH_LAUNCH_CHECK();
     }
     return read_scalar(ng_d, s);
