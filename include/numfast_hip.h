@@ -297,12 +297,13 @@ int32_t nfh_selector_predict_cost(const float *params, const float *X, size_t n,
 
 /* Batched group-by (no reference counterpart): nq independent dense
  * sum / count queries over concatenated device rows; query q owns rows
- * [row_off[q], row_off[q+1]) (< 2^24 rows) and keys [0, card[q]).  Output
- * (device, cap lanes): present groups query by query, keys ascending, as
- * ukeys / counts / sums (int64), and group_off[nq + 1].  The plan autotunes
- * rows per work item and the persistent grid on these buffers and captures
- * the batched run in a hipGraph; it keeps the pointers (the buffers must
- * outlive it).  NULL on failure with *rc_out = -1, -3 or -6. */
+ * [row_off[q], row_off[q+1]) (< 2^24 rows) and keys [0, card[q])
+ * (card <= 2^24).  Output (device, cap lanes): present groups query by
+ * query, keys ascending, as ukeys / counts / sums (int64), and
+ * group_off[nq + 1].  The plan autotunes rows per work item and the
+ * persistent grid of the small-query pass (card <= 4096) on these buffers
+ * and captures the batched run in a hipGraph; it keeps the pointers (the
+ * buffers must outlive it).  NULL on failure with *rc_out = -1, -3 or -6. */
 void *nfh_group_batch_plan(const int32_t *keys, const int32_t *vals, const int64_t *row_off,
                            const int32_t *card, size_t nq, int64_t *ukeys, int64_t *counts,
                            int64_t *sums, int64_t *group_off, size_t cap, int32_t *rc_out);
@@ -310,8 +311,11 @@ void *nfh_group_batch_plan(const int32_t *keys, const int32_t *vals, const int64
  * mode 1: the plain form, kernels launched query by query.  Returns the
  * group count, -2 (a key outside its query's range) or -7 (cap too small). */
 int64_t nfh_group_batch_run(void *plan, int32_t mode);
-/* rows per work item, workgroups, work items, key slots. */
+/* small-query pass: rows per work item, workgroups, work items, key slots. */
 int32_t nfh_group_batch_info(void *plan, uint64_t *out4);
+/* wide queries (card > 4096), cut into 16384-key units: partition tiles,
+ * units, (tile, unit) record entries. */
+int32_t nfh_group_batch_info2(void *plan, uint64_t *out3);
 void nfh_group_batch_free(void *plan);
 
 #ifdef __cplusplus

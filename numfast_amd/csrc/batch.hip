@@ -1,6 +1,6 @@
 // batch.hip -- batched group-by: many independent (keys, values) queries,
 // each a dense sum + count over its own key range [0, card_q), in one
-// persistent launch that a plan captures into a hipGraph.
+// persistent launch set that a plan captures into a hipGraph.
 //
 // Per query the semantics are those of the dense group-by lane
 // (numfast-native/src/groupby/dense.rs:18-37, lib.rs:85-103: sums and
@@ -9,26 +9,37 @@
 // (core/compact.rs:16-37): the present groups, ascending, as (ukey, count,
 // sum), with per-query group offsets.
 //
-// Device layout:
-//   slots: one packed u64 per (query, key): bits [40, 64) the row count,
-//     [0, 40) the sum of (v + 2^15).  A row adds (1 << 40) + (v + 2^15) --
-//     one atomic for count and sum, exact while a query has fewer than 2^24
-//     rows (checked when the plan is built).  A value outside
-//     [-2^15, 2^15) adds (1 << 40) + 2^15 (as if 0) and goes to an overflow
-//     list (slot, value) that is added to the compacted sums at the end.
-//   work items (query, row range) pulled from an atomic counter by
-//     persistent workgroups; queries with card <= BT_LDS_SLOTS aggregate in
-//     LDS and flush once per item, larger ones add to their slots directly.
-//   compaction: one stable scan over per-query slot chunks (decoupled
-//     look-back); chunk 0 of a query writes its group offset.
-// The plain form launches the same kernels per query (one fill, one
-// accumulate, one compaction per query), chaining the group offset on the
-// device.
+// Aggregates are packed: one u64 per key, bits [40, 64) the row count,
+// [0, 40) the sum of (v + 2^15).  A row adds (1 << 40) + (v + 2^15) -- one
+// add for count and sum, exact while a query has fewer than 2^24 rows
+// (checked when the plan is built).  A value outside [-2^15, 2^15) adds
+// (1 << 40) + 2^15 (as if 0) and goes to an overflow list (query, key,
+// value) whose values join the compacted sums at the end.
+//
+// Two classes of query:
+//   small (card <= BT_LDS_SLOTS): work items (query, row range) pulled by
+//     persistent workgroups aggregate in LDS and flush into the query's
+//     slots (one packed u64 per key, zeroed per run) -- the rows-per-item
+//     and grid of this pass are autotuned;
+//   ranged (card > BT_LDS_SLOTS): the key range is cut into units of
+//     BT_RANGE keys.  A partition pass turns each 8192-row tile into
+//     32-bit records (key within its unit : biased value) grouped by unit
+//     (tile-local LDS ranks), with one (start, count) entry per (tile,
+//     unit); a unit's records are then all in those entries.  No slot array
+//     and no device-scope atomics: the card-sized dense slots of the first
+//     form (3.6e8 slots, zero-filled and scanned for 2.7e8 rows, with a
+//     global atomic per row of every wide query) took 13.7 ms.
+// Output: every query has at least one unit (a small query one, a ranged
+// query ceil(card / BT_RANGE)); a count pass gives each unit's present
+// groups (small: present slots; ranged: an LDS presence bitmap over its
+// records), one scan gives unit and query offsets, and an emit pass writes
+// each unit's groups in key order (ranged units aggregate their records in
+// LDS first).  The plain form (mode 1) launches the same kernels query by
+// query, chaining the group offset on the device.
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
-
-#include <memory>
 
 #include "device.hpp"
 #include "groupagg.hpp"
@@ -40,54 +51,69 @@ namespace nfh {
 typedef unsigned long long u64;
 
 constexpr int BT_BLOCK = 256;
-constexpr int BT_LDS_SLOTS = 4096;  // queries up to this card aggregate in LDS
-constexpr int BT_CHUNK = 4096;      // slots per compaction chunk
-constexpr int BT_CIPT = BT_CHUNK / BT_BLOCK;
+constexpr int BT_LDS_SLOTS = 4096;  // small queries: up to this card
 constexpr int BT_SHIFT = 40;
 constexpr u64 BT_MASK = (u64(1) << BT_SHIFT) - 1;
-constexpr uint32_t BT_MAX_ROWS = 1u << 24;  // rows per query for the packed slots
+constexpr uint32_t BT_MAX_ROWS = 1u << 24;  // rows per query for the packed aggregates
+constexpr int BT_RANGE_BITS = 14;
+constexpr uint32_t BT_RANGE = 1u << BT_RANGE_BITS;  // keys per ranged unit (128 KB of LDS)
+constexpr uint32_t BT_MAX_RANGES = 1024;            // card <= 2^24
+constexpr int BT_PBLOCK = 512;                      // partition pass
+constexpr int BT_PIPT = 16;
+constexpr uint32_t BT_PTILE = uint32_t(BT_PBLOCK) * BT_PIPT;  // 8192 rows
+constexpr int BT_CBLOCK = 256;   // count pass
+constexpr int BT_EBLOCK = 1024;  // emit pass
+constexpr int BT_SCAN = 1024;    // unit scan (one workgroup)
 
-struct BtItem {
+struct BtItem {  // accumulate work item of a small query
   uint32_t q;
   uint32_t r0, r1;  // row range (global row indices)
 };
-struct BtChunk {
+struct BtTile {  // partition tile of a ranged query
   uint32_t q;
-  uint32_t pad;
-  u64 s0, s1;  // slot range (global slot indices)
+  uint32_t r0, r1;  // row range
+  uint32_t e0;      // entry of (this tile, unit 0); units follow
 };
 struct BtQuery {
-  u64 slot0;     // first slot of the query
+  u64 slot0;      // small: first slot of the query
   uint32_t card;
-  uint32_t pad;
+  uint32_t nr;    // ranged: units of the query; 0: small
+};
+struct BtUnit {  // keys [key0, key0 + nkeys) of query q, in output order
+  uint32_t q, j;
+  uint32_t key0, nkeys;
+  uint32_t t0, t1;  // ranged: the query's tiles
+  uint32_t e0, nr;  // ranged: entry of (tile t0, unit 0); units per tile
 };
 struct BtOver {  // a row whose value is outside [-2^15, 2^15)
-  u64 slot;
+  uint32_t q, key;
   int32_t v;
   uint32_t pad;
 };
 // device counters
 struct BtMeta {
-  uint32_t work;      // next work item
-  uint32_t chunk;     // next compaction chunk
-  uint32_t nover;     // overflow entries
-  uint32_t bad_row;   // first row with a key out of range (0xFFFFFFFF: none)
-  u64 base;           // group offset before the first chunk (0 when batched)
-  u64 total;          // groups written
+  uint32_t work;     // next work item
+  uint32_t pad;
+  uint32_t nover;    // overflow entries
+  uint32_t bad_row;  // first row with a key out of range (0xFFFFFFFF: none)
+  u64 base;          // group offset before the first unit (0 when batched)
+  u64 total;         // groups written
 };
 
-__device__ __forceinline__ u64 bt_packed(int32_t v, bool* over) {
+__device__ __forceinline__ uint32_t bt_biased(int32_t v, bool* over) {
   const uint32_t b = uint32_t(v) + 32768u;
   *over = b >= 65536u;
-  return (u64(1) << BT_SHIFT) + (*over ? u64(32768) : u64(b));
+  return *over ? 32768u : b;
 }
 
-// Persistent accumulation over work items [item0, item1).
+// Small queries: persistent accumulation over work items [item0, item1),
+// four rows in flight per thread, LDS aggregation, one flush per item.
 __global__ __launch_bounds__(BT_BLOCK) void k_bt_accumulate(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const BtItem* __restrict__ items, uint32_t item0, uint32_t item1,
     const BtQuery* __restrict__ qs, u64* __restrict__ slots, BtOver* __restrict__ over,
     BtMeta* __restrict__ meta) {
+  constexpr int U = 4;
   __shared__ u64 s_acc[BT_LDS_SLOTS];
   __shared__ uint32_t s_item;
   for (;;) {
@@ -99,127 +125,281 @@ __global__ __launch_bounds__(BT_BLOCK) void k_bt_accumulate(
     const BtItem w = items[it];
     const BtQuery qd = qs[w.q];
     const uint32_t card = qd.card;
+    for (uint32_t s = threadIdx.x; s < card; s += BT_BLOCK) s_acc[s] = 0;
+    __syncthreads();
+    for (uint32_t r = w.r0 + threadIdx.x; r < w.r1; r += U * BT_BLOCK) {
+      int32_t k[U], x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t rr = r + uint32_t(u) * BT_BLOCK;
+        k[u] = rr < w.r1 ? keys[rr] : 0;
+        x[u] = rr < w.r1 ? vals[rr] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t rr = r + uint32_t(u) * BT_BLOCK;
+        if (rr >= w.r1) continue;
+        if (uint32_t(k[u]) >= card) {
+          atomicMin(&meta->bad_row, rr);
+          continue;
+        }
+        bool ov;
+        const uint32_t b = bt_biased(x[u], &ov);
+        atomicAdd(&s_acc[k[u]], (u64(1) << BT_SHIFT) + u64(b));
+        if (ov) over[atomicAdd(&meta->nover, 1u)] = BtOver{w.q, uint32_t(k[u]), x[u], 0u};
+      }
+    }
+    __syncthreads();
     u64* qslots = slots + qd.slot0;
-    if (card <= uint32_t(BT_LDS_SLOTS)) {
-      for (uint32_t s = threadIdx.x; s < card; s += BT_BLOCK) s_acc[s] = 0;
-      __syncthreads();
-      for (uint32_t r = w.r0 + threadIdx.x; r < w.r1; r += BT_BLOCK) {
-        const int32_t k = keys[r];
-        if (uint32_t(k) >= card) {
-          atomicMin(&meta->bad_row, r);
-          continue;
-        }
-        bool ov;
-        const u64 add = bt_packed(vals[r], &ov);
-        atomicAdd(&s_acc[k], add);
-        if (ov) over[atomicAdd(&meta->nover, 1u)] = BtOver{qd.slot0 + u64(k), vals[r], 0u};
-      }
-      __syncthreads();
-      for (uint32_t s = threadIdx.x; s < card; s += BT_BLOCK) {
-        const u64 a = s_acc[s];
-        if (a) atomicAdd(&qslots[s], a);
-      }
-      __syncthreads();
-    } else {
-      for (uint32_t r = w.r0 + threadIdx.x; r < w.r1; r += BT_BLOCK) {
-        const int32_t k = keys[r];
-        if (uint32_t(k) >= card) {
-          atomicMin(&meta->bad_row, r);
-          continue;
-        }
-        bool ov;
-        const u64 add = bt_packed(vals[r], &ov);
-        atomicAdd(&qslots[k], add);
-        if (ov) over[atomicAdd(&meta->nover, 1u)] = BtOver{qd.slot0 + u64(k), vals[r], 0u};
-      }
+    for (uint32_t s = threadIdx.x; s < card; s += BT_BLOCK) {
+      const u64 a = s_acc[s];
+      if (a) atomicAdd(&qslots[s], a);
     }
+    __syncthreads();
   }
 }
 
-// Stable compaction of present slots over chunks [c0, c1) (chunk order =
-// query order, then key order).  Present slot s of query q becomes output
-// group pos = base + prefix: ukeys[pos] = s - slot0(q), counts / sums from
-// the packed word; slots[s] is overwritten with pos (for the overflow
-// list).  Chunk 0 of each query writes group_off[q]; the last chunk writes
-// meta->total and group_off[nq].
-__global__ __launch_bounds__(BT_BLOCK) void k_bt_compact(
-    const BtChunk* __restrict__ chunks, uint32_t c0, uint32_t c1,
-    const BtQuery* __restrict__ qs, u64* __restrict__ slots, int64_t* __restrict__ ukeys,
-    int64_t* __restrict__ counts, int64_t* __restrict__ sums, int64_t* __restrict__ group_off,
-    uint32_t nq_end, u64 cap, uint64_t* __restrict__ lookback, uint32_t epoch,
+// Ranged queries: one workgroup per tile tiles[t_first + blockIdx.x].  Each
+// row becomes the record (key & (BT_RANGE - 1)) << 16 | biased value,
+// grouped by unit key >> BT_RANGE_BITS within the tile's own row range:
+// rec[r0 + start_j + rank] (ranks from LDS counters, in no particular
+// order -- aggregation does not depend on it), entry[e0 + j] = (r0 +
+// start_j, count_j).  A key outside [0, card) is dropped and marks the run.
+__global__ __launch_bounds__(BT_PBLOCK) void k_bt_part(
+    const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
+    const BtTile* __restrict__ tiles, uint32_t t_first, const BtQuery* __restrict__ qs,
+    uint32_t* __restrict__ rec, uint2* __restrict__ entry, BtOver* __restrict__ over,
     BtMeta* __restrict__ meta) {
-  __shared__ uint32_t s_cnt[BT_CIPT][BT_BLOCK / WAVE];
-  __shared__ uint32_t s_tile;
-  __shared__ u64 s_pre;
-  const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-  if (tid == 0) s_tile = atomicAdd(&meta->chunk, 1u);
+  __shared__ uint32_t s_cnt[BT_MAX_RANGES];
+  __shared__ uint32_t s_tmp[BT_PBLOCK / WAVE];
+  const BtTile t = tiles[t_first + blockIdx.x];
+  const BtQuery qd = qs[t.q];
+  const uint32_t nr = qd.nr, card = qd.card;
+  for (uint32_t j = threadIdx.x; j < nr; j += BT_PBLOCK) s_cnt[j] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;  // position in [0, c1 - c0)
-  const BtChunk ch = chunks[c0 + tile];
-  const u64 qslot0 = qs[ch.q].slot0;
-  const uint64_t lt = lanemask_lt();
-  u64 pk[BT_CIPT];
-  uint32_t r0[BT_CIPT];
+  int32_t k[BT_PIPT], x[BT_PIPT];
 #pragma unroll
-  for (int j = 0; j < BT_CIPT; ++j) {
-    const u64 s = ch.s0 + u64(j * BT_BLOCK + tid);
-    pk[j] = s < ch.s1 ? slots[s] : 0ull;
-    const uint64_t b = __ballot((pk[j] >> BT_SHIFT) != 0);
-    r0[j] = uint32_t(__popcll(b & lt));
-    if (lane == 0) s_cnt[j][wave] = uint32_t(__popcll(b));
+  for (int u = 0; u < BT_PIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * BT_PBLOCK + threadIdx.x;
+    k[u] = r < t.r1 ? keys[r] : -1;
+    x[u] = r < t.r1 ? vals[r] : 0;
+  }
+  uint32_t rc[BT_PIPT], loc[BT_PIPT];
+#pragma unroll
+  for (int u = 0; u < BT_PIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * BT_PBLOCK + threadIdx.x;
+    rc[u] = 0xFFFFFFFFu;  // no record
+    if (r >= t.r1) continue;
+    if (uint32_t(k[u]) >= card) {
+      atomicMin(&meta->bad_row, r);
+      continue;
+    }
+    bool ov;
+    const uint32_t b = bt_biased(x[u], &ov);
+    if (ov) over[atomicAdd(&meta->nover, 1u)] = BtOver{t.q, uint32_t(k[u]), x[u], 0u};
+    rc[u] = ((uint32_t(k[u]) & (BT_RANGE - 1)) << 16) | b;
+    loc[u] = atomicAdd(&s_cnt[uint32_t(k[u]) >> BT_RANGE_BITS], 1u);
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t t = 0;
+  // unit starts: exclusive scan of the counts, two units per thread
+  const uint32_t j0 = 2 * threadIdx.x;
+  const uint32_t c0 = j0 < nr ? s_cnt[j0] : 0u, c1 = j0 + 1 < nr ? s_cnt[j0 + 1] : 0u;
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan<BT_PBLOCK / WAVE>(c0 + c1, s_tmp, &tot);
+  if (j0 < nr) entry[t.e0 + j0] = make_uint2(t.r0 + ex, c0);
+  if (j0 + 1 < nr) entry[t.e0 + j0 + 1] = make_uint2(t.r0 + ex + c0, c1);
+  __syncthreads();  // every count read before the starts overwrite them
+  if (j0 < nr) s_cnt[j0] = ex;
+  if (j0 + 1 < nr) s_cnt[j0 + 1] = ex + c0;
+  __syncthreads();
 #pragma unroll
-    for (int j = 0; j < BT_CIPT; ++j)
+  for (int u = 0; u < BT_PIPT; ++u)
+    if (rc[u] != 0xFFFFFFFFu)
+      rec[t.r0 + s_cnt[uint32_t(k[u]) >> BT_RANGE_BITS] + loc[u]] = rc[u];
+}
+
+// The records of ranged unit `un`, one wave per tile entry, four in flight
+// per lane: f(record) for each.
+template <int NT, class F>
+__device__ __forceinline__ void bt_for_records(const BtUnit& un, const uint32_t* __restrict__ rec,
+                                               const uint2* __restrict__ entry, F f) {
+  constexpr uint32_t NW = NT / WAVE;
+  const uint32_t lane = uint32_t(lane_id());
+  for (uint32_t t = un.t0 + uint32_t(wave_id()); t < un.t1; t += NW) {
+    const uint2 e = entry[un.e0 + (t - un.t0) * un.nr + un.j];
+    for (uint32_t i = lane; i < e.y; i += 4 * WAVE) {
+      uint32_t r[4];
 #pragma unroll
-      for (int w = 0; w < BT_BLOCK / WAVE; ++w) {
-        const uint32_t c = s_cnt[j][w];
-        s_cnt[j][w] = t;
-        t += c;
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t ii = i + uint32_t(u) * WAVE;
+        r[u] = ii < e.y ? rec[e.x + ii] : 0xFFFFFFFFu;
       }
-    uint64_t* my = lookback + tile;
-    uint32_t ex = 0;
-    if (tile == 0) {
-      lb_store(my, lb_pack(epoch, LB_PREFIX, t));
-    } else {
-      lb_store(my, lb_pack(epoch, LB_AGG, t));
-      ex = lb_lookback(lookback, tile, 1, epoch);
-      lb_store(my, lb_pack(epoch, LB_PREFIX, ex + t));
-    }
-    const u64 pre = meta->base + u64(ex);
-    s_pre = pre;
-    if (ch.s0 == qslot0) group_off[ch.q] = int64_t(pre);
-    if (tile == c1 - c0 - 1) {
-      meta->total = pre + t;
-      group_off[nq_end] = int64_t(pre + t);
-    }
-  }
-  __syncthreads();
-  const u64 pre = s_pre;
 #pragma unroll
-  for (int j = 0; j < BT_CIPT; ++j) {
-    const u64 c = pk[j] >> BT_SHIFT;
-    if (c == 0) continue;
-    const u64 pos = pre + s_cnt[j][wave] + r0[j];
-    const u64 s = ch.s0 + u64(j * BT_BLOCK + tid);
-    if (pos < cap) {
-      ukeys[pos] = int64_t(s - qslot0);
-      counts[pos] = int64_t(c);
-      sums[pos] = int64_t(pk[j] & BT_MASK) - int64_t(c) * 32768;
+      for (int u = 0; u < 4; ++u)
+        if (r[u] != 0xFFFFFFFFu) f(r[u]);
     }
-    slots[s] = pos;
   }
 }
 
-// Overflow rows: their true values join the compacted sums.
+// Present groups of units [u_first, u_first + gridDim.x): small units count
+// their present slots, ranged units the distinct keys of their records.
+__global__ __launch_bounds__(BT_CBLOCK) void k_bt_count(
+    const BtUnit* __restrict__ units, uint32_t u_first, const BtQuery* __restrict__ qs,
+    const u64* __restrict__ slots, const uint32_t* __restrict__ rec,
+    const uint2* __restrict__ entry, uint32_t* __restrict__ ucount) {
+  __shared__ uint32_t s_bits[BT_RANGE / 32];
+  __shared__ uint32_t s_tmp[BT_CBLOCK / WAVE];
+  const uint32_t u = u_first + blockIdx.x;
+  const BtUnit un = units[u];
+  const BtQuery qd = qs[un.q];
+  uint32_t c = 0;
+  if (qd.nr == 0) {
+    for (uint32_t i = threadIdx.x; i < un.nkeys; i += BT_CBLOCK)
+      c += (slots[qd.slot0 + i] >> BT_SHIFT) != 0 ? 1u : 0u;
+  } else {
+    for (uint32_t w = threadIdx.x; w < BT_RANGE / 32; w += BT_CBLOCK) s_bits[w] = 0;
+    __syncthreads();
+    bt_for_records<BT_CBLOCK>(un, rec, entry, [&](uint32_t r) {
+      const uint32_t kl = r >> 16;
+      atomicOr(&s_bits[kl >> 5], 1u << (kl & 31u));
+    });
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < BT_RANGE / 32; w += BT_CBLOCK) c += __popc(s_bits[w]);
+  }
+  uint32_t tot;
+  (void)block_excl_scan<BT_CBLOCK / WAVE>(c, s_tmp, &tot);
+  if (threadIdx.x == 0) ucount[u] = tot;
+}
+
+// Exclusive scan of the unit counts of [u0, u1) from meta->base (one
+// workgroup): uoff per unit, group_off[q] at each query's first unit,
+// meta->total and group_off[nq_end] at the end.
+__global__ __launch_bounds__(BT_SCAN) void k_bt_scan(const BtUnit* __restrict__ units,
+                                                     uint32_t u0, uint32_t u1,
+                                                     const uint32_t* __restrict__ ucount,
+                                                     u64* __restrict__ uoff,
+                                                     int64_t* __restrict__ group_off,
+                                                     uint32_t nq_end, BtMeta* __restrict__ meta) {
+  constexpr int PER = 4;
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  __shared__ u64 s_carry;
+  if (threadIdx.x == 0) s_carry = meta->base;
+  __syncthreads();
+  for (uint32_t b = u0; b < u1; b += uint32_t(BT_SCAN) * PER) {
+    uint32_t c[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t u = b + threadIdx.x * PER + uint32_t(i);
+      c[i] = u < u1 ? ucount[u] : 0u;
+      sum += c[i];
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BT_SCAN / WAVE>(sum, s_tmp, &tot);
+    u64 run = s_carry + ex;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t u = b + threadIdx.x * PER + uint32_t(i);
+      if (u < u1) {
+        uoff[u] = run;
+        if (units[u].j == 0) group_off[units[u].q] = int64_t(run);
+      }
+      run += c[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    meta->total = s_carry;
+    group_off[nq_end] = int64_t(s_carry);
+  }
+}
+
+// Groups of units [u_first, u_first + gridDim.x) in key order at their
+// offsets (positions at or past cap are not written).  Ranged units first
+// aggregate their records in LDS; small units read their slots and leave
+// each present slot's output position in it (for the overflow pass).
+// Each wave compacts one contiguous segment of the unit's keys by ballot.
+__global__ __launch_bounds__(BT_EBLOCK) void k_bt_emit(
+    const BtUnit* __restrict__ units, uint32_t u_first, const BtQuery* __restrict__ qs,
+    u64* __restrict__ slots, const uint32_t* __restrict__ rec, const uint2* __restrict__ entry,
+    const u64* __restrict__ uoff, int64_t* __restrict__ ukeys, int64_t* __restrict__ counts,
+    int64_t* __restrict__ sums, u64 cap) {
+  constexpr uint32_t NW = BT_EBLOCK / WAVE;
+  __shared__ u64 s_acc[BT_RANGE];
+  __shared__ uint32_t s_wc[NW];
+  const uint32_t u = u_first + blockIdx.x;
+  const BtUnit un = units[u];
+  const BtQuery qd = qs[un.q];
+  const bool small = qd.nr == 0;
+  if (!small) {
+    for (uint32_t i = threadIdx.x; i < un.nkeys; i += BT_EBLOCK) s_acc[i] = 0;
+    __syncthreads();
+    bt_for_records<BT_EBLOCK>(un, rec, entry, [&](uint32_t r) {
+      atomicAdd(&s_acc[r >> 16], (u64(1) << BT_SHIFT) + u64(r & 0xFFFFu));
+    });
+    __syncthreads();
+  }
+  auto packed = [&](uint32_t i) -> u64 {
+    if (i >= un.nkeys) return 0ull;
+    return small ? slots[qd.slot0 + i] : s_acc[i];
+  };
+  const uint32_t wave = uint32_t(wave_id()), lane = uint32_t(lane_id());
+  const uint32_t seg = (un.nkeys + NW * WAVE - 1) / (NW * WAVE) * WAVE;  // keys per wave
+  const uint32_t k0 = wave * seg;
+  uint32_t c = 0;
+  for (uint32_t i = k0; i < k0 + seg; i += WAVE)
+    c += uint32_t(__popcll(__ballot((packed(i + lane) >> BT_SHIFT) != 0)));
+  if (lane == 0) s_wc[wave] = c;
+  __syncthreads();
+  u64 pos = uoff[u];
+  for (uint32_t w = 0; w < wave; ++w) pos += s_wc[w];
+  const uint64_t lt = lanemask_lt();
+  for (uint32_t i = k0; i < k0 + seg; i += WAVE) {
+    const uint32_t key = i + lane;
+    const u64 pk = packed(key);
+    const u64 cnt = pk >> BT_SHIFT;
+    const uint64_t b = __ballot(cnt != 0);
+    if (cnt != 0) {
+      const u64 p = pos + u64(__popcll(b & lt));
+      if (p < cap) {
+        ukeys[p] = int64_t(un.key0 + key);
+        counts[p] = int64_t(cnt);
+        sums[p] = int64_t(pk & BT_MASK) - int64_t(cnt) * 32768;
+      }
+      if (small) slots[qd.slot0 + key] = p;
+    }
+    pos += u64(__popcll(b));
+  }
+}
+
+// Overflow rows: their true values join the compacted sums.  Small
+// queries find the output position in the slot; ranged queries search
+// their (ascending) output keys.
 __global__ void k_bt_overflow(const BtOver* __restrict__ over, const BtMeta* __restrict__ meta,
-                              const u64* __restrict__ slots, int64_t* __restrict__ sums, u64 cap) {
+                              const BtQuery* __restrict__ qs, const u64* __restrict__ slots,
+                              const int64_t* __restrict__ group_off,
+                              const int64_t* __restrict__ ukeys, int64_t* __restrict__ sums,
+                              u64 cap) {
   const uint32_t n = meta->nover;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const BtOver e = over[i];
-    const u64 pos = slots[e.slot];
+    const BtQuery qd = qs[e.q];
+    u64 pos = ~0ull;
+    if (qd.nr == 0) {
+      pos = slots[qd.slot0 + e.key];
+    } else {
+      u64 lo = u64(group_off[e.q]), hi = std::min<u64>(u64(group_off[e.q + 1]), cap);
+      while (lo < hi) {  // first output key >= e.key
+        const u64 mid = (lo + hi) / 2;
+        if (ukeys[mid] < int64_t(e.key))
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < std::min<u64>(u64(group_off[e.q + 1]), cap) && ukeys[lo] == int64_t(e.key))
+        pos = lo;
+    }
     if (pos < cap) atomicAdd(reinterpret_cast<u64*>(sums + pos), u64(int64_t(e.v)));
   }
 }
@@ -227,16 +407,14 @@ __global__ void k_bt_overflow(const BtOver* __restrict__ over, const BtMeta* __r
 // meta reset for a batched run: counters 0, bad_row none, base 0.
 __global__ void k_bt_reset(BtMeta* meta) {
   meta->work = 0;
-  meta->chunk = 0;
   meta->nover = 0;
   meta->bad_row = 0xFFFFFFFFu;
   meta->base = 0;
   meta->total = 0;
 }
-// per-query chain: counters 0, base = previous query's total.
+// per-query chain: work counter 0, base = previous query's total.
 __global__ void k_bt_next(BtMeta* meta) {
   meta->work = 0;
-  meta->chunk = 0;
   meta->base = meta->total;
 }
 
@@ -264,15 +442,21 @@ struct BatchPlan {
   u64 cap = 0;
   uint32_t nq = 0;
   uint64_t nrows = 0, nslots = 0;
-  std::vector<uint32_t> item_start, chunk_start;  // per query (+1): ranges
+  // per query (+1): first work item, tile and unit
+  std::vector<uint32_t> item_start, tile_start, unit_start;
   BtItem* items = nullptr;
-  BtChunk* chunks = nullptr;
+  BtTile* tiles = nullptr;
+  BtUnit* units = nullptr;
   BtQuery* qs = nullptr;
   u64* slots = nullptr;
+  uint32_t* rec = nullptr;
+  uint2* entry = nullptr;
+  uint32_t* ucount = nullptr;
+  u64* uoff = nullptr;
   BtOver* over = nullptr;
   BtMeta* meta = nullptr;
-  uint64_t* lookback = nullptr;
-  uint32_t nitems = 0, nchunks = 0;
+  uint32_t nitems = 0, ntiles = 0, nunits = 0;
+  uint64_t nentries = 0;
   uint32_t rows_per_item = 0, grid = 0;
   hipStream_t s = nullptr;
   hipGraph_t graph = nullptr;
@@ -295,50 +479,67 @@ struct BatchPlan {
   }
 };
 
-static void build_items(BatchPlan& P, const std::vector<int64_t>& roff, uint32_t R) {
+static bool bt_small(int32_t card) { return card <= BT_LDS_SLOTS; }
+
+// Work items of the small queries, R rows each.
+static void build_items(BatchPlan& P, uint32_t R) {
   std::vector<BtItem> it;
   P.item_start.assign(P.nq + 1, 0);
   for (uint32_t q = 0; q < P.nq; ++q) {
     P.item_start[q] = uint32_t(it.size());
-    for (int64_t r = roff[q]; r < roff[q + 1]; r += R)
-      it.push_back({q, uint32_t(r), uint32_t(std::min<int64_t>(r + R, roff[q + 1]))});
+    if (!bt_small(P.card[q])) continue;
+    for (int64_t r = P.roff[q]; r < P.roff[q + 1]; r += R)
+      it.push_back({q, uint32_t(r), uint32_t(std::min<int64_t>(r + R, P.roff[q + 1]))});
   }
   P.item_start[P.nq] = uint32_t(it.size());
   P.nitems = uint32_t(it.size());
   if (!P.items) P.items = P.alloc<BtItem>(P.nrows / 512 + P.nq + 16);  // R >= 512
-  NFH_CHECK(hipMemcpy(P.items, it.data(), it.size() * sizeof(BtItem), hipMemcpyHostToDevice));
+  if (!it.empty())
+    NFH_CHECK(hipMemcpy(P.items, it.data(), it.size() * sizeof(BtItem), hipMemcpyHostToDevice));
   P.rows_per_item = R;
 }
 
-// The whole batched run on P.s: reset, zero slots, accumulate, compact,
-// overflow.  No host synchronisation (graph-capturable).
+// The whole batched run on P.s: reset, zero the small queries' slots,
+// accumulate (small), partition (ranged), count, scan, emit, overflow.  No
+// host synchronisation (graph-capturable).
 static void enqueue_batched(BatchPlan& P) {
   hipLaunchKernelGGL(k_bt_reset, dim3(1), dim3(1), 0, P.s, P.meta);
   NFH_LAUNCH_CHECK();
-  fill_u64(P.slots, 0, P.nslots, P.s);
-  fill_u64(P.lookback, 0, P.nchunks + 1, P.s);  // look-back words of the last run
-  hipLaunchKernelGGL(k_bt_accumulate, dim3(P.grid), dim3(BT_BLOCK), 0, P.s, P.keys, P.vals,
-                     P.items, 0u, P.nitems, P.qs, P.slots, P.over, P.meta);
+  if (P.nslots) fill_u64(P.slots, 0, P.nslots, P.s);
+  if (P.nitems) {
+    hipLaunchKernelGGL(k_bt_accumulate, dim3(P.grid), dim3(BT_BLOCK), 0, P.s, P.keys, P.vals,
+                       P.items, 0u, P.nitems, P.qs, P.slots, P.over, P.meta);
+    NFH_LAUNCH_CHECK();
+  }
+  if (P.ntiles) {
+    hipLaunchKernelGGL(k_bt_part, dim3(P.ntiles), dim3(BT_PBLOCK), 0, P.s, P.keys, P.vals,
+                       P.tiles, 0u, P.qs, P.rec, P.entry, P.over, P.meta);
+    NFH_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_bt_count, dim3(P.nunits), dim3(BT_CBLOCK), 0, P.s, P.units, 0u, P.qs,
+                     P.slots, P.rec, P.entry, P.ucount);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_compact, dim3(P.nchunks), dim3(BT_BLOCK), 0, P.s, P.chunks, 0u,
-                     P.nchunks, P.qs, P.slots, P.ukeys, P.counts, P.sums, P.group_off, P.nq,
-                     P.cap, P.lookback, 1u, P.meta);
+  hipLaunchKernelGGL(k_bt_scan, dim3(1), dim3(BT_SCAN), 0, P.s, P.units, 0u, P.nunits,
+                     P.ucount, P.uoff, P.group_off, P.nq, P.meta);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_overflow, dim3(256), dim3(256), 0, P.s, P.over, P.meta, P.slots,
-                     P.sums, P.cap);
+  hipLaunchKernelGGL(k_bt_emit, dim3(P.nunits), dim3(BT_EBLOCK), 0, P.s, P.units, 0u, P.qs,
+                     P.slots, P.rec, P.entry, P.uoff, P.ukeys, P.counts, P.sums, P.cap);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_overflow, dim3(256), dim3(256), 0, P.s, P.over, P.meta, P.qs, P.slots,
+                     P.group_off, P.ukeys, P.sums, P.cap);
   NFH_LAUNCH_CHECK();
 }
 
-// The plain form: per query a slot fill, an accumulate launch over its
-// items and a compaction launch over its chunks, chained on the device.
+// The plain form: per query its accumulate or partition launch, then its
+// count, scan and emit launches, chained on the device.
 static void enqueue_per_query(BatchPlan& P) {
   hipLaunchKernelGGL(k_bt_reset, dim3(1), dim3(1), 0, P.s, P.meta);
   NFH_LAUNCH_CHECK();
-  fill_u64(P.slots, 0, P.nslots, P.s);
-  fill_u64(P.lookback, 0, P.nchunks + 1, P.s);  // epochs 2.. below are unique per query
+  if (P.nslots) fill_u64(P.slots, 0, P.nslots, P.s);
   for (uint32_t q = 0; q < P.nq; ++q) {
     const uint32_t i0 = P.item_start[q], i1 = P.item_start[q + 1];
-    const uint32_t c0 = P.chunk_start[q], c1 = P.chunk_start[q + 1];
+    const uint32_t t0 = P.tile_start[q], t1 = P.tile_start[q + 1];
+    const uint32_t u0 = P.unit_start[q], u1 = P.unit_start[q + 1];
     if (q > 0) {
       hipLaunchKernelGGL(k_bt_next, dim3(1), dim3(1), 0, P.s, P.meta);
       NFH_LAUNCH_CHECK();
@@ -349,13 +550,23 @@ static void enqueue_per_query(BatchPlan& P) {
                          P.over, P.meta);
       NFH_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_bt_compact, dim3(c1 - c0), dim3(BT_BLOCK), 0, P.s, P.chunks, c0, c1,
-                       P.qs, P.slots, P.ukeys, P.counts, P.sums, P.group_off, q + 1, P.cap,
-                       P.lookback, 2u + (q & 0x3FFFFFFu), P.meta);
+    if (t1 > t0) {
+      hipLaunchKernelGGL(k_bt_part, dim3(t1 - t0), dim3(BT_PBLOCK), 0, P.s, P.keys, P.vals,
+                         P.tiles, t0, P.qs, P.rec, P.entry, P.over, P.meta);
+      NFH_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_bt_count, dim3(u1 - u0), dim3(BT_CBLOCK), 0, P.s, P.units, u0, P.qs,
+                       P.slots, P.rec, P.entry, P.ucount);
+    NFH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bt_scan, dim3(1), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.ucount,
+                       P.uoff, P.group_off, q + 1, P.meta);
+    NFH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bt_emit, dim3(u1 - u0), dim3(BT_EBLOCK), 0, P.s, P.units, u0, P.qs,
+                       P.slots, P.rec, P.entry, P.uoff, P.ukeys, P.counts, P.sums, P.cap);
     NFH_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_bt_overflow, dim3(256), dim3(256), 0, P.s, P.over, P.meta, P.slots,
-                     P.sums, P.cap);
+  hipLaunchKernelGGL(k_bt_overflow, dim3(256), dim3(256), 0, P.s, P.over, P.meta, P.qs, P.slots,
+                     P.group_off, P.ukeys, P.sums, P.cap);
   NFH_LAUNCH_CHECK();
 }
 
@@ -399,7 +610,7 @@ static float time_accumulate(BatchPlan& P) {
   float best = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {
     hipLaunchKernelGGL(k_bt_reset, dim3(1), dim3(1), 0, P.s, P.meta);
-    fill_u64(P.slots, 0, P.nslots, P.s);
+    if (P.nslots) fill_u64(P.slots, 0, P.nslots, P.s);
     NFH_CHECK(hipEventRecord(a, P.s));
     hipLaunchKernelGGL(k_bt_accumulate, dim3(P.grid), dim3(BT_BLOCK), 0, P.s, P.keys, P.vals,
                        P.items, 0u, P.nitems, P.qs, P.slots, P.over, P.meta);
@@ -425,10 +636,11 @@ using namespace nfh;
 // (device, cap lanes): ukeys / counts / sums of the present groups, query
 // by query, keys ascending; group_off[nq + 1] the per-query group offsets.
 // row_off / card may be host or device arrays (read once here).  The plan
-// picks rows per work item and the persistent grid by timing candidates on
-// these buffers (autotuning), then captures reset, slot fill, accumulate,
-// compaction and overflow into one hipGraph.  Returns a handle or NULL
-// (*rc_out: -1 null, -3 a query over 2^24 rows or bad offsets, -6 device).
+// picks rows per work item and the persistent grid of the small-query pass
+// by timing candidates on these buffers (autotuning), then captures the
+// whole run into one hipGraph.  Returns a handle or NULL (*rc_out: -1 null,
+// -3 a query over 2^24 rows or keys, bad offsets, or buffers beyond the
+// free device memory, -6 device).
 NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
                                      const int64_t* row_off, const int32_t* card, size_t nq,
                                      int64_t* ukeys, int64_t* counts, int64_t* sums,
@@ -446,18 +658,52 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     std::vector<int32_t> cd(nq);
     NFH_CHECK(hipMemcpy(roff.data(), row_off, (nq + 1) * 8, hipMemcpyDefault));
     NFH_CHECK(hipMemcpy(cd.data(), card, nq * 4, hipMemcpyDefault));
-    if (roff[0] != 0) return fail(MALFORMED);
+    if (roff[0] != 0 || nq >= size_t(INT32_MAX)) return fail(MALFORMED);
     for (size_t q = 0; q < nq; ++q)
-      if (roff[q + 1] < roff[q] || roff[q + 1] - roff[q] >= int64_t(BT_MAX_ROWS) || cd[q] < 0)
+      if (roff[q + 1] < roff[q] || roff[q + 1] - roff[q] >= int64_t(BT_MAX_ROWS) || cd[q] < 0 ||
+          uint32_t(cd[q]) > BT_MAX_RANGES * BT_RANGE)
         return fail(MALFORMED);
     if (roff[nq] > int64_t(INT32_MAX)) return fail(MALFORMED);
-    // the dense slots (8 B per key of every query) and the overflow list
-    // (16 B per row) must fit the free device memory
-    uint64_t total_card = 0;
-    for (size_t q = 0; q < nq; ++q) total_card += uint64_t(cd[q]);
+    // queries, slots (small), tiles and entries (ranged), units
+    std::vector<BtQuery> qs(nq);
+    std::vector<BtTile> tl;
+    std::vector<BtUnit> un;
+    std::vector<uint32_t> tile_start(nq + 1), unit_start(nq + 1);
+    u64 slot = 0, ent = 0;
+    for (size_t q = 0; q < nq; ++q) {
+      const uint32_t c = uint32_t(cd[q]);
+      tile_start[q] = uint32_t(tl.size());
+      unit_start[q] = uint32_t(un.size());
+      if (bt_small(cd[q])) {
+        qs[q] = {slot, c, 0};
+        un.push_back({uint32_t(q), 0, 0, c, 0, 0, 0, 0});
+        slot += c;
+        continue;
+      }
+      const uint32_t nr = (c + BT_RANGE - 1) / BT_RANGE;
+      qs[q] = {0, c, nr};
+      const uint32_t t0 = uint32_t(tl.size());
+      const uint64_t e0 = ent;
+      for (int64_t r = roff[q]; r < roff[q + 1]; r += BT_PTILE) {
+        tl.push_back({uint32_t(q), uint32_t(r),
+                      uint32_t(std::min<int64_t>(r + BT_PTILE, roff[q + 1])), uint32_t(ent)});
+        ent += nr;
+      }
+      const uint32_t t1 = uint32_t(tl.size());
+      for (uint32_t j = 0; j < nr; ++j)
+        un.push_back({uint32_t(q), j, j * BT_RANGE, std::min(BT_RANGE, c - j * BT_RANGE), t0, t1,
+                      uint32_t(e0), nr});
+    }
+    tile_start[nq] = uint32_t(tl.size());
+    unit_start[nq] = uint32_t(un.size());
+    if (ent > uint64_t(UINT32_MAX) || un.size() >= size_t(INT32_MAX)) return fail(MALFORMED);
+    // slots (8 B per key of each small query), records (4 B per row), the
+    // overflow list (16 B per row) and the entries must fit the free memory
     size_t free_b = 0, total_b = 0;
     NFH_CHECK(hipMemGetInfo(&free_b, &total_b));
-    if (total_card * 8 + uint64_t(roff[nq]) * 16 > uint64_t(free_b)) return fail(MALFORMED);
+    const uint64_t need = slot * 8 + uint64_t(roff[nq]) * 20 + ent * 8 + un.size() * 48 +
+                          tl.size() * 16;
+    if (need > uint64_t(free_b)) return fail(MALFORMED);
     std::unique_ptr<BatchPlan> P(new BatchPlan());
     P->roff = roff;
     P->card = cd;
@@ -470,53 +716,47 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     P->cap = cap;
     P->nq = uint32_t(nq);
     P->nrows = uint64_t(roff[nq]);
-    NFH_CHECK(hipStreamCreate(&P->s));
-    // queries, slots, chunks
-    std::vector<BtQuery> qs(nq);
-    std::vector<BtChunk> chs;
-    P->chunk_start.assign(nq + 1, 0);
-    u64 slot = 0;
-    for (size_t q = 0; q < nq; ++q) {
-      qs[q] = {slot, uint32_t(cd[q]), 0};
-      P->chunk_start[q] = uint32_t(chs.size());
-      u64 s0 = slot;
-      do {
-        const u64 s1 = std::min<u64>(s0 + BT_CHUNK, slot + u64(cd[q]));
-        chs.push_back({uint32_t(q), 0, s0, s1});
-        s0 = s1;
-      } while (s0 < slot + u64(cd[q]));
-      slot += u64(cd[q]);
-    }
-    P->chunk_start[nq] = uint32_t(chs.size());
     P->nslots = slot;
-    P->nchunks = uint32_t(chs.size());
+    P->ntiles = uint32_t(tl.size());
+    P->nunits = uint32_t(un.size());
+    P->nentries = ent;
+    P->tile_start = tile_start;
+    P->unit_start = unit_start;
+    NFH_CHECK(hipStreamCreate(&P->s));
     P->qs = P->alloc<BtQuery>(nq);
-    P->chunks = P->alloc<BtChunk>(chs.size());
+    P->tiles = P->alloc<BtTile>(tl.size());
+    P->units = P->alloc<BtUnit>(un.size());
     P->slots = P->alloc<u64>(P->nslots);
+    P->rec = P->alloc<uint32_t>(P->nrows);
+    P->entry = P->alloc<uint2>(ent);
+    P->ucount = P->alloc<uint32_t>(un.size());
+    P->uoff = P->alloc<u64>(un.size());
     P->over = P->alloc<BtOver>(P->nrows + 1);
     P->meta = P->alloc<BtMeta>(1);
-    P->lookback = P->alloc<uint64_t>(chs.size() + 1);
     NFH_CHECK(hipMemcpy(P->qs, qs.data(), nq * sizeof(BtQuery), hipMemcpyHostToDevice));
-    NFH_CHECK(hipMemcpy(P->chunks, chs.data(), chs.size() * sizeof(BtChunk),
-                        hipMemcpyHostToDevice));
-    fill_u64(P->lookback, 0, chs.size() + 1, P->s);
-    // autotune rows per work item and the persistent grid
-    float best = 1e30f;
+    if (!tl.empty())
+      NFH_CHECK(hipMemcpy(P->tiles, tl.data(), tl.size() * sizeof(BtTile), hipMemcpyHostToDevice));
+    NFH_CHECK(hipMemcpy(P->units, un.data(), un.size() * sizeof(BtUnit), hipMemcpyHostToDevice));
+    // autotune rows per work item and the persistent grid of the small pass
     uint32_t bestR = 8192, bestG = 4;
     const uint32_t cus = uint32_t(num_cus());
-    for (uint32_t R : {2048u, 8192u, 32768u}) {
-      build_items(*P, roff, R);
-      for (uint32_t g : {2u, 4u, 8u}) {
-        P->grid = cus * g;
-        const float ms = time_accumulate(*P);
-        if (ms < best) {
-          best = ms;
-          bestR = R;
-          bestG = g;
+    build_items(*P, bestR);
+    if (P->nitems) {
+      float best = 1e30f;
+      for (uint32_t R : {2048u, 8192u, 32768u}) {
+        build_items(*P, R);
+        for (uint32_t g : {2u, 4u, 8u}) {
+          P->grid = cus * g;
+          const float ms = time_accumulate(*P);
+          if (ms < best) {
+            best = ms;
+            bestR = R;
+            bestG = g;
+          }
         }
       }
+      build_items(*P, bestR);
     }
-    build_items(*P, roff, bestR);
     P->grid = cus * bestG;
     // capture the batched run (an error inside ends the capture first)
     NFH_CHECK(hipStreamBeginCapture(P->s, hipStreamCaptureModeThreadLocal));
@@ -570,8 +810,8 @@ NF_EXPORT int64_t nfh_group_batch_run(void* plan, int32_t mode) {
   }
 }
 
-// Plan details for reports: rows per work item, workgroups, work items,
-// slots.
+// Plan details for reports: rows per work item and workgroups of the
+// small-query pass, its work items, the small queries' slots.
 NF_EXPORT int32_t nfh_group_batch_info(void* plan, uint64_t* out4) {
   if (!plan || !out4) return NULL_OR_ABORT;
   auto* P = static_cast<BatchPlan*>(plan);
@@ -579,6 +819,17 @@ NF_EXPORT int32_t nfh_group_batch_info(void* plan, uint64_t* out4) {
   out4[1] = P->grid;
   out4[2] = P->nitems;
   out4[3] = P->nslots;
+  return OK;
+}
+
+// Ranged-path sizes for reports: partition tiles, units, (tile, unit)
+// entries.
+NF_EXPORT int32_t nfh_group_batch_info2(void* plan, uint64_t* out3) {
+  if (!plan || !out3) return NULL_OR_ABORT;
+  auto* P = static_cast<BatchPlan*>(plan);
+  out3[0] = P->ntiles;
+  out3[1] = P->nunits;
+  out3[2] = P->nentries;
   return OK;
 }
 

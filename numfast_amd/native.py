@@ -122,6 +122,7 @@ GROUPS = {
         ("nfh_group_batch_plan", [_vp] * 4 + [_sz] + [_vp] * 4 + [_sz, _vp], _vp),
         ("nfh_group_batch_run", [_vp, _i32], _i64),
         ("nfh_group_batch_info", [_vp, _vp], _i32),
+        ("nfh_group_batch_info2", [_vp, _vp], _i32),
         ("nfh_group_batch_free", [_vp], None),
     ],
     # Additive surface of this library (no reference counterpart).
@@ -978,8 +979,11 @@ class GroupBatch:
     def info(self):
         out = np.zeros(4, np.uint64)
         self._L.nfh_group_batch_info(self.plan, out.ctypes.data)
+        o2 = np.zeros(3, np.uint64)
+        self._L.nfh_group_batch_info2(self.plan, o2.ctypes.data)
         return {"rows_per_item": int(out[0]), "workgroups": int(out[1]),
-                "work_items": int(out[2]), "slots": int(out[3])}
+                "work_items": int(out[2]), "slots": int(out[3]),
+                "wide_tiles": int(o2[0]), "units": int(o2[1]), "entries": int(o2[2])}
 
     def close(self):
         if self.plan:

@@ -120,3 +120,66 @@ def test_batch_plan_rejects_bad_offsets(hip):
                                out.data_ptr(), out.data_ptr(), out.data_ptr(), go.data_ptr(), 4,
                                ctypes.byref(rc))
     assert not p and rc.value == -3
+
+
+def test_batch_wide_queries_edges(hip):
+    """Queries wider than 4096 keys go through the by-key-range path
+    (16384-key units): a wide query with no rows, one whose last unit holds
+    a single key, values outside the packed range in multi-unit queries, a
+    wide query's key outside its range (-2) and a capacity below the group
+    count (-7); both modes against the oracle."""
+    import torch
+    rng = np.random.default_rng(5)
+    cards = [4097, 16384 * 3 + 1, 70000, 5000, 100, 250000]
+    rows = [3000, 40000, 0, 9000, 500, 120000]
+    ks, vs = [], []
+    for c, n in zip(cards, rows):
+        k = rng.integers(0, c, n).astype(np.int32)
+        if c == 16384 * 3 + 1 and n:
+            k[:3] = [c - 1, c - 1, 0]  # the last unit's only key
+        ks.append(k)
+        v = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
+        pick = rng.random(n) < 0.05
+        v[pick] = rng.integers(-(2 ** 31), 2 ** 31 - 1, int(pick.sum()), dtype=np.int64).astype(
+            np.int32)
+        vs.append(v)
+    k, v = np.concatenate(ks), np.concatenate(vs)
+    row_off = np.concatenate([[0], np.cumsum(rows)]).astype(np.int64)
+    card = np.array(cards, np.int32)
+    ref = ox.group_batch(k, v, row_off, card)
+    b, _, _ = _plan(torch, k, v, row_off, card)
+    info = b.info()
+    assert info["units"] == 1 + 4 + 5 + 1 + 1 + 16 and info["wide_tiles"] > 0
+    for _ in range(2):
+        _check(b.run(0), ref)
+        _check(b.run(1), ref)
+    b.close()
+    kb = k.copy()
+    kb[row_off[5] + 7] = 250000  # the wide last query's range is [0, 250000)
+    assert ox.group_batch(kb, v, row_off, card) == -2
+    b, _, _ = _plan(torch, kb, v, row_off, card)
+    assert b.run(0)[0] == -2 and b.run(1)[0] == -2
+    b.close()
+    b, _, _ = _plan(torch, k, v, row_off, card, cap=int(ref[0].size) - 1)
+    assert b.run(0)[0] == -7 and b.run(1)[0] == -7
+    b.close()
+
+
+def test_batch_plan_rejects_card_over_2p24(hip):
+    import ctypes
+    import torch
+    L = native.lib("batch")
+    kt = torch.zeros(4, dtype=torch.int32, device="cuda")
+    ro = np.array([0, 2, 4], np.int64)
+    out = torch.zeros(4, dtype=torch.int64, device="cuda")
+    go = torch.zeros(3, dtype=torch.int64, device="cuda")
+    rc = ctypes.c_int32(0)
+    for cd, ok in ((np.array([3, 1 << 24], np.int32), True),
+                   (np.array([3, (1 << 24) + 1], np.int32), False)):
+        p = L.nfh_group_batch_plan(kt.data_ptr(), kt.data_ptr(), ro.ctypes.data, cd.ctypes.data,
+                                   2, out.data_ptr(), out.data_ptr(), out.data_ptr(),
+                                   go.data_ptr(), 4, ctypes.byref(rc))
+        assert bool(p) == ok and rc.value == (0 if ok else -3)
+        if p:
+            assert L.nfh_group_batch_run(p, 0) == 2  # keys 0 in both queries
+            L.nfh_group_batch_free(p)
