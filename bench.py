@@ -81,7 +81,9 @@ def kernel_table(L):
 # kernel per launch of the scope, dispatch-weighted over the rows; "sum":
 # every row runs once per launch of the scope.
 TRAFFIC_ROWS = {
-    "unique_mark": ("groupby_unique_zipf_2p28.txt", ["k_mark<true>"], "mean"),
+    "unique_part": ("groupby_unique_zipf_2p28.txt", ["k_umark_part"], "mean"),
+    "unique_inv_dense": ("groupby_unique_zipf_2p28.txt", ["k_inv<true>"], "mean"),
+    "gp_hist": ("groupby_unique_zipf_2p28.txt", ["k_gp_hist"], "mean"),
     "gp_part_first": ("groupby_unique_zipf_2p28.txt", ["k_gp_part<true, true, true>"], "mean"),
     "radix_onesweep": ("argsort_uniform_2p28.txt",
                        ["k_onesweep<int, true, false, true, true, RadixDigit>",
