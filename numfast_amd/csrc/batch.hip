@@ -22,13 +22,16 @@
 //     slots (one packed u64 per key, zeroed per run) -- the rows-per-item
 //     and grid of this pass are autotuned;
 //   ranged (card > BT_LDS_SLOTS): the key range is cut into units of
-//     BT_RANGE keys.  A partition pass turns each 8192-row tile into
-//     32-bit records (key within its unit : biased value) grouped by unit
-//     (tile-local LDS ranks), with one (start, count) entry per (tile,
-//     unit); a unit's records are then all in those entries.  No slot array
-//     and no device-scope atomics: the card-sized dense slots of the first
-//     form (3.6e8 slots, zero-filled and scanned for 2.7e8 rows, with a
-//     global atomic per row of every wide query) took 13.7 ms.
+//     BT_RANGE keys.  A histogram pass counts each 8192-row tile's rows per
+//     unit (one entry per (tile, unit), laid out unit-major within the
+//     query), a scan of the entries gives every (tile, unit) its place, and
+//     a partition pass writes each row as a 32-bit record (key within its
+//     unit : biased value) there, so each unit's records are contiguous.
+//     No slot array and no device-scope atomics: the card-sized dense slots
+//     of the first form (3.6e8 slots, zero-filled and scanned for 2.7e8
+//     rows, with a global atomic per row of every wide query) took 13.7 ms;
+//     records left in per-tile segments (read through one entry per tile)
+//     made the unit passes latency-bound (2.5 ms).
 // Output: every query has at least one unit (a small query one, a ranged
 // query ceil(card / BT_RANGE)); a count pass gives each unit's present
 // groups (small: present slots; ranged: an LDS presence bitmap over its
@@ -55,15 +58,22 @@ constexpr int BT_LDS_SLOTS = 4096;  // small queries: up to this card
 constexpr int BT_SHIFT = 40;
 constexpr u64 BT_MASK = (u64(1) << BT_SHIFT) - 1;
 constexpr uint32_t BT_MAX_ROWS = 1u << 24;  // rows per query for the packed aggregates
-constexpr int BT_RANGE_BITS = 14;
-constexpr uint32_t BT_RANGE = 1u << BT_RANGE_BITS;  // keys per ranged unit (128 KB of LDS)
-constexpr uint32_t BT_MAX_RANGES = 1024;            // card <= 2^24
+constexpr int BT_RANGE_BITS = 12;
+constexpr uint32_t BT_RANGE = 1u << BT_RANGE_BITS;  // keys per ranged unit (32 KB of LDS)
+constexpr uint32_t BT_MAX_RANGES = 1u << (24 - BT_RANGE_BITS);  // card <= 2^24
 constexpr int BT_PBLOCK = 512;                      // partition pass
 constexpr int BT_PIPT = 16;
 constexpr uint32_t BT_PTILE = uint32_t(BT_PBLOCK) * BT_PIPT;  // 8192 rows
 constexpr int BT_CBLOCK = 256;   // count pass
-constexpr int BT_EBLOCK = 1024;  // emit pass
+constexpr int BT_EBLOCK = 256;   // emit pass (five workgroups per CU)
+static_assert(uint32_t(BT_LDS_SLOTS) <= BT_RANGE, "a small query's slots are staged in LDS");
 constexpr int BT_SCAN = 1024;    // unit scan (one workgroup)
+// A ranged unit with more than BT_SLICE records is cut into slices that
+// aggregate on separate workgroups (partial presence bitmaps / packed
+// aggregates in staging, merged per unit by a second launch): one unit of
+// 6e5 records on one workgroup, hot keys and all, set the length of the
+// whole pass.
+constexpr uint32_t BT_SLICE = 32768;
 
 struct BtItem {  // accumulate work item of a small query
   uint32_t q;
@@ -72,7 +82,9 @@ struct BtItem {  // accumulate work item of a small query
 struct BtTile {  // partition tile of a ranged query
   uint32_t q;
   uint32_t r0, r1;  // row range
-  uint32_t e0;      // entry of (this tile, unit 0); units follow
+  uint32_t p0;      // entry of (this tile, unit 0); unit j's entry is p0 + j * nt
+  uint32_t nt;      // tiles of the query
+  uint32_t pad;
 };
 struct BtQuery {
   u64 slot0;      // small: first slot of the query
@@ -82,8 +94,8 @@ struct BtQuery {
 struct BtUnit {  // keys [key0, key0 + nkeys) of query q, in output order
   uint32_t q, j;
   uint32_t key0, nkeys;
-  uint32_t t0, t1;  // ranged: the query's tiles
-  uint32_t e0, nr;  // ranged: entry of (tile t0, unit 0); units per tile
+  uint32_t p0, nt;  // ranged: entries [p0, p0 + nt) (its records follow off[p0] .. off[p0 + nt])
+  u64 slot0;        // small: the query's first slot; ranged: ~0
 };
 struct BtOver {  // a row whose value is outside [-2^15, 2^15)
   uint32_t q, key;
@@ -93,11 +105,13 @@ struct BtOver {  // a row whose value is outside [-2^15, 2^15)
 // device counters
 struct BtMeta {
   uint32_t work;     // next work item
-  uint32_t pad;
+  uint32_t nmulti;   // units of several slices among the units in flight
   uint32_t nover;    // overflow entries
   uint32_t bad_row;  // first row with a key out of range (0xFFFFFFFF: none)
   u64 base;          // group offset before the first unit (0 when batched)
   u64 total;         // groups written
+  uint32_t nslices;  // slices of the units in flight
+  uint32_t pad;
 };
 
 __device__ __forceinline__ uint32_t bt_biased(int32_t v, bool* over) {
@@ -159,19 +173,109 @@ __global__ __launch_bounds__(BT_BLOCK) void k_bt_accumulate(
   }
 }
 
-// Ranged queries: one workgroup per tile tiles[t_first + blockIdx.x].  Each
-// row becomes the record (key & (BT_RANGE - 1)) << 16 | biased value,
-// grouped by unit key >> BT_RANGE_BITS within the tile's own row range:
-// rec[r0 + start_j + rank] (ranks from LDS counters, in no particular
-// order -- aggregation does not depend on it), entry[e0 + j] = (r0 +
-// start_j, count_j).  A key outside [0, card) is dropped and marks the run.
+// Ranged queries, histogram: one workgroup per tile tiles[t_first +
+// blockIdx.x] counts its rows per unit into ecnt[p0 + j * nt].  A key
+// outside [0, card) is not counted and marks the run.
+__global__ __launch_bounds__(BT_PBLOCK) void k_bt_hist(const int32_t* __restrict__ keys,
+                                                       const BtTile* __restrict__ tiles,
+                                                       uint32_t t_first,
+                                                       const BtQuery* __restrict__ qs,
+                                                       uint32_t* __restrict__ ecnt,
+                                                       BtMeta* __restrict__ meta) {
+  __shared__ uint32_t s_cnt[BT_MAX_RANGES];
+  const BtTile t = tiles[t_first + blockIdx.x];
+  const BtQuery qd = qs[t.q];
+  const uint32_t nr = qd.nr, card = qd.card;
+  for (uint32_t j = threadIdx.x; j < nr; j += BT_PBLOCK) s_cnt[j] = 0;
+  __syncthreads();
+  int32_t k[BT_PIPT];
+#pragma unroll
+  for (int u = 0; u < BT_PIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * BT_PBLOCK + threadIdx.x;
+    k[u] = r < t.r1 ? keys[r] : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < BT_PIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * BT_PBLOCK + threadIdx.x;
+    if (r >= t.r1) continue;
+    if (uint32_t(k[u]) >= card)
+      atomicMin(&meta->bad_row, r);
+    else
+      atomicAdd(&s_cnt[uint32_t(k[u]) >> BT_RANGE_BITS], 1u);
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nr; j += BT_PBLOCK) ecnt[t.p0 + j * t.nt] = s_cnt[j];
+}
+
+// Entry scan in three steps over chunks of BT_ECHUNK entries: chunk sums,
+// one workgroup's scan of the sums, then each chunk's own scan from its
+// base.  off[p] = base + records before entry p; off[np] = base + all
+// records.
+constexpr uint32_t BT_ECHUNK = uint32_t(BT_SCAN) * 4;
+__global__ __launch_bounds__(BT_SCAN) void k_bt_esum(const uint32_t* __restrict__ ecnt,
+                                                     uint32_t np, uint32_t* __restrict__ csum) {
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  const uint32_t b = blockIdx.x * BT_ECHUNK;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t p = b + threadIdx.x + uint32_t(i) * BT_SCAN;
+    c += p < np ? ecnt[p] : 0u;
+  }
+  uint32_t tot;
+  (void)block_excl_scan<BT_SCAN / WAVE>(c, s_tmp, &tot);
+  if (threadIdx.x == 0) csum[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(BT_SCAN) void k_bt_etop(uint32_t* __restrict__ csum, uint32_t nc,
+                                                     uint32_t* __restrict__ off, uint32_t np,
+                                                     uint32_t base) {
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = base;
+  __syncthreads();
+  for (uint32_t b = 0; b < nc; b += BT_SCAN) {
+    const uint32_t c = b + threadIdx.x;
+    const uint32_t v = c < nc ? csum[c] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BT_SCAN / WAVE>(v, s_tmp, &tot);
+    if (c < nc) csum[c] = s_carry + ex;  // in place: chunk base
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[np] = s_carry;
+}
+__global__ __launch_bounds__(BT_SCAN) void k_bt_escan(const uint32_t* __restrict__ ecnt,
+                                                      uint32_t np,
+                                                      const uint32_t* __restrict__ cbase,
+                                                      uint32_t* __restrict__ off) {
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  const uint32_t b = blockIdx.x * BT_ECHUNK + threadIdx.x * 4;  // blocked: 4 per thread
+  uint32_t c[4], sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c[i] = b + i < np ? ecnt[b + i] : 0u;
+    sum += c[i];
+  }
+  uint32_t tot;
+  uint32_t run = cbase[blockIdx.x] + block_excl_scan<BT_SCAN / WAVE>(sum, s_tmp, &tot);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (b + i < np) off[b + i] = run;
+    run += c[i];
+  }
+}
+
+// Ranged queries, partition: one workgroup per tile.  Each row becomes the
+// record (key & (BT_RANGE - 1)) << 16 | biased value at off[p0 + j * nt] +
+// its rank among the tile's rows of unit j (LDS counters: in no particular
+// order, which aggregation does not depend on).
 __global__ __launch_bounds__(BT_PBLOCK) void k_bt_part(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const BtTile* __restrict__ tiles, uint32_t t_first, const BtQuery* __restrict__ qs,
-    uint32_t* __restrict__ rec, uint2* __restrict__ entry, BtOver* __restrict__ over,
+    const uint32_t* __restrict__ off, uint32_t* __restrict__ rec, BtOver* __restrict__ over,
     BtMeta* __restrict__ meta) {
   __shared__ uint32_t s_cnt[BT_MAX_RANGES];
-  __shared__ uint32_t s_tmp[BT_PBLOCK / WAVE];
   const BtTile t = tiles[t_first + blockIdx.x];
   const BtQuery qd = qs[t.q];
   const uint32_t nr = qd.nr, card = qd.card;
@@ -189,86 +293,171 @@ __global__ __launch_bounds__(BT_PBLOCK) void k_bt_part(
   for (int u = 0; u < BT_PIPT; ++u) {
     const uint32_t r = t.r0 + uint32_t(u) * BT_PBLOCK + threadIdx.x;
     rc[u] = 0xFFFFFFFFu;  // no record
-    if (r >= t.r1) continue;
-    if (uint32_t(k[u]) >= card) {
-      atomicMin(&meta->bad_row, r);
-      continue;
-    }
+    if (r >= t.r1 || uint32_t(k[u]) >= card) continue;
     bool ov;
     const uint32_t b = bt_biased(x[u], &ov);
     if (ov) over[atomicAdd(&meta->nover, 1u)] = BtOver{t.q, uint32_t(k[u]), x[u], 0u};
     rc[u] = ((uint32_t(k[u]) & (BT_RANGE - 1)) << 16) | b;
     loc[u] = atomicAdd(&s_cnt[uint32_t(k[u]) >> BT_RANGE_BITS], 1u);
   }
+  // the unit bases of this tile
   __syncthreads();
-  // unit starts: exclusive scan of the counts, two units per thread
-  const uint32_t j0 = 2 * threadIdx.x;
-  const uint32_t c0 = j0 < nr ? s_cnt[j0] : 0u, c1 = j0 + 1 < nr ? s_cnt[j0 + 1] : 0u;
-  uint32_t tot;
-  const uint32_t ex = block_excl_scan<BT_PBLOCK / WAVE>(c0 + c1, s_tmp, &tot);
-  if (j0 < nr) entry[t.e0 + j0] = make_uint2(t.r0 + ex, c0);
-  if (j0 + 1 < nr) entry[t.e0 + j0 + 1] = make_uint2(t.r0 + ex + c0, c1);
-  __syncthreads();  // every count read before the starts overwrite them
-  if (j0 < nr) s_cnt[j0] = ex;
-  if (j0 + 1 < nr) s_cnt[j0 + 1] = ex + c0;
+  for (uint32_t j = threadIdx.x; j < nr; j += BT_PBLOCK) s_cnt[j] = off[t.p0 + j * t.nt];
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < BT_PIPT; ++u)
-    if (rc[u] != 0xFFFFFFFFu)
-      rec[t.r0 + s_cnt[uint32_t(k[u]) >> BT_RANGE_BITS] + loc[u]] = rc[u];
+    if (rc[u] != 0xFFFFFFFFu) rec[s_cnt[uint32_t(k[u]) >> BT_RANGE_BITS] + loc[u]] = rc[u];
 }
 
-// The records of ranged unit `un`, one wave per tile entry, four in flight
-// per lane: f(record) for each.
-template <int NT, class F>
-__device__ __forceinline__ void bt_for_records(const BtUnit& un, const uint32_t* __restrict__ rec,
-                                               const uint2* __restrict__ entry, F f) {
-  constexpr uint32_t NW = NT / WAVE;
-  const uint32_t lane = uint32_t(lane_id());
-  for (uint32_t t = un.t0 + uint32_t(wave_id()); t < un.t1; t += NW) {
-    const uint2 e = entry[un.e0 + (t - un.t0) * un.nr + un.j];
-    for (uint32_t i = lane; i < e.y; i += 4 * WAVE) {
-      uint32_t r[4];
+// The records [a, e) of a ranged unit (contiguous), U per thread in
+// flight: f(record) for each.
+template <int NT, int U, class F>
+__device__ __forceinline__ void bt_for_records(uint32_t a, uint32_t e,
+                                               const uint32_t* __restrict__ rec, F f) {
+  for (uint32_t i = a + threadIdx.x; i < e; i += U * NT) {
+    uint32_t r[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t ii = i + uint32_t(u) * WAVE;
-        r[u] = ii < e.y ? rec[e.x + ii] : 0xFFFFFFFFu;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (r[u] != 0xFFFFFFFFu) f(r[u]);
+    for (int u = 0; u < U; ++u) {
+      const uint32_t ii = i + uint32_t(u) * NT;
+      r[u] = ii < e ? __builtin_nontemporal_load(rec + ii) : 0xFFFFFFFFu;
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] != 0xFFFFFFFFu) f(r[u]);
   }
 }
 
-// Present groups of units [u_first, u_first + gridDim.x): small units count
-// their present slots, ranged units the distinct keys of their records.
-__global__ __launch_bounds__(BT_CBLOCK) void k_bt_count(
-    const BtUnit* __restrict__ units, uint32_t u_first, const BtQuery* __restrict__ qs,
-    const u64* __restrict__ slots, const uint32_t* __restrict__ rec,
-    const uint2* __restrict__ entry, uint32_t* __restrict__ ucount) {
-  __shared__ uint32_t s_bits[BT_RANGE / 32];
-  __shared__ uint32_t s_tmp[BT_CBLOCK / WAVE];
-  const uint32_t u = u_first + blockIdx.x;
+// Slices of units [u0, u1): a small unit or a ranged unit of up to
+// BT_SLICE records is one slice, a larger one ceil(records / BT_SLICE).
+// Two launches over chunks of BT_SCAN units: k_bt_slice_sums leaves each
+// chunk's totals (slices, staging slots, split units) in csl[3 * c ..];
+// k_bt_slice_scan scans within its chunk from the sum of the chunks before
+// it: sstart[u] = first slice (from 0), sstart[u1] = all slices,
+// slice_unit[sl] = the unit of slice sl, pstart[u] = first staging slot of
+// a unit of several slices, multi[] = those units in order.
+__device__ __forceinline__ uint32_t bt_unit_slices(const BtUnit* __restrict__ units, uint32_t u,
+                                                   uint32_t u1, const uint32_t* __restrict__ off) {
+  if (u >= u1) return 0;
   const BtUnit un = units[u];
-  const BtQuery qd = qs[un.q];
+  if (un.slot0 != ~0ull) return 1;
+  const uint32_t R = off[un.p0 + un.nt] - off[un.p0];
+  return R > BT_SLICE ? (R + BT_SLICE - 1) / BT_SLICE : 1u;
+}
+__global__ __launch_bounds__(BT_SCAN) void k_bt_slice_sums(const BtUnit* __restrict__ units,
+                                                           uint32_t u0, uint32_t u1,
+                                                           const uint32_t* __restrict__ off,
+                                                           uint32_t* __restrict__ csl) {
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  const uint32_t S = bt_unit_slices(units, u0 + blockIdx.x * BT_SCAN + threadIdx.x, u1, off);
+  uint32_t ts, tp, tm;
+  (void)block_excl_scan<BT_SCAN / WAVE>(S, s_tmp, &ts);
+  (void)block_excl_scan<BT_SCAN / WAVE>(S > 1 ? S : 0u, s_tmp, &tp);
+  (void)block_excl_scan<BT_SCAN / WAVE>(S > 1 ? 1u : 0u, s_tmp, &tm);
+  if (threadIdx.x == 0) {
+    csl[3 * blockIdx.x] = ts;
+    csl[3 * blockIdx.x + 1] = tp;
+    csl[3 * blockIdx.x + 2] = tm;
+  }
+}
+__global__ __launch_bounds__(BT_SCAN) void k_bt_slice_scan(
+    const BtUnit* __restrict__ units, uint32_t u0, uint32_t u1, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ csl, uint32_t* __restrict__ sstart,
+    uint32_t* __restrict__ pstart, uint32_t* __restrict__ slice_unit,
+    uint32_t* __restrict__ multi, BtMeta* __restrict__ meta) {
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  __shared__ uint32_t s_base[3];
+  if (threadIdx.x < 3) {
+    uint32_t b = 0;
+    for (uint32_t c = 0; c < blockIdx.x; ++c) b += csl[3 * c + threadIdx.x];
+    s_base[threadIdx.x] = b;
+  }
+  __syncthreads();
+  const uint32_t u = u0 + blockIdx.x * BT_SCAN + threadIdx.x;
+  const uint32_t S = bt_unit_slices(units, u, u1, off);
+  const uint32_t P = S > 1 ? S : 0u, M = S > 1 ? 1u : 0u;
+  uint32_t ts, tp, tm;
+  const uint32_t xs = s_base[0] + block_excl_scan<BT_SCAN / WAVE>(S, s_tmp, &ts);
+  const uint32_t xp = s_base[1] + block_excl_scan<BT_SCAN / WAVE>(P, s_tmp, &tp);
+  const uint32_t xm = s_base[2] + block_excl_scan<BT_SCAN / WAVE>(M, s_tmp, &tm);
+  if (u < u1) {
+    sstart[u] = xs;
+    pstart[u] = xp;
+    for (uint32_t i = 0; i < S; ++i) slice_unit[xs + i] = u;
+    if (M) multi[xm] = u;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    sstart[u1] = s_base[0] + ts;
+    meta->nslices = s_base[0] + ts;
+    meta->nmulti = s_base[2] + tm;
+  }
+}
+
+// Count pass, one workgroup per slice (workgroups past the last slice
+// exit): present groups of a one-slice unit go to ucount (small units count
+// their present slots, ranged units the distinct keys of their records); a
+// slice of a larger unit leaves its presence bitmap in cstage.
+__global__ __launch_bounds__(BT_CBLOCK) void k_bt_count(
+    const BtUnit* __restrict__ units, const uint32_t* __restrict__ sstart,
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ slice_unit,
+    const u64* __restrict__ slots, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ rec, uint32_t* __restrict__ cstage,
+    uint32_t* __restrict__ ucount, const BtMeta* __restrict__ meta) {
+  constexpr uint32_t W = BT_RANGE / 32;
+  __shared__ uint32_t s_bits[W];
+  __shared__ uint32_t s_tmp[BT_CBLOCK / WAVE];
+  const uint32_t sl = blockIdx.x;
+  if (sl >= meta->nslices) return;  // uniform
+  const uint32_t u = slice_unit[sl];
+  const uint32_t i = sl - sstart[u], S = sstart[u + 1] - sstart[u];
+  const BtUnit un = units[u];
   uint32_t c = 0;
-  if (qd.nr == 0) {
-    for (uint32_t i = threadIdx.x; i < un.nkeys; i += BT_CBLOCK)
-      c += (slots[qd.slot0 + i] >> BT_SHIFT) != 0 ? 1u : 0u;
+  if (un.slot0 != ~0ull) {
+    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_CBLOCK)
+      c += (slots[un.slot0 + k] >> BT_SHIFT) != 0 ? 1u : 0u;
   } else {
-    for (uint32_t w = threadIdx.x; w < BT_RANGE / 32; w += BT_CBLOCK) s_bits[w] = 0;
+    const uint32_t a0 = off[un.p0], e0 = off[un.p0 + un.nt];
+    const uint32_t a = a0 + i * BT_SLICE, e = S > 1 ? min(a + BT_SLICE, e0) : e0;
+    for (uint32_t w = threadIdx.x; w < W; w += BT_CBLOCK) s_bits[w] = 0;
     __syncthreads();
-    bt_for_records<BT_CBLOCK>(un, rec, entry, [&](uint32_t r) {
+    bt_for_records<BT_CBLOCK, 16>(a, e, rec, [&](uint32_t r) {
       const uint32_t kl = r >> 16;
       atomicOr(&s_bits[kl >> 5], 1u << (kl & 31u));
     });
     __syncthreads();
-    for (uint32_t w = threadIdx.x; w < BT_RANGE / 32; w += BT_CBLOCK) c += __popc(s_bits[w]);
+    if (S > 1) {
+      uint32_t* dst = cstage + size_t(pstart[u] + i) * W;
+      for (uint32_t w = threadIdx.x; w < W; w += BT_CBLOCK) dst[w] = s_bits[w];
+      return;  // uniform
+    }
+    for (uint32_t w = threadIdx.x; w < W; w += BT_CBLOCK) c += __popc(s_bits[w]);
   }
   uint32_t tot;
   (void)block_excl_scan<BT_CBLOCK / WAVE>(c, s_tmp, &tot);
   if (threadIdx.x == 0) ucount[u] = tot;
+}
+
+// Units of several slices (one workgroup each, past the last: exit): the
+// OR of their slices' bitmaps.
+__global__ __launch_bounds__(BT_CBLOCK) void k_bt_cmerge(
+    const uint32_t* __restrict__ multi, const uint32_t* __restrict__ sstart,
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ cstage,
+    uint32_t* __restrict__ ucount, const BtMeta* __restrict__ meta) {
+  constexpr uint32_t W = BT_RANGE / 32;
+  __shared__ uint32_t s_tmp[BT_CBLOCK / WAVE];
+  if (blockIdx.x < meta->nmulti) {  // uniform
+    const uint32_t u = multi[blockIdx.x];
+    const uint32_t S = sstart[u + 1] - sstart[u];
+    const uint32_t* src = cstage + size_t(pstart[u]) * W;
+    uint32_t c = 0;
+    for (uint32_t w = threadIdx.x; w < W; w += BT_CBLOCK) {
+      uint32_t x = 0;
+      for (uint32_t i = 0; i < S; ++i) x |= src[size_t(i) * W + w];
+      c += __popc(x);
+    }
+    uint32_t tot;
+    (void)block_excl_scan<BT_CBLOCK / WAVE>(c, s_tmp, &tot);
+    if (threadIdx.x == 0) ucount[u] = tot;
+  }
 }
 
 // Exclusive scan of the unit counts of [u0, u1) from meta->base (one
@@ -315,49 +504,35 @@ __global__ __launch_bounds__(BT_SCAN) void k_bt_scan(const BtUnit* __restrict__ 
   }
 }
 
-// Groups of units [u_first, u_first + gridDim.x) in key order at their
-// offsets (positions at or past cap are not written).  Ranged units first
-// aggregate their records in LDS; small units read their slots and leave
-// each present slot's output position in it (for the overflow pass).
-// Each wave compacts one contiguous segment of the unit's keys by ballot.
-__global__ __launch_bounds__(BT_EBLOCK) void k_bt_emit(
-    const BtUnit* __restrict__ units, uint32_t u_first, const BtQuery* __restrict__ qs,
-    u64* __restrict__ slots, const uint32_t* __restrict__ rec, const uint2* __restrict__ entry,
-    const u64* __restrict__ uoff, int64_t* __restrict__ ukeys, int64_t* __restrict__ counts,
-    int64_t* __restrict__ sums, u64 cap) {
-  constexpr uint32_t NW = BT_EBLOCK / WAVE;
-  __shared__ u64 s_acc[BT_RANGE];
-  __shared__ uint32_t s_wc[NW];
-  const uint32_t u = u_first + blockIdx.x;
-  const BtUnit un = units[u];
-  const BtQuery qd = qs[un.q];
-  const bool small = qd.nr == 0;
-  if (!small) {
-    for (uint32_t i = threadIdx.x; i < un.nkeys; i += BT_EBLOCK) s_acc[i] = 0;
-    __syncthreads();
-    bt_for_records<BT_EBLOCK>(un, rec, entry, [&](uint32_t r) {
-      atomicAdd(&s_acc[r >> 16], (u64(1) << BT_SHIFT) + u64(r & 0xFFFFu));
-    });
-    __syncthreads();
-  }
-  auto packed = [&](uint32_t i) -> u64 {
-    if (i >= un.nkeys) return 0ull;
-    return small ? slots[qd.slot0 + i] : s_acc[i];
-  };
+// Groups of unit `un` in key order from pos (positions at or past cap are
+// not written); packed(k) is key k's aggregate.  Small units leave each
+// present slot's output position in it (for the overflow pass).  Each wave
+// compacts one contiguous segment of the unit's keys by ballot.
+template <class F>
+__device__ __forceinline__ void bt_compact(const BtUnit& un, u64 pos, F packed, uint32_t* s_wc,
+                                           u64* __restrict__ slots, int64_t* __restrict__ ukeys,
+                                           int64_t* __restrict__ counts,
+                                           int64_t* __restrict__ sums, u64 cap) {
+  // wave w holds keys [w * PER * 64, (w + 1) * PER * 64), all PER reads in
+  // flight at once and kept in registers for both passes
+  constexpr int PER = int(BT_RANGE) / BT_EBLOCK;
+  const bool small = un.slot0 != ~0ull;
   const uint32_t wave = uint32_t(wave_id()), lane = uint32_t(lane_id());
-  const uint32_t seg = (un.nkeys + NW * WAVE - 1) / (NW * WAVE) * WAVE;  // keys per wave
-  const uint32_t k0 = wave * seg;
+  const uint32_t k0 = wave * uint32_t(PER * WAVE);
+  u64 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) v[j] = packed(k0 + uint32_t(j * WAVE) + lane);
   uint32_t c = 0;
-  for (uint32_t i = k0; i < k0 + seg; i += WAVE)
-    c += uint32_t(__popcll(__ballot((packed(i + lane) >> BT_SHIFT) != 0)));
+#pragma unroll
+  for (int j = 0; j < PER; ++j) c += uint32_t(__popcll(__ballot((v[j] >> BT_SHIFT) != 0)));
   if (lane == 0) s_wc[wave] = c;
   __syncthreads();
-  u64 pos = uoff[u];
   for (uint32_t w = 0; w < wave; ++w) pos += s_wc[w];
   const uint64_t lt = lanemask_lt();
-  for (uint32_t i = k0; i < k0 + seg; i += WAVE) {
-    const uint32_t key = i + lane;
-    const u64 pk = packed(key);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t key = k0 + uint32_t(j * WAVE) + lane;
+    const u64 pk = v[j];
     const u64 cnt = pk >> BT_SHIFT;
     const uint64_t b = __ballot(cnt != 0);
     if (cnt != 0) {
@@ -367,9 +542,82 @@ __global__ __launch_bounds__(BT_EBLOCK) void k_bt_emit(
         counts[p] = int64_t(cnt);
         sums[p] = int64_t(pk & BT_MASK) - int64_t(cnt) * 32768;
       }
-      if (small) slots[qd.slot0 + key] = p;
+      if (small) slots[un.slot0 + key] = p;
     }
     pos += u64(__popcll(b));
+  }
+  __syncthreads();  // s_wc is reused by the caller's next unit
+}
+
+// Emit pass, one workgroup per slice (past the last: exit): a one-slice
+// unit is aggregated (ranged: its records in LDS) and written at its
+// offset; a slice of a larger unit leaves its packed aggregates in estage.
+__global__ __launch_bounds__(BT_EBLOCK) void k_bt_emit(
+    const BtUnit* __restrict__ units, const uint32_t* __restrict__ sstart,
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ slice_unit,
+    u64* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ rec,
+    const u64* __restrict__ uoff, u64* __restrict__ estage, int64_t* __restrict__ ukeys,
+    int64_t* __restrict__ counts, int64_t* __restrict__ sums, u64 cap,
+    const BtMeta* __restrict__ meta) {
+  __shared__ u64 s_acc[BT_RANGE];
+  __shared__ uint32_t s_wc[BT_EBLOCK / WAVE];
+  const uint32_t sl = blockIdx.x;
+  if (sl >= meta->nslices) return;  // uniform
+  const uint32_t u = slice_unit[sl];
+  const uint32_t i = sl - sstart[u], S = sstart[u + 1] - sstart[u];
+  const BtUnit un = units[u];
+  const bool small = un.slot0 != ~0ull;
+  const u64 pos = uoff[u];
+  if (small) {
+    // staged in LDS: the compaction then reads LDS only (a load that may be
+    // global or LDS compiles to a flat load, whose wait also waits for the
+    // group stores in flight -- 0.9 ms of this pass)
+    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) s_acc[k] = slots[un.slot0 + k];
+    __syncthreads();
+  } else {
+    const uint32_t a0 = off[un.p0], e0 = off[un.p0 + un.nt];
+    const uint32_t a = a0 + i * BT_SLICE, e = S > 1 ? min(a + BT_SLICE, e0) : e0;
+    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) s_acc[k] = 0;
+    __syncthreads();
+    bt_for_records<BT_EBLOCK, 8>(a, e, rec, [&](uint32_t r) {
+      atomicAdd(&s_acc[r >> 16], (u64(1) << BT_SHIFT) + u64(r & 0xFFFFu));
+    });
+    __syncthreads();
+    if (S > 1) {
+      u64* dst = estage + size_t(pstart[u] + i) * BT_RANGE;
+      for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) dst[k] = s_acc[k];
+      return;  // uniform
+    }
+  }
+  bt_compact(
+      un, pos, [&](uint32_t k) -> u64 { return k < un.nkeys ? s_acc[k] : 0ull; }, s_wc, slots,
+      ukeys, counts, sums, cap);
+}
+
+// Units of several slices: the sum of their slices' packed aggregates,
+// then the groups in key order.
+__global__ __launch_bounds__(BT_EBLOCK) void k_bt_emerge(
+    const BtUnit* __restrict__ units, const uint32_t* __restrict__ multi,
+    const uint32_t* __restrict__ sstart, const uint32_t* __restrict__ pstart,
+    const u64* __restrict__ uoff, const u64* __restrict__ estage, u64* __restrict__ slots,
+    int64_t* __restrict__ ukeys, int64_t* __restrict__ counts, int64_t* __restrict__ sums,
+    u64 cap, const BtMeta* __restrict__ meta) {
+  __shared__ u64 s_acc[BT_RANGE];
+  __shared__ uint32_t s_wc[BT_EBLOCK / WAVE];
+  if (blockIdx.x < meta->nmulti) {  // uniform
+    const uint32_t u = multi[blockIdx.x];
+    const uint32_t S = sstart[u + 1] - sstart[u];
+    const BtUnit un = units[u];
+    const u64* src = estage + size_t(pstart[u]) * BT_RANGE;
+    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) {
+      u64 x = 0;
+      for (uint32_t i = 0; i < S; ++i) x += src[size_t(i) * BT_RANGE + k];
+      s_acc[k] = x;
+    }
+    __syncthreads();
+    bt_compact(
+        un, uoff[u], [&](uint32_t k) -> u64 { return k < un.nkeys ? s_acc[k] : 0ull; }, s_wc,
+        slots, ukeys, counts, sums, cap);
   }
 }
 
@@ -450,7 +698,17 @@ struct BatchPlan {
   BtQuery* qs = nullptr;
   u64* slots = nullptr;
   uint32_t* rec = nullptr;
-  uint2* entry = nullptr;
+  uint32_t* ecnt = nullptr;  // rows per (tile, unit) entry, unit-major per query
+  uint32_t* eoff = nullptr;  // records before each entry (+1)
+  uint32_t* csum = nullptr;  // entry-scan chunk sums / bases
+  uint32_t* sstart = nullptr;  // per unit (+1): first slice
+  uint32_t* pstart = nullptr;  // per unit: first staging slot (units of several slices)
+  uint32_t* cstage = nullptr;  // presence bitmaps of slices
+  u64* estage = nullptr;       // packed aggregates of slices
+  uint32_t* slice_unit = nullptr;  // unit of each slice
+  uint32_t* multi = nullptr;       // units of several slices
+  uint32_t* csl = nullptr;         // slice-scan chunk totals
+  std::vector<uint32_t> p_start;  // per query (+1): first entry
   uint32_t* ucount = nullptr;
   u64* uoff = nullptr;
   BtOver* over = nullptr;
@@ -499,6 +757,61 @@ static void build_items(BatchPlan& P, uint32_t R) {
   P.rows_per_item = R;
 }
 
+// Histogram, entry scan (from base) and partition of tiles [t0, t1), whose
+// entries are [p0, p1).
+static void enqueue_ranged(BatchPlan& P, uint32_t t0, uint32_t t1, uint32_t p0, uint32_t p1,
+                           uint32_t base) {
+  if (t1 == t0) return;
+  hipLaunchKernelGGL(k_bt_hist, dim3(t1 - t0), dim3(BT_PBLOCK), 0, P.s, P.keys, P.tiles, t0,
+                     P.qs, P.ecnt, P.meta);
+  NFH_LAUNCH_CHECK();
+  const uint32_t np = p1 - p0, nc = (np + BT_ECHUNK - 1) / BT_ECHUNK;
+  hipLaunchKernelGGL(k_bt_esum, dim3(nc), dim3(BT_SCAN), 0, P.s, P.ecnt + p0, np, P.csum);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_etop, dim3(1), dim3(BT_SCAN), 0, P.s, P.csum, nc, P.eoff + p0, np,
+                     base);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_escan, dim3(nc), dim3(BT_SCAN), 0, P.s, P.ecnt + p0, np, P.csum,
+                     P.eoff + p0);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_part, dim3(t1 - t0), dim3(BT_PBLOCK), 0, P.s, P.keys, P.vals,
+                     P.tiles, t0, P.qs, P.eoff, P.rec, P.over, P.meta);
+  NFH_LAUNCH_CHECK();
+}
+
+// Count, scan and emit of units [u0, u1) (their queries hold `rows` rows),
+// group offsets up to group_off[nq_end].  Grids are the bounds on slices
+// and split units; workgroups past the counts the slice pass leaves exit.
+static void enqueue_units(BatchPlan& P, uint32_t u0, uint32_t u1, uint32_t nq_end,
+                          uint64_t rows) {
+  const uint32_t max_sl = uint32_t(uint64_t(u1 - u0) + rows / BT_SLICE + 1);
+  const uint32_t max_multi = uint32_t(rows / BT_SLICE + 1);
+  const uint32_t nch = (u1 - u0 + BT_SCAN - 1) / BT_SCAN;
+  hipLaunchKernelGGL(k_bt_slice_sums, dim3(nch), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.eoff,
+                     P.csl);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_slice_scan, dim3(nch), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.eoff,
+                     P.csl, P.sstart, P.pstart, P.slice_unit, P.multi, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_count, dim3(max_sl), dim3(BT_CBLOCK), 0, P.s, P.units, P.sstart,
+                     P.pstart, P.slice_unit, P.slots, P.eoff, P.rec, P.cstage, P.ucount, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_cmerge, dim3(max_multi), dim3(BT_CBLOCK), 0, P.s, P.multi, P.sstart,
+                     P.pstart, P.cstage, P.ucount, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_scan, dim3(1), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.ucount,
+                     P.uoff, P.group_off, nq_end, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_emit, dim3(max_sl), dim3(BT_EBLOCK), 0, P.s, P.units, P.sstart,
+                     P.pstart, P.slice_unit, P.slots, P.eoff, P.rec, P.uoff, P.estage, P.ukeys,
+                     P.counts, P.sums, P.cap, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_emerge, dim3(max_multi), dim3(BT_EBLOCK), 0, P.s, P.units, P.multi,
+                     P.sstart, P.pstart, P.uoff, P.estage, P.slots, P.ukeys, P.counts, P.sums,
+                     P.cap, P.meta);
+  NFH_LAUNCH_CHECK();
+}
+
 // The whole batched run on P.s: reset, zero the small queries' slots,
 // accumulate (small), partition (ranged), count, scan, emit, overflow.  No
 // host synchronisation (graph-capturable).
@@ -511,20 +824,8 @@ static void enqueue_batched(BatchPlan& P) {
                        P.items, 0u, P.nitems, P.qs, P.slots, P.over, P.meta);
     NFH_LAUNCH_CHECK();
   }
-  if (P.ntiles) {
-    hipLaunchKernelGGL(k_bt_part, dim3(P.ntiles), dim3(BT_PBLOCK), 0, P.s, P.keys, P.vals,
-                       P.tiles, 0u, P.qs, P.rec, P.entry, P.over, P.meta);
-    NFH_LAUNCH_CHECK();
-  }
-  hipLaunchKernelGGL(k_bt_count, dim3(P.nunits), dim3(BT_CBLOCK), 0, P.s, P.units, 0u, P.qs,
-                     P.slots, P.rec, P.entry, P.ucount);
-  NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_scan, dim3(1), dim3(BT_SCAN), 0, P.s, P.units, 0u, P.nunits,
-                     P.ucount, P.uoff, P.group_off, P.nq, P.meta);
-  NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_emit, dim3(P.nunits), dim3(BT_EBLOCK), 0, P.s, P.units, 0u, P.qs,
-                     P.slots, P.rec, P.entry, P.uoff, P.ukeys, P.counts, P.sums, P.cap);
-  NFH_LAUNCH_CHECK();
+  enqueue_ranged(P, 0, P.ntiles, 0, uint32_t(P.nentries), 0);
+  enqueue_units(P, 0, P.nunits, P.nq, P.nrows);
   hipLaunchKernelGGL(k_bt_overflow, dim3(256), dim3(256), 0, P.s, P.over, P.meta, P.qs, P.slots,
                      P.group_off, P.ukeys, P.sums, P.cap);
   NFH_LAUNCH_CHECK();
@@ -550,20 +851,9 @@ static void enqueue_per_query(BatchPlan& P) {
                          P.over, P.meta);
       NFH_LAUNCH_CHECK();
     }
-    if (t1 > t0) {
-      hipLaunchKernelGGL(k_bt_part, dim3(t1 - t0), dim3(BT_PBLOCK), 0, P.s, P.keys, P.vals,
-                         P.tiles, t0, P.qs, P.rec, P.entry, P.over, P.meta);
-      NFH_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_bt_count, dim3(u1 - u0), dim3(BT_CBLOCK), 0, P.s, P.units, u0, P.qs,
-                       P.slots, P.rec, P.entry, P.ucount);
-    NFH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bt_scan, dim3(1), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.ucount,
-                       P.uoff, P.group_off, q + 1, P.meta);
-    NFH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bt_emit, dim3(u1 - u0), dim3(BT_EBLOCK), 0, P.s, P.units, u0, P.qs,
-                       P.slots, P.rec, P.entry, P.uoff, P.ukeys, P.counts, P.sums, P.cap);
-    NFH_LAUNCH_CHECK();
+    // this query's records start at its first row
+    enqueue_ranged(P, t0, t1, P.p_start[q], P.p_start[q + 1], uint32_t(P.roff[q]));
+    enqueue_units(P, u0, u1, q + 1, uint64_t(P.roff[q + 1] - P.roff[q]));
   }
   hipLaunchKernelGGL(k_bt_overflow, dim3(256), dim3(256), 0, P.s, P.over, P.meta, P.qs, P.slots,
                      P.group_off, P.ukeys, P.sums, P.cap);
@@ -668,41 +958,45 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     std::vector<BtQuery> qs(nq);
     std::vector<BtTile> tl;
     std::vector<BtUnit> un;
-    std::vector<uint32_t> tile_start(nq + 1), unit_start(nq + 1);
+    std::vector<uint32_t> tile_start(nq + 1), unit_start(nq + 1), p_start(nq + 1);
     u64 slot = 0, ent = 0;
     for (size_t q = 0; q < nq; ++q) {
       const uint32_t c = uint32_t(cd[q]);
       tile_start[q] = uint32_t(tl.size());
       unit_start[q] = uint32_t(un.size());
+      p_start[q] = uint32_t(std::min<u64>(ent, UINT32_MAX));
       if (bt_small(cd[q])) {
         qs[q] = {slot, c, 0};
-        un.push_back({uint32_t(q), 0, 0, c, 0, 0, 0, 0});
+        un.push_back({uint32_t(q), 0, 0, c, 0, 0, slot});
         slot += c;
         continue;
       }
       const uint32_t nr = (c + BT_RANGE - 1) / BT_RANGE;
       qs[q] = {0, c, nr};
-      const uint32_t t0 = uint32_t(tl.size());
-      const uint64_t e0 = ent;
-      for (int64_t r = roff[q]; r < roff[q + 1]; r += BT_PTILE) {
+      const uint64_t nrow = uint64_t(roff[q + 1] - roff[q]);
+      const uint32_t nt = uint32_t((nrow + BT_PTILE - 1) / BT_PTILE);
+      for (uint32_t t = 0; t < nt; ++t) {
+        const int64_t r = roff[q] + int64_t(t) * BT_PTILE;
         tl.push_back({uint32_t(q), uint32_t(r),
-                      uint32_t(std::min<int64_t>(r + BT_PTILE, roff[q + 1])), uint32_t(ent)});
-        ent += nr;
+                      uint32_t(std::min<int64_t>(r + BT_PTILE, roff[q + 1])),
+                      uint32_t(std::min<u64>(ent + t, UINT32_MAX)), nt, 0});
       }
-      const uint32_t t1 = uint32_t(tl.size());
       for (uint32_t j = 0; j < nr; ++j)
-        un.push_back({uint32_t(q), j, j * BT_RANGE, std::min(BT_RANGE, c - j * BT_RANGE), t0, t1,
-                      uint32_t(e0), nr});
+        un.push_back({uint32_t(q), j, j * BT_RANGE, std::min(BT_RANGE, c - j * BT_RANGE),
+                      uint32_t(std::min<u64>(ent + u64(j) * nt, UINT32_MAX)), nt, ~0ull});
+      ent += u64(nt) * nr;
     }
     tile_start[nq] = uint32_t(tl.size());
     unit_start[nq] = uint32_t(un.size());
-    if (ent > uint64_t(UINT32_MAX) || un.size() >= size_t(INT32_MAX)) return fail(MALFORMED);
+    p_start[nq] = uint32_t(std::min<u64>(ent, UINT32_MAX));
+    if (ent >= uint64_t(UINT32_MAX) || un.size() >= size_t(INT32_MAX)) return fail(MALFORMED);
     // slots (8 B per key of each small query), records (4 B per row), the
     // overflow list (16 B per row) and the entries must fit the free memory
     size_t free_b = 0, total_b = 0;
     NFH_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t need = slot * 8 + uint64_t(roff[nq]) * 20 + ent * 8 + un.size() * 48 +
-                          tl.size() * 16;
+    const uint64_t maxp = 2 * uint64_t(roff[nq]) / BT_SLICE + 2;  // slices of split units
+    const uint64_t need = slot * 8 + uint64_t(roff[nq]) * 20 + ent * 8 + un.size() * 60 +
+                          tl.size() * 24 + maxp * (BT_RANGE * 8 + BT_RANGE / 8);
     if (need > uint64_t(free_b)) return fail(MALFORMED);
     std::unique_ptr<BatchPlan> P(new BatchPlan());
     P->roff = roff;
@@ -722,13 +1016,23 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     P->nentries = ent;
     P->tile_start = tile_start;
     P->unit_start = unit_start;
+    P->p_start = p_start;
     NFH_CHECK(hipStreamCreate(&P->s));
     P->qs = P->alloc<BtQuery>(nq);
     P->tiles = P->alloc<BtTile>(tl.size());
     P->units = P->alloc<BtUnit>(un.size());
     P->slots = P->alloc<u64>(P->nslots);
     P->rec = P->alloc<uint32_t>(P->nrows);
-    P->entry = P->alloc<uint2>(ent);
+    P->ecnt = P->alloc<uint32_t>(ent);
+    P->eoff = P->alloc<uint32_t>(ent + 1);
+    P->csum = P->alloc<uint32_t>(ent / BT_ECHUNK + 2);
+    P->sstart = P->alloc<uint32_t>(un.size() + 1);
+    P->pstart = P->alloc<uint32_t>(un.size());
+    P->cstage = P->alloc<uint32_t>(maxp * (BT_RANGE / 32));
+    P->estage = P->alloc<u64>(maxp * BT_RANGE);
+    P->slice_unit = P->alloc<uint32_t>(un.size() + uint64_t(roff[nq]) / BT_SLICE + 2);
+    P->multi = P->alloc<uint32_t>(uint64_t(roff[nq]) / BT_SLICE + 2);
+    P->csl = P->alloc<uint32_t>(3 * (un.size() / BT_SCAN + 2));
     P->ucount = P->alloc<uint32_t>(un.size());
     P->uoff = P->alloc<u64>(un.size());
     P->over = P->alloc<BtOver>(P->nrows + 1);

@@ -9,6 +9,7 @@ from numfast_amd import native
 from oracle import executor as ox
 
 pytestmark = pytest.mark.gpu
+BATCH_UNIT_KEYS = 4096  # keys per unit of a query wider than 4096 keys
 
 
 def _plan(torch, k, v, row_off, card, cap=None):
@@ -124,18 +125,18 @@ def test_batch_plan_rejects_bad_offsets(hip):
 
 def test_batch_wide_queries_edges(hip):
     """Queries wider than 4096 keys go through the by-key-range path
-    (16384-key units): a wide query with no rows, one whose last unit holds
+    (4096-key units): a wide query with no rows, one whose last unit holds
     a single key, values outside the packed range in multi-unit queries, a
     wide query's key outside its range (-2) and a capacity below the group
     count (-7); both modes against the oracle."""
     import torch
     rng = np.random.default_rng(5)
-    cards = [4097, 16384 * 3 + 1, 70000, 5000, 100, 250000]
+    cards = [4097, 4096 * 12 + 1, 70000, 5000, 100, 250000]
     rows = [3000, 40000, 0, 9000, 500, 120000]
     ks, vs = [], []
     for c, n in zip(cards, rows):
         k = rng.integers(0, c, n).astype(np.int32)
-        if c == 16384 * 3 + 1 and n:
+        if c == 4096 * 12 + 1 and n:
             k[:3] = [c - 1, c - 1, 0]  # the last unit's only key
         ks.append(k)
         v = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
@@ -149,7 +150,9 @@ def test_batch_wide_queries_edges(hip):
     ref = ox.group_batch(k, v, row_off, card)
     b, _, _ = _plan(torch, k, v, row_off, card)
     info = b.info()
-    assert info["units"] == 1 + 4 + 5 + 1 + 1 + 16 and info["wide_tiles"] > 0
+    unit = BATCH_UNIT_KEYS
+    assert info["units"] == sum(-(-c // unit) if c > 4096 else 1 for c in cards)
+    assert info["wide_tiles"] > 0
     for _ in range(2):
         _check(b.run(0), ref)
         _check(b.run(1), ref)
