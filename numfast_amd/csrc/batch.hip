@@ -78,6 +78,7 @@ constexpr uint32_t BT_SLICE = 32768;
 struct BtItem {  // accumulate work item of a small query
   uint32_t q;
   uint32_t r0, r1;  // row range (global row indices)
+  uint32_t whole;   // 1: the item holds all of the query's rows
 };
 struct BtTile {  // partition tile of a ranged query
   uint32_t q;
@@ -96,6 +97,16 @@ struct BtUnit {  // keys [key0, key0 + nkeys) of query q, in output order
   uint32_t key0, nkeys;
   uint32_t p0, nt;  // ranged: entries [p0, p0 + nt) (its records follow off[p0] .. off[p0 + nt])
   u64 slot0;        // small: the query's first slot; ranged: ~0
+};
+struct BtSlice {  // one slice of a unit, everything its passes read first
+  u64 slot0;        // small: the query's first slot; ranged: ~0
+  u64 pos;          // output position of the unit's first group (one-slice units)
+  uint32_t u;       // the unit
+  uint32_t key0, nkeys;
+  uint32_t a, e;    // ranged: records [a, e)
+  uint32_t stage;   // slice of a split unit: its staging slot
+  uint32_t split;   // 1: the unit has several slices
+  uint32_t pad;
 };
 struct BtOver {  // a row whose value is outside [-2^15, 2^15)
   uint32_t q, key;
@@ -121,13 +132,13 @@ __device__ __forceinline__ uint32_t bt_biased(int32_t v, bool* over) {
 }
 
 // Small queries: persistent accumulation over work items [item0, item1),
-// four rows in flight per thread, LDS aggregation, one flush per item.
+// sixteen rows in flight per thread, LDS aggregation, one flush per item.
 __global__ __launch_bounds__(BT_BLOCK) void k_bt_accumulate(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const BtItem* __restrict__ items, uint32_t item0, uint32_t item1,
     const BtQuery* __restrict__ qs, u64* __restrict__ slots, BtOver* __restrict__ over,
     BtMeta* __restrict__ meta) {
-  constexpr int U = 4;
+  constexpr int U = 16;
   __shared__ u64 s_acc[BT_LDS_SLOTS];
   __shared__ uint32_t s_item;
   for (;;) {
@@ -139,7 +150,12 @@ __global__ __launch_bounds__(BT_BLOCK) void k_bt_accumulate(
     const BtItem w = items[it];
     const BtQuery qd = qs[w.q];
     const uint32_t card = qd.card;
-    for (uint32_t s = threadIdx.x; s < card; s += BT_BLOCK) s_acc[s] = 0;
+    // R copies of each slot (lane l adds into copy l % R): a query of few
+    // keys would otherwise send most lanes of a wave to one LDS word
+    uint32_t R = 1;
+    while (R < 32 && card * (2 * R) <= uint32_t(BT_LDS_SLOTS)) R *= 2;
+    const uint32_t rl = uint32_t(lane_id()) & (R - 1);
+    for (uint32_t s = threadIdx.x; s < card * R; s += BT_BLOCK) s_acc[s] = 0;
     __syncthreads();
     for (uint32_t r = w.r0 + threadIdx.x; r < w.r1; r += U * BT_BLOCK) {
       int32_t k[U], x[U];
@@ -159,15 +175,22 @@ __global__ __launch_bounds__(BT_BLOCK) void k_bt_accumulate(
         }
         bool ov;
         const uint32_t b = bt_biased(x[u], &ov);
-        atomicAdd(&s_acc[k[u]], (u64(1) << BT_SHIFT) + u64(b));
+        atomicAdd(&s_acc[uint32_t(k[u]) * R + rl], (u64(1) << BT_SHIFT) + u64(b));
         if (ov) over[atomicAdd(&meta->nover, 1u)] = BtOver{w.q, uint32_t(k[u]), x[u], 0u};
       }
     }
     __syncthreads();
+    // an item that holds the whole query stores its slots; the items of a
+    // larger query add into them (device atomics: the cost that sets the
+    // autotuned item size)
     u64* qslots = slots + qd.slot0;
     for (uint32_t s = threadIdx.x; s < card; s += BT_BLOCK) {
-      const u64 a = s_acc[s];
-      if (a) atomicAdd(&qslots[s], a);
+      u64 a = 0;
+      for (uint32_t r = 0; r < R; ++r) a += s_acc[s * R + r];
+      if (w.whole)
+        qslots[s] = a;
+      else if (a)
+        atomicAdd(&qslots[s], a);
     }
     __syncthreads();
   }
@@ -332,23 +355,27 @@ __device__ __forceinline__ void bt_for_records(uint32_t a, uint32_t e,
 // Two launches over chunks of BT_SCAN units: k_bt_slice_sums leaves each
 // chunk's totals (slices, staging slots, split units) in csl[3 * c ..];
 // k_bt_slice_scan scans within its chunk from the sum of the chunks before
-// it: sstart[u] = first slice (from 0), sstart[u1] = all slices,
-// slice_unit[sl] = the unit of slice sl, pstart[u] = first staging slot of
-// a unit of several slices, multi[] = those units in order.
-__device__ __forceinline__ uint32_t bt_unit_slices(const BtUnit* __restrict__ units, uint32_t u,
-                                                   uint32_t u1, const uint32_t* __restrict__ off) {
-  if (u >= u1) return 0;
-  const BtUnit un = units[u];
-  if (un.slot0 != ~0ull) return 1;
-  const uint32_t R = off[un.p0 + un.nt] - off[un.p0];
-  return R > BT_SLICE ? (R + BT_SLICE - 1) / BT_SLICE : 1u;
+// it: sstart[u] = first slice (from 0), sstart[u1] = all slices, desc[sl]
+// = slice sl's descriptor (its output position is filled in by the unit
+// scan), pstart[u] = first staging slot of a unit of several slices,
+// multi[] = those units in order.
+struct BtUnitSpan {
+  uint32_t S, a, e;
+};
+__device__ __forceinline__ BtUnitSpan bt_unit_span(const BtUnit& un,
+                                                   const uint32_t* __restrict__ off) {
+  if (un.slot0 != ~0ull) return {1u, 0u, 0u};
+  const uint32_t a = off[un.p0], e = off[un.p0 + un.nt];
+  const uint32_t R = e - a;
+  return {R > BT_SLICE ? (R + BT_SLICE - 1) / BT_SLICE : 1u, a, e};
 }
 __global__ __launch_bounds__(BT_SCAN) void k_bt_slice_sums(const BtUnit* __restrict__ units,
                                                            uint32_t u0, uint32_t u1,
                                                            const uint32_t* __restrict__ off,
                                                            uint32_t* __restrict__ csl) {
   __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
-  const uint32_t S = bt_unit_slices(units, u0 + blockIdx.x * BT_SCAN + threadIdx.x, u1, off);
+  const uint32_t u = u0 + blockIdx.x * BT_SCAN + threadIdx.x;
+  const uint32_t S = u < u1 ? bt_unit_span(units[u], off).S : 0u;
   uint32_t ts, tp, tm;
   (void)block_excl_scan<BT_SCAN / WAVE>(S, s_tmp, &ts);
   (void)block_excl_scan<BT_SCAN / WAVE>(S > 1 ? S : 0u, s_tmp, &tp);
@@ -362,8 +389,8 @@ __global__ __launch_bounds__(BT_SCAN) void k_bt_slice_sums(const BtUnit* __restr
 __global__ __launch_bounds__(BT_SCAN) void k_bt_slice_scan(
     const BtUnit* __restrict__ units, uint32_t u0, uint32_t u1, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ csl, uint32_t* __restrict__ sstart,
-    uint32_t* __restrict__ pstart, uint32_t* __restrict__ slice_unit,
-    uint32_t* __restrict__ multi, BtMeta* __restrict__ meta) {
+    uint32_t* __restrict__ pstart, BtSlice* __restrict__ desc, uint32_t* __restrict__ multi,
+    BtMeta* __restrict__ meta) {
   __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
   __shared__ uint32_t s_base[3];
   if (threadIdx.x < 3) {
@@ -373,8 +400,13 @@ __global__ __launch_bounds__(BT_SCAN) void k_bt_slice_scan(
   }
   __syncthreads();
   const uint32_t u = u0 + blockIdx.x * BT_SCAN + threadIdx.x;
-  const uint32_t S = bt_unit_slices(units, u, u1, off);
-  const uint32_t P = S > 1 ? S : 0u, M = S > 1 ? 1u : 0u;
+  BtUnit un{};
+  BtUnitSpan sp{0u, 0u, 0u};
+  if (u < u1) {
+    un = units[u];
+    sp = bt_unit_span(un, off);
+  }
+  const uint32_t S = sp.S, P = S > 1 ? S : 0u, M = S > 1 ? 1u : 0u;
   uint32_t ts, tp, tm;
   const uint32_t xs = s_base[0] + block_excl_scan<BT_SCAN / WAVE>(S, s_tmp, &ts);
   const uint32_t xp = s_base[1] + block_excl_scan<BT_SCAN / WAVE>(P, s_tmp, &tp);
@@ -382,7 +414,20 @@ __global__ __launch_bounds__(BT_SCAN) void k_bt_slice_scan(
   if (u < u1) {
     sstart[u] = xs;
     pstart[u] = xp;
-    for (uint32_t i = 0; i < S; ++i) slice_unit[xs + i] = u;
+    for (uint32_t i = 0; i < S; ++i) {
+      BtSlice d;
+      d.slot0 = un.slot0;
+      d.pos = 0;
+      d.u = u;
+      d.key0 = un.key0;
+      d.nkeys = un.nkeys;
+      d.a = S > 1 ? sp.a + i * BT_SLICE : sp.a;
+      d.e = S > 1 ? min(sp.a + (i + 1) * BT_SLICE, sp.e) : sp.e;
+      d.stage = S > 1 ? xp + i : 0u;
+      d.split = S > 1 ? 1u : 0u;
+      d.pad = 0;
+      desc[xs + i] = d;
+    }
     if (M) multi[xm] = u;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
@@ -397,9 +442,7 @@ __global__ __launch_bounds__(BT_SCAN) void k_bt_slice_scan(
 // their present slots, ranged units the distinct keys of their records); a
 // slice of a larger unit leaves its presence bitmap in cstage.
 __global__ __launch_bounds__(BT_CBLOCK) void k_bt_count(
-    const BtUnit* __restrict__ units, const uint32_t* __restrict__ sstart,
-    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ slice_unit,
-    const u64* __restrict__ slots, const uint32_t* __restrict__ off,
+    const BtSlice* __restrict__ desc, const u64* __restrict__ slots,
     const uint32_t* __restrict__ rec, uint32_t* __restrict__ cstage,
     uint32_t* __restrict__ ucount, const BtMeta* __restrict__ meta) {
   constexpr uint32_t W = BT_RANGE / 32;
@@ -407,25 +450,21 @@ __global__ __launch_bounds__(BT_CBLOCK) void k_bt_count(
   __shared__ uint32_t s_tmp[BT_CBLOCK / WAVE];
   const uint32_t sl = blockIdx.x;
   if (sl >= meta->nslices) return;  // uniform
-  const uint32_t u = slice_unit[sl];
-  const uint32_t i = sl - sstart[u], S = sstart[u + 1] - sstart[u];
-  const BtUnit un = units[u];
+  const BtSlice d = desc[sl];
   uint32_t c = 0;
-  if (un.slot0 != ~0ull) {
-    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_CBLOCK)
-      c += (slots[un.slot0 + k] >> BT_SHIFT) != 0 ? 1u : 0u;
+  if (d.slot0 != ~0ull) {
+    for (uint32_t k = threadIdx.x; k < d.nkeys; k += BT_CBLOCK)
+      c += (slots[d.slot0 + k] >> BT_SHIFT) != 0 ? 1u : 0u;
   } else {
-    const uint32_t a0 = off[un.p0], e0 = off[un.p0 + un.nt];
-    const uint32_t a = a0 + i * BT_SLICE, e = S > 1 ? min(a + BT_SLICE, e0) : e0;
     for (uint32_t w = threadIdx.x; w < W; w += BT_CBLOCK) s_bits[w] = 0;
     __syncthreads();
-    bt_for_records<BT_CBLOCK, 16>(a, e, rec, [&](uint32_t r) {
+    bt_for_records<BT_CBLOCK, 16>(d.a, d.e, rec, [&](uint32_t r) {
       const uint32_t kl = r >> 16;
       atomicOr(&s_bits[kl >> 5], 1u << (kl & 31u));
     });
     __syncthreads();
-    if (S > 1) {
-      uint32_t* dst = cstage + size_t(pstart[u] + i) * W;
+    if (d.split) {
+      uint32_t* dst = cstage + size_t(d.stage) * W;
       for (uint32_t w = threadIdx.x; w < W; w += BT_CBLOCK) dst[w] = s_bits[w];
       return;  // uniform
     }
@@ -433,7 +472,7 @@ __global__ __launch_bounds__(BT_CBLOCK) void k_bt_count(
   }
   uint32_t tot;
   (void)block_excl_scan<BT_CBLOCK / WAVE>(c, s_tmp, &tot);
-  if (threadIdx.x == 0) ucount[u] = tot;
+  if (threadIdx.x == 0) ucount[d.u] = tot;
 }
 
 // Units of several slices (one workgroup each, past the last: exit): the
@@ -460,47 +499,47 @@ __global__ __launch_bounds__(BT_CBLOCK) void k_bt_cmerge(
   }
 }
 
-// Exclusive scan of the unit counts of [u0, u1) from meta->base (one
-// workgroup): uoff per unit, group_off[q] at each query's first unit,
-// meta->total and group_off[nq_end] at the end.
-__global__ __launch_bounds__(BT_SCAN) void k_bt_scan(const BtUnit* __restrict__ units,
-                                                     uint32_t u0, uint32_t u1,
+// Exclusive scan of the unit counts of [u0, u1) from meta->base, over
+// chunks of BT_SCAN units: k_bt_usum leaves each chunk's total in csl[c];
+// k_bt_uscan scans its chunk from the sum of the chunks before it: uoff per
+// unit, the output position in the descriptor of a one-slice unit,
+// group_off[q] at each query's first unit, meta->total and
+// group_off[nq_end] at the end.
+__global__ __launch_bounds__(BT_SCAN) void k_bt_usum(uint32_t u0, uint32_t u1,
                                                      const uint32_t* __restrict__ ucount,
-                                                     u64* __restrict__ uoff,
-                                                     int64_t* __restrict__ group_off,
-                                                     uint32_t nq_end, BtMeta* __restrict__ meta) {
-  constexpr int PER = 4;
+                                                     uint32_t* __restrict__ csl) {
   __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
-  __shared__ u64 s_carry;
-  if (threadIdx.x == 0) s_carry = meta->base;
-  __syncthreads();
-  for (uint32_t b = u0; b < u1; b += uint32_t(BT_SCAN) * PER) {
-    uint32_t c[PER], sum = 0;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const uint32_t u = b + threadIdx.x * PER + uint32_t(i);
-      c[i] = u < u1 ? ucount[u] : 0u;
-      sum += c[i];
-    }
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan<BT_SCAN / WAVE>(sum, s_tmp, &tot);
-    u64 run = s_carry + ex;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const uint32_t u = b + threadIdx.x * PER + uint32_t(i);
-      if (u < u1) {
-        uoff[u] = run;
-        if (units[u].j == 0) group_off[units[u].q] = int64_t(run);
-      }
-      run += c[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry += tot;
-    __syncthreads();
-  }
+  const uint32_t u = u0 + blockIdx.x * BT_SCAN + threadIdx.x;
+  uint32_t tot;
+  (void)block_excl_scan<BT_SCAN / WAVE>(u < u1 ? ucount[u] : 0u, s_tmp, &tot);
+  if (threadIdx.x == 0) csl[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(BT_SCAN) void k_bt_uscan(
+    const BtUnit* __restrict__ units, uint32_t u0, uint32_t u1,
+    const uint32_t* __restrict__ ucount, const uint32_t* __restrict__ csl,
+    const uint32_t* __restrict__ sstart, BtSlice* __restrict__ desc, u64* __restrict__ uoff,
+    int64_t* __restrict__ group_off, uint32_t nq_end, BtMeta* __restrict__ meta) {
+  __shared__ uint32_t s_tmp[BT_SCAN / WAVE];
+  __shared__ u64 s_base;
   if (threadIdx.x == 0) {
-    meta->total = s_carry;
-    group_off[nq_end] = int64_t(s_carry);
+    u64 b = meta->base;
+    for (uint32_t c = 0; c < blockIdx.x; ++c) b += csl[c];
+    s_base = b;
+  }
+  __syncthreads();
+  const uint32_t u = u0 + blockIdx.x * BT_SCAN + threadIdx.x;
+  const uint32_t c = u < u1 ? ucount[u] : 0u;
+  uint32_t tot;
+  const u64 run = s_base + block_excl_scan<BT_SCAN / WAVE>(c, s_tmp, &tot);
+  if (u < u1) {
+    uoff[u] = run;
+    if (units[u].j == 0) group_off[units[u].q] = int64_t(run);
+    const uint32_t s0 = sstart[u];
+    if (sstart[u + 1] - s0 == 1) desc[s0].pos = run;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    meta->total = s_base + tot;
+    group_off[nq_end] = int64_t(s_base + tot);
   }
 }
 
@@ -509,14 +548,15 @@ __global__ __launch_bounds__(BT_SCAN) void k_bt_scan(const BtUnit* __restrict__ 
 // present slot's output position in it (for the overflow pass).  Each wave
 // compacts one contiguous segment of the unit's keys by ballot.
 template <class F>
-__device__ __forceinline__ void bt_compact(const BtUnit& un, u64 pos, F packed, uint32_t* s_wc,
-                                           u64* __restrict__ slots, int64_t* __restrict__ ukeys,
+__device__ __forceinline__ void bt_compact(u64 slot0, uint32_t key0, u64 pos, F packed,
+                                           uint32_t* s_wc, u64* __restrict__ slots,
+                                           int64_t* __restrict__ ukeys,
                                            int64_t* __restrict__ counts,
                                            int64_t* __restrict__ sums, u64 cap) {
   // wave w holds keys [w * PER * 64, (w + 1) * PER * 64), all PER reads in
   // flight at once and kept in registers for both passes
   constexpr int PER = int(BT_RANGE) / BT_EBLOCK;
-  const bool small = un.slot0 != ~0ull;
+  const bool small = slot0 != ~0ull;
   const uint32_t wave = uint32_t(wave_id()), lane = uint32_t(lane_id());
   const uint32_t k0 = wave * uint32_t(PER * WAVE);
   u64 v[PER];
@@ -538,11 +578,11 @@ __device__ __forceinline__ void bt_compact(const BtUnit& un, u64 pos, F packed, 
     if (cnt != 0) {
       const u64 p = pos + u64(__popcll(b & lt));
       if (p < cap) {
-        ukeys[p] = int64_t(un.key0 + key);
+        ukeys[p] = int64_t(key0 + key);
         counts[p] = int64_t(cnt);
         sums[p] = int64_t(pk & BT_MASK) - int64_t(cnt) * 32768;
       }
-      if (small) slots[un.slot0 + key] = p;
+      if (small) slots[slot0 + key] = p;
     }
     pos += u64(__popcll(b));
   }
@@ -553,45 +593,36 @@ __device__ __forceinline__ void bt_compact(const BtUnit& un, u64 pos, F packed, 
 // unit is aggregated (ranged: its records in LDS) and written at its
 // offset; a slice of a larger unit leaves its packed aggregates in estage.
 __global__ __launch_bounds__(BT_EBLOCK) void k_bt_emit(
-    const BtUnit* __restrict__ units, const uint32_t* __restrict__ sstart,
-    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ slice_unit,
-    u64* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ rec,
-    const u64* __restrict__ uoff, u64* __restrict__ estage, int64_t* __restrict__ ukeys,
-    int64_t* __restrict__ counts, int64_t* __restrict__ sums, u64 cap,
-    const BtMeta* __restrict__ meta) {
+    const BtSlice* __restrict__ desc, u64* __restrict__ slots, const uint32_t* __restrict__ rec,
+    u64* __restrict__ estage, int64_t* __restrict__ ukeys, int64_t* __restrict__ counts,
+    int64_t* __restrict__ sums, u64 cap, const BtMeta* __restrict__ meta) {
   __shared__ u64 s_acc[BT_RANGE];
   __shared__ uint32_t s_wc[BT_EBLOCK / WAVE];
   const uint32_t sl = blockIdx.x;
   if (sl >= meta->nslices) return;  // uniform
-  const uint32_t u = slice_unit[sl];
-  const uint32_t i = sl - sstart[u], S = sstart[u + 1] - sstart[u];
-  const BtUnit un = units[u];
-  const bool small = un.slot0 != ~0ull;
-  const u64 pos = uoff[u];
-  if (small) {
+  const BtSlice d = desc[sl];
+  if (d.slot0 != ~0ull) {
     // staged in LDS: the compaction then reads LDS only (a load that may be
     // global or LDS compiles to a flat load, whose wait also waits for the
-    // group stores in flight -- 0.9 ms of this pass)
-    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) s_acc[k] = slots[un.slot0 + k];
+    // group stores in flight)
+    for (uint32_t k = threadIdx.x; k < d.nkeys; k += BT_EBLOCK) s_acc[k] = slots[d.slot0 + k];
     __syncthreads();
   } else {
-    const uint32_t a0 = off[un.p0], e0 = off[un.p0 + un.nt];
-    const uint32_t a = a0 + i * BT_SLICE, e = S > 1 ? min(a + BT_SLICE, e0) : e0;
-    for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) s_acc[k] = 0;
+    for (uint32_t k = threadIdx.x; k < d.nkeys; k += BT_EBLOCK) s_acc[k] = 0;
     __syncthreads();
-    bt_for_records<BT_EBLOCK, 8>(a, e, rec, [&](uint32_t r) {
+    bt_for_records<BT_EBLOCK, 8>(d.a, d.e, rec, [&](uint32_t r) {
       atomicAdd(&s_acc[r >> 16], (u64(1) << BT_SHIFT) + u64(r & 0xFFFFu));
     });
     __syncthreads();
-    if (S > 1) {
-      u64* dst = estage + size_t(pstart[u] + i) * BT_RANGE;
-      for (uint32_t k = threadIdx.x; k < un.nkeys; k += BT_EBLOCK) dst[k] = s_acc[k];
+    if (d.split) {
+      u64* dst = estage + size_t(d.stage) * BT_RANGE;
+      for (uint32_t k = threadIdx.x; k < d.nkeys; k += BT_EBLOCK) dst[k] = s_acc[k];
       return;  // uniform
     }
   }
   bt_compact(
-      un, pos, [&](uint32_t k) -> u64 { return k < un.nkeys ? s_acc[k] : 0ull; }, s_wc, slots,
-      ukeys, counts, sums, cap);
+      d.slot0, d.key0, d.pos, [&](uint32_t k) -> u64 { return k < d.nkeys ? s_acc[k] : 0ull; },
+      s_wc, slots, ukeys, counts, sums, cap);
 }
 
 // Units of several slices: the sum of their slices' packed aggregates,
@@ -616,8 +647,8 @@ __global__ __launch_bounds__(BT_EBLOCK) void k_bt_emerge(
     }
     __syncthreads();
     bt_compact(
-        un, uoff[u], [&](uint32_t k) -> u64 { return k < un.nkeys ? s_acc[k] : 0ull; }, s_wc,
-        slots, ukeys, counts, sums, cap);
+        un.slot0, un.key0, uoff[u], [&](uint32_t k) -> u64 { return k < un.nkeys ? s_acc[k] : 0ull; },
+        s_wc, slots, ukeys, counts, sums, cap);
   }
 }
 
@@ -705,7 +736,7 @@ struct BatchPlan {
   uint32_t* pstart = nullptr;  // per unit: first staging slot (units of several slices)
   uint32_t* cstage = nullptr;  // presence bitmaps of slices
   u64* estage = nullptr;       // packed aggregates of slices
-  uint32_t* slice_unit = nullptr;  // unit of each slice
+  BtSlice* desc = nullptr;         // descriptor of each slice
   uint32_t* multi = nullptr;       // units of several slices
   uint32_t* csl = nullptr;         // slice-scan chunk totals
   std::vector<uint32_t> p_start;  // per query (+1): first entry
@@ -746,8 +777,10 @@ static void build_items(BatchPlan& P, uint32_t R) {
   for (uint32_t q = 0; q < P.nq; ++q) {
     P.item_start[q] = uint32_t(it.size());
     if (!bt_small(P.card[q])) continue;
+    const bool whole = P.roff[q + 1] - P.roff[q] <= int64_t(R);
     for (int64_t r = P.roff[q]; r < P.roff[q + 1]; r += R)
-      it.push_back({q, uint32_t(r), uint32_t(std::min<int64_t>(r + R, P.roff[q + 1]))});
+      it.push_back({q, uint32_t(r), uint32_t(std::min<int64_t>(r + R, P.roff[q + 1])),
+                    whole ? 1u : 0u});
   }
   P.item_start[P.nq] = uint32_t(it.size());
   P.nitems = uint32_t(it.size());
@@ -791,20 +824,21 @@ static void enqueue_units(BatchPlan& P, uint32_t u0, uint32_t u1, uint32_t nq_en
                      P.csl);
   NFH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bt_slice_scan, dim3(nch), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.eoff,
-                     P.csl, P.sstart, P.pstart, P.slice_unit, P.multi, P.meta);
+                     P.csl, P.sstart, P.pstart, P.desc, P.multi, P.meta);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_count, dim3(max_sl), dim3(BT_CBLOCK), 0, P.s, P.units, P.sstart,
-                     P.pstart, P.slice_unit, P.slots, P.eoff, P.rec, P.cstage, P.ucount, P.meta);
+  hipLaunchKernelGGL(k_bt_count, dim3(max_sl), dim3(BT_CBLOCK), 0, P.s, P.desc, P.slots, P.rec,
+                     P.cstage, P.ucount, P.meta);
   NFH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bt_cmerge, dim3(max_multi), dim3(BT_CBLOCK), 0, P.s, P.multi, P.sstart,
                      P.pstart, P.cstage, P.ucount, P.meta);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_scan, dim3(1), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.ucount,
-                     P.uoff, P.group_off, nq_end, P.meta);
+  hipLaunchKernelGGL(k_bt_usum, dim3(nch), dim3(BT_SCAN), 0, P.s, u0, u1, P.ucount, P.csl);
   NFH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bt_emit, dim3(max_sl), dim3(BT_EBLOCK), 0, P.s, P.units, P.sstart,
-                     P.pstart, P.slice_unit, P.slots, P.eoff, P.rec, P.uoff, P.estage, P.ukeys,
-                     P.counts, P.sums, P.cap, P.meta);
+  hipLaunchKernelGGL(k_bt_uscan, dim3(nch), dim3(BT_SCAN), 0, P.s, P.units, u0, u1, P.ucount,
+                     P.csl, P.sstart, P.desc, P.uoff, P.group_off, nq_end, P.meta);
+  NFH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bt_emit, dim3(max_sl), dim3(BT_EBLOCK), 0, P.s, P.desc, P.slots, P.rec,
+                     P.estage, P.ukeys, P.counts, P.sums, P.cap, P.meta);
   NFH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bt_emerge, dim3(max_multi), dim3(BT_EBLOCK), 0, P.s, P.units, P.multi,
                      P.sstart, P.pstart, P.uoff, P.estage, P.slots, P.ukeys, P.counts, P.sums,
@@ -995,7 +1029,7 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     size_t free_b = 0, total_b = 0;
     NFH_CHECK(hipMemGetInfo(&free_b, &total_b));
     const uint64_t maxp = 2 * uint64_t(roff[nq]) / BT_SLICE + 2;  // slices of split units
-    const uint64_t need = slot * 8 + uint64_t(roff[nq]) * 20 + ent * 8 + un.size() * 60 +
+    const uint64_t need = slot * 8 + uint64_t(roff[nq]) * 24 + ent * 8 + un.size() * 104 +
                           tl.size() * 24 + maxp * (BT_RANGE * 8 + BT_RANGE / 8);
     if (need > uint64_t(free_b)) return fail(MALFORMED);
     std::unique_ptr<BatchPlan> P(new BatchPlan());
@@ -1030,7 +1064,7 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     P->pstart = P->alloc<uint32_t>(un.size());
     P->cstage = P->alloc<uint32_t>(maxp * (BT_RANGE / 32));
     P->estage = P->alloc<u64>(maxp * BT_RANGE);
-    P->slice_unit = P->alloc<uint32_t>(un.size() + uint64_t(roff[nq]) / BT_SLICE + 2);
+    P->desc = P->alloc<BtSlice>(un.size() + uint64_t(roff[nq]) / BT_SLICE + 2);
     P->multi = P->alloc<uint32_t>(uint64_t(roff[nq]) / BT_SLICE + 2);
     P->csl = P->alloc<uint32_t>(3 * (un.size() / BT_SCAN + 2));
     P->ucount = P->alloc<uint32_t>(un.size());
@@ -1047,7 +1081,7 @@ NF_EXPORT void* nfh_group_batch_plan(const int32_t* keys, const int32_t* vals,
     build_items(*P, bestR);
     if (P->nitems) {
       float best = 1e30f;
-      for (uint32_t R : {2048u, 8192u, 32768u}) {
+      for (uint32_t R : {8192u, 32768u, 131072u, 524288u}) {
         build_items(*P, R);
         for (uint32_t g : {2u, 4u, 8u}) {
           P->grid = cus * g;
