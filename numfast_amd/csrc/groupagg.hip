@@ -215,8 +215,8 @@ __global__ void k_sample(const int32_t* __restrict__ keys, uint32_t n, uint32_t 
 // {key : 32 | count : 32} (count 0 = empty; claimed by compare-and-swap,
 // counted by atomic add, no waiting on other lanes) and appends the keys
 // seen at least HOT_MIN_COUNT times to the candidate list.
-constexpr int HOT_PARTS = 16;
-constexpr int HOT_PART_SLOTS = 8192;  // a part holds about S / 16 sampled keys
+constexpr int HOT_PARTS = 64;
+constexpr int HOT_PART_SLOTS = 2048;  // a part holds about S / 64 sampled keys
 
 __global__ __launch_bounds__(1024) void k_hot_count(const int32_t* __restrict__ sample, uint32_t S,
                                                     int2* __restrict__ cand,
@@ -224,20 +224,33 @@ __global__ __launch_bounds__(1024) void k_hot_count(const int32_t* __restrict__ 
   __shared__ unsigned long long t[HOT_PART_SLOTS];
   for (int i = threadIdx.x; i < HOT_PART_SLOTS; i += blockDim.x) t[i] = 0ull;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-    const int32_t k = sample[i];
-    const uint32_t h = hot_mix(k);
-    if ((h >> 28) != blockIdx.x) continue;
-    const unsigned long long mine = ((unsigned long long)uint32_t(k) << 32) | 1ull;
-    uint32_t slot = h & (HOT_PART_SLOTS - 1);
-    for (int probe = 0; probe < HOT_PART_SLOTS; ++probe) {
-      const unsigned long long old = atomicCAS(&t[slot], 0ull, mine);
-      if (old == 0ull) break;                          // claimed, count 1
-      if (uint32_t(old >> 32) == uint32_t(k)) {       // counted
-        atomicAdd(&t[slot], 1ull);
-        break;
+  // eight sample loads in flight per thread (one at a time, the pass was
+  // a chain of 64 dependent loads per thread)
+  constexpr uint32_t U = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < S; i0 += U * blockDim.x) {
+    int32_t ks[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * blockDim.x;
+      ks[u] = i < S ? sample[i] : 0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (i0 + u * blockDim.x >= S) continue;
+      const int32_t k = ks[u];
+      const uint32_t h = hot_mix(k);
+      if ((h >> 26) != blockIdx.x) continue;
+      const unsigned long long mine = ((unsigned long long)uint32_t(k) << 32) | 1ull;
+      uint32_t slot = h & (HOT_PART_SLOTS - 1);
+      for (int probe = 0; probe < HOT_PART_SLOTS; ++probe) {
+        const unsigned long long old = atomicCAS(&t[slot], 0ull, mine);
+        if (old == 0ull) break;                          // claimed, count 1
+        if (uint32_t(old >> 32) == uint32_t(k)) {       // counted
+          atomicAdd(&t[slot], 1ull);
+          break;
+        }
+        slot = (slot + 1) & (HOT_PART_SLOTS - 1);
       }
-      slot = (slot + 1) & (HOT_PART_SLOTS - 1);
     }
   }
   __syncthreads();
@@ -261,23 +274,28 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int2* __restrict__ cand
   __shared__ int32_t t_idx[HOT_TBL];
   __shared__ int32_t t_key[HOT_TBL];
   __shared__ uint32_t s_thr, s_cnt;
+  __shared__ uint32_t s_tmp[1024 / WAVE];
+  static_assert(HOT_LEN_BUCKETS == 1024, "one count bucket per thread below");
   const uint32_t nc = *ncand;
   for (int b = threadIdx.x; b < HOT_LEN_BUCKETS; b += blockDim.x) hist[b] = 0;
+  if (threadIdx.x == 0) {
+    s_thr = 0xFFFFFFFFu;
+    s_cnt = 0;
+  }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
     const uint32_t L = uint32_t(cand[i].y);
     atomicAdd(&hist[L < HOT_LEN_BUCKETS ? L : HOT_LEN_BUCKETS - 1], 1u);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0, thr = 0xFFFFFFFFu;
-    for (int b = HOT_LEN_BUCKETS - 1; b >= int(HOT_MIN_COUNT); --b) {
-      if (acc + hist[b] > uint32_t(HOT_MAX)) break;
-      acc += hist[b];
-      thr = uint32_t(b);
-    }
-    s_thr = thr;
-    s_cnt = 0;
+  // threshold: the smallest count bucket b >= HOT_MIN_COUNT whose suffix
+  // (candidates counted b times or more) holds at most HOT_MAX keys --
+  // thread t takes bucket 1023 - t, so an inclusive scan is the suffix sum
+  {
+    const uint32_t b = uint32_t(HOT_LEN_BUCKETS - 1) - threadIdx.x;
+    uint32_t tot;
+    const uint32_t suf = block_excl_scan<1024 / WAVE>(hist[b], s_tmp, &tot) + hist[b];
+    if (b >= HOT_MIN_COUNT && suf <= uint32_t(HOT_MAX)) atomicMin(&s_thr, b);
   }
   __syncthreads();
   const uint32_t thr = s_thr;
