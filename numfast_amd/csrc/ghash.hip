@@ -31,7 +31,6 @@
 
 #include "device.hpp"
 #include "runtime.hpp"
-#include "sort.hpp"
 
 namespace nfh {
 
@@ -328,30 +327,255 @@ __global__ void k_gh_first(const uint32_t* __restrict__ hold, uint64_t m,
   }
 }
 
-// 2. place, in LDS windows.  A partition's table is cut into windows of
-// min(slots, GH_WIN) slots, the grid offset by (q mod slots): the
-// reference's homes in partition q all sit at q modulo P, so windows that
-// start at a home rarely begin inside a cluster.  The first rows are
-// sorted by the window of
-// their reference home (fp & mask), stable in row order.  Each window runs
-// the priority claim in LDS (atomic min of row ids, evicted ids carried
-// on) over its slots plus GH_EXT extension slots; the ids that end in the
-// extension spill into the next window (cyclically), where they start at
-// the first slot: every slot they passed holds a smaller id.  A window of
-// the whole table wraps inside LDS and never spills.  Windows are first
-// solved alone (spill-ins empty); then one workgroup per partition walks
-// its windows in cyclic order from one whose predecessor spilled nothing
-// and re-solves each window whose predecessor spilled, with those spills.
-// A cluster longer than the extension sets the overflow flag, and the
-// partition-free global form (k_gh_place) solves the whole call instead.
-constexpr int GH_WIN = 8192;
-constexpr int GH_EXT = 1024;
-constexpr int GH_WBLOCK = 256;
+// 2. place, in LDS windows (the form used whenever it fits; the global
+// dedupe above and the global place below are the fallback).  A
+// partition's table of S slots is cut into windows of W = min(S, GH_WIN)
+// slots, window i covering slots (q + i*W) mod S onward: the reference's
+// homes in partition q all sit at q modulo P, so a window starts at a home
+// and clusters rarely cross its end.  A pair's window follows from its
+// home, so every row of a pair falls in the same window:
+//   k_gh_whist counts each 8192-row tile's rows per window of its partition
+//     into one entry per (window, tile), laid out window-major within the
+//     partition; a chunked scan of the entries gives each its place;
+//   k_gh_wscatter writes each row there as a (key, value, row) record,
+//     staged in LDS in window order so that the stores are contiguous runs;
+//   k_gh_wsolve, one workgroup per window, inserts the window's records
+//     into an LDS hash table (64-bit compare-and-swap of (hash tag, record),
+//     the full pair compared through the records on equal tags, atomic
+//     minimum of the row id), so each distinct pair keeps its first row;
+//     runs the priority claim over the window's slots plus GH_EXT extension
+//     slots (atomic minimum of (row id, record), evicted ids carried on);
+//     and writes used / tk / tv of the window's occupied slots from the
+//     records.  Ids that end in the extension spill into the next window
+//     (cyclically), where they start at the first slot: every slot they
+//     passed holds a smaller id.  A window of the whole table wraps inside
+//     LDS and never spills;
+//   k_gh_wfix, one workgroup per partition of several windows, walks them in
+//     cyclic order from one whose predecessor spilled nothing (else from
+//     window 0, three laps) and re-solves each window whose predecessor
+//     spills, with those spills.
+// Spills only add ids to a window, so its occupied slots only grow and a
+// re-solve overwrites every slot the first solve wrote.  A cluster longer
+// than the extension, or a window with more distinct pairs than its LDS
+// table holds, sets bit 32 of err and the caller runs the global form,
+// which rewrites every occupied slot.  The windowed form runs only when no
+// partition has more rows than slots, so it never meets a full table.
+constexpr int GH_WIN = 4096;             // window slots
+constexpr int GH_EXT = 1024;             // extension slots
+constexpr int GH_WTHREADS = 512;         // threads of the tile kernels
+constexpr int GH_WIPT = 16;
+constexpr uint32_t GH_WTILE = uint32_t(GH_WTHREADS) * GH_WIPT;  // 8192 rows
+constexpr uint32_t GH_WQ_MAX = 4096;     // windows per partition
+constexpr int GH_SBLOCK = 512;           // threads of the window solve
+constexpr uint32_t GH_SD = 4096;         // LDS hash table slots of the window solve
+constexpr int GH_SU = 4;                 // records per thread per pass
+constexpr unsigned long long GH_NONE = ~0ull;
+constexpr uint32_t GH_ERR_WIN = 32u;     // fall back to the global form
+
+// Mixed 64-bit hash of a pair for the LDS table (its low bits pick the
+// slot, its high half is the tag).
+__device__ __forceinline__ uint64_t gh_dedupe_hash(int64_t k, int64_t v) {
+  uint64_t h = gh_pair_fp(k, v);
+  h ^= h >> 31;
+  h *= 0x7FB5D329728EA185ull;
+  h ^= h >> 27;
+  h *= 0x81DADEF4BC2DD44Dull;
+  h ^= h >> 33;
+  return h;
+}
+
+struct WTile {
+  uint32_t r0, r1;  // rows
+  uint32_t e0;      // entry of (window 0, this tile); window i at e0 + i * nt
+  uint32_t nt;      // tiles of the partition
+  uint32_t q;       // partition
+  uint32_t nwq;     // windows of the partition
+  uint32_t lw;      // log2 of the window size
+  uint32_t pad;
+};
+
+// Window (within its partition) of the reference home of pair (k, v).
+__device__ __forceinline__ uint32_t gh_win_of(int64_t k, int64_t v, uint64_t mask, uint32_t q,
+                                              uint32_t lw) {
+  const uint64_t h = gh_pair_fp(k, v) & mask;
+  return uint32_t(((h - q) & mask) >> lw);
+}
+
+__global__ __launch_bounds__(GH_WTHREADS) void k_gh_whist(const int64_t* __restrict__ pk,
+                                                          const int64_t* __restrict__ pv,
+                                                          const WTile* __restrict__ tiles,
+                                                          const int64_t* __restrict__ tm,
+                                                          uint32_t* __restrict__ ecnt) {
+  __shared__ uint32_t s_cnt[GH_WQ_MAX];
+  const WTile t = tiles[blockIdx.x];
+  const uint64_t mask = uint64_t(tm[t.q]);
+  for (uint32_t b = threadIdx.x; b < t.nwq; b += GH_WTHREADS) s_cnt[b] = 0;
+  __syncthreads();
+  int64_t k[GH_WIPT], v[GH_WIPT];
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    k[u] = r < t.r1 ? pk[r] : 0;
+    v[u] = r < t.r1 ? pv[r] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    if (r < t.r1) atomicAdd(&s_cnt[gh_win_of(k[u], v[u], mask, t.q, t.lw)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < t.nwq; b += GH_WTHREADS) ecnt[t.e0 + b * t.nt] = s_cnt[b];
+}
+
+// Entry scan in three steps over chunks of GH_ECHUNK entries (off[np] =
+// all records).
+constexpr int GH_SCAN = 1024;
+constexpr uint32_t GH_ECHUNK = uint32_t(GH_SCAN) * 4;
+__global__ __launch_bounds__(GH_SCAN) void k_gh_esum(const uint32_t* __restrict__ ecnt,
+                                                     uint32_t np, uint32_t* __restrict__ csum) {
+  __shared__ uint32_t s_tmp[GH_SCAN / WAVE];
+  const uint32_t b = blockIdx.x * GH_ECHUNK;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t e = b + threadIdx.x + uint32_t(i) * GH_SCAN;
+    c += e < np ? ecnt[e] : 0u;
+  }
+  uint32_t tot;
+  (void)block_excl_scan<GH_SCAN / WAVE>(c, s_tmp, &tot);
+  if (threadIdx.x == 0) csum[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(GH_SCAN) void k_gh_etop(uint32_t* __restrict__ csum, uint32_t nc,
+                                                     uint32_t* __restrict__ off, uint32_t np) {
+  __shared__ uint32_t s_tmp[GH_SCAN / WAVE];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < nc; b += GH_SCAN) {
+    const uint32_t c = b + threadIdx.x;
+    const uint32_t x = c < nc ? csum[c] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<GH_SCAN / WAVE>(x, s_tmp, &tot);
+    if (c < nc) csum[c] = s_carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[np] = s_carry;
+}
+__global__ __launch_bounds__(GH_SCAN) void k_gh_escan(const uint32_t* __restrict__ ecnt,
+                                                      uint32_t np,
+                                                      const uint32_t* __restrict__ cbase,
+                                                      uint32_t* __restrict__ off) {
+  __shared__ uint32_t s_tmp[GH_SCAN / WAVE];
+  const uint32_t b = blockIdx.x * GH_ECHUNK + threadIdx.x * 4;
+  uint32_t c[4], sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c[i] = b + i < np ? ecnt[b + i] : 0u;
+    sum += c[i];
+  }
+  uint32_t tot;
+  uint32_t run = cbase[blockIdx.x] + block_excl_scan<GH_SCAN / WAVE>(sum, s_tmp, &tot);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (b + i < np) off[b + i] = run;
+    run += c[i];
+  }
+}
+
+// Records of a tile: each row's rank within (its window, this tile) comes
+// from an LDS counter; the tile's rows are staged in LDS in window order,
+// one array at a time, and stored from there, so consecutive lanes store
+// consecutive records of one (window, tile) run.
+__global__ __launch_bounds__(GH_WTHREADS) void k_gh_wscatter(
+    const int64_t* __restrict__ pk, const int64_t* __restrict__ pv,
+    const WTile* __restrict__ tiles, const int64_t* __restrict__ tm,
+    const uint32_t* __restrict__ off, int64_t* __restrict__ rk, int64_t* __restrict__ rv,
+    uint32_t* __restrict__ rr) {
+  constexpr int PER = GH_WQ_MAX / GH_WTHREADS;
+  __shared__ uint32_t s_cnt[GH_WQ_MAX];  // counts, then the tile's window offsets
+  __shared__ uint32_t s_gb[GH_WQ_MAX];   // first record of (window, this tile)
+  __shared__ uint32_t s_dst[GH_WTILE];   // record of each staged position
+  __shared__ unsigned long long s_stage[GH_WTILE];
+  __shared__ uint32_t s_tmp[GH_WTHREADS / WAVE];
+  const WTile t = tiles[blockIdx.x];
+  const uint64_t mask = uint64_t(tm[t.q]);
+  for (uint32_t b = threadIdx.x; b < t.nwq; b += GH_WTHREADS) s_cnt[b] = 0;
+  __syncthreads();
+  int64_t k[GH_WIPT], v[GH_WIPT];
+  uint32_t wi[GH_WIPT], pos[GH_WIPT];
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    k[u] = r < t.r1 ? pk[r] : 0;
+    v[u] = r < t.r1 ? pv[r] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    wi[u] = gh_win_of(k[u], v[u], mask, t.q, t.lw);
+    pos[u] = r < t.r1 ? atomicAdd(&s_cnt[wi[u]], 1u) : 0u;
+  }
+  __syncthreads();
+  // exclusive scan of the counts: thread i owns windows [i*PER, i*PER+PER)
+  uint32_t c[PER], sum = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t b = threadIdx.x * PER + j;
+    c[j] = b < t.nwq ? s_cnt[b] : 0u;
+    sum += c[j];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan<GH_WTHREADS / WAVE>(sum, s_tmp, &tot);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t b = threadIdx.x * PER + j;
+    if (b < t.nwq) {
+      s_cnt[b] = run;
+      s_gb[b] = off[t.e0 + b * t.nt];
+    }
+    run += c[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    if (r < t.r1) {
+      const uint32_t loc = pos[u];
+      pos[u] = s_cnt[wi[u]] + loc;
+      s_dst[pos[u]] = s_gb[wi[u]] + loc;
+      s_stage[pos[u]] = (unsigned long long)k[u];
+    }
+  }
+  __syncthreads();
+  const uint32_t nrow = t.r1 - t.r0;
+  for (uint32_t j = threadIdx.x; j < nrow; j += GH_WTHREADS) rk[s_dst[j]] = int64_t(s_stage[j]);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    if (r < t.r1) s_stage[pos[u]] = (unsigned long long)v[u];
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nrow; j += GH_WTHREADS) rv[s_dst[j]] = int64_t(s_stage[j]);
+  __syncthreads();
+  uint32_t* s_row = reinterpret_cast<uint32_t*>(s_stage);
+#pragma unroll
+  for (int u = 0; u < GH_WIPT; ++u) {
+    const uint32_t r = t.r0 + uint32_t(u) * GH_WTHREADS + threadIdx.x;
+    if (r < t.r1) s_row[pos[u]] = r;
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nrow; j += GH_WTHREADS) rr[s_dst[j]] = s_row[j];
+}
 
 struct WinGeom {
   const int64_t* toff;       // p + 1
   const int64_t* tm;         // p
   const uint32_t* winbase;   // p + 1: first window of each partition
+  const uint2* qe;           // p: (entry of (window 0, tile 0), tiles) of each partition
+  const uint32_t* off;       // first record of each entry (np + 1)
   uint32_t p;
   uint32_t nwin;
 };
@@ -368,92 +592,201 @@ __device__ __forceinline__ uint32_t gh_win_part(const uint32_t* s_wb, uint32_t p
   return lo;
 }
 
-// Window id of each row's reference home; GH_EMPTY for rows that are not
-// their pair's first row (sorted last).
-__global__ void k_gh_winkey(const int64_t* __restrict__ pk, const int64_t* __restrict__ pv,
-                            const uint8_t* __restrict__ first, PinsGeom g,
-                            const uint32_t* __restrict__ winbase, int32_t* __restrict__ key) {
-  __shared__ int64_t s_starts[GH_PART_MAX + 1];
-  for (uint32_t q = threadIdx.x; q <= g.p; q += blockDim.x) s_starts[q] = g.starts[q];
-  __syncthreads();
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < g.n; j += stride) {
-    if (!first[j]) {
-      key[j] = int32_t(GH_EMPTY ^ 0x80000000u);  // sorts after every window id
-      continue;
-    }
-    const uint32_t q = gh_part_of(s_starts, g.p, j);
-    const uint64_t mask = uint64_t(g.tm[q]);
-    const uint64_t wsz = mask + 1 < uint64_t(GH_WIN) ? mask + 1 : uint64_t(GH_WIN);
-    const uint64_t h = gh_pair_fp(pk[j], pv[j]) & mask;
-    const uint64_t off = uint64_t(q) & mask;
-    key[j] = int32_t((winbase[q] + uint32_t(((h - off) & mask) / wsz)) ^ 0x80000000u);
-  }
-}
-
-// wstart[w] = first sorted position of window w (w <= nwin).
-__global__ void k_gh_winstart(const int32_t* __restrict__ skey, uint64_t n, uint32_t nwin,
-                              uint32_t* __restrict__ wstart) {
-  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w > nwin) return;
-  const int32_t target = int32_t(w ^ 0x80000000u);
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) / 2;
-    if (skey[mid] < target)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  wstart[w] = uint32_t(lo);
-}
-
 // LDS address of window slot i.  A partition's reference homes are
-// congruent modulo P (a multiple of the 64 LDS banks), so unswizzled every
-// claim of a window starts in one bank; XOR-ing the low 6 bits with bits
-// 8..13 spreads homes 256 slots apart over the banks.  It permutes each
-// aligned 64-slot block, so [0, span) maps onto itself (span is a multiple
-// of 64) and the probe order, which follows slot numbers, is unchanged.
+// congruent modulo P (a multiple of the 32 two-bank slots of 64-bit
+// entries), so unswizzled every claim of a window would start in one bank
+// pair; XOR-ing the low 6 bits with bits 8..13 spreads homes 256 slots
+// apart.  It permutes each aligned 64-slot block, so [0, span) maps onto
+// itself (span is a multiple of 64, or below 256 where it is the identity)
+// and the probe order, which follows slot numbers, is unchanged.
 __device__ __forceinline__ uint32_t gh_swz(uint32_t i) { return i ^ ((i >> 8) & 63u); }
 
-// Solve window w in LDS: its rows perm[wstart[w] .. wstart[w+1]) from
-// their homes, plus `nin` spilled ids from slot 0.  Writes the window's
-// slots to `hold` and its extension's ids (slot order) to spill[w];
-// returns through *changed whether the spill list differs from before.
-__device__ void gh_solve_window(uint32_t w, const WinGeom& g, const uint32_t* s_wb,
-                                const int64_t* __restrict__ pk, const int64_t* __restrict__ pv,
-                                const int32_t* __restrict__ perm,
-                                const uint32_t* __restrict__ wstart,
-                                const uint32_t* __restrict__ spill_in, uint32_t nin,
-                                uint32_t* __restrict__ hold, uint32_t* __restrict__ spill,
-                                uint32_t* __restrict__ nspill, uint32_t* s_hold,
-                                uint32_t* s_misc, uint32_t* __restrict__ err) {
-  const uint32_t q = gh_win_part(s_wb, g.p, w);
+union WinTable {
+  struct {
+    unsigned long long key[GH_SD];  // (hash tag << 32) | claimer's record; GH_NONE if free
+    uint32_t first[GH_SD];          // smallest row id of the pair
+  } d;
+  unsigned long long hold[GH_WIN + GH_EXT];  // (row id << 32) | record; GH_NONE if free
+  uint32_t rank[GH_WIN + 2 * GH_SD];          // home counters, grouped rows, ranks
+};
+static_assert(sizeof(WinTable) <= 48 * 1024, "two window solves per CU");
+
+struct WinLds {
+  WinTable u;
+  uint16_t home[GH_SD];  // home slot within the window, set by the claimer
+  uint32_t wb[GH_PART_MAX + 1];
+  uint32_t bad, over, changed, start;
+  uint32_t tmp[GH_SBLOCK / WAVE];
+};
+
+// Solve window w from its records, plus `nin` spilled ids from slot 0.
+// Writes used / tk / tv of the window's occupied slots and its
+// extension's ids (slot order) to spill[w]; L.changed = whether that list
+// differs from before.  False (uniform) when the window cannot be solved
+// in LDS (bit 32 of err set).
+__device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restrict__ rk,
+                          const int64_t* __restrict__ rv, const uint32_t* __restrict__ rr,
+                          const unsigned long long* __restrict__ spill_in, uint32_t nin,
+                          int64_t* __restrict__ tk, int64_t* __restrict__ tv,
+                          uint8_t* __restrict__ used, unsigned long long* __restrict__ spill,
+                          uint32_t* __restrict__ nspill, WinLds& L,
+                          uint32_t* __restrict__ err) {
+  constexpr int PER = GH_SD / GH_SBLOCK;
+  const uint32_t q = gh_win_part(L.wb, g.p, w);
   const uint64_t mask = uint64_t(g.tm[q]);
   const uint32_t wsz = mask + 1 < uint64_t(GH_WIN) ? uint32_t(mask + 1) : uint32_t(GH_WIN);
   const bool whole = uint64_t(wsz) == mask + 1;
   const uint32_t ext = whole ? 0u : uint32_t(GH_EXT);
-  // first slot, within the partition (cyclic)
-  const uint64_t a = (uint64_t(q) + uint64_t(w - s_wb[q]) * wsz) & mask;
   const uint32_t span = wsz + ext;
-  for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) s_hold[i] = GH_EMPTY;
+  const uint32_t wi = w - L.wb[q];
+  const uint64_t a = (uint64_t(q) + uint64_t(wi) * wsz) & mask;  // first slot (cyclic)
+  const uint2 qe = g.qe[q];
+  const uint32_t r0 = g.off[qe.x + wi * qe.y], r1 = g.off[qe.x + (wi + 1) * qe.y];
+  const uint32_t nrec = r1 - r0;
+  uint32_t sd = 64;
+  while (sd < GH_SD && sd < 2 * nrec) sd *= 2;
+  for (uint32_t i = threadIdx.x; i < sd; i += GH_SBLOCK) {
+    L.u.d.key[i] = GH_NONE;
+    L.u.d.first[i] = GH_EMPTY;
+  }
   if (threadIdx.x == 0) {
-    s_misc[0] = 0;
-    s_misc[1] = 0;
+    L.bad = 0;
+    L.over = 0;
   }
   __syncthreads();
-  const uint32_t r0 = wstart[w], r1 = wstart[w + 1];
-  const uint32_t total = (r1 - r0) + nin;
-  bool over = false;
-  for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
-    uint32_t carried, c;
-    if (t < nin) {
-      carried = spill_in[t];
-      c = 0;
-    } else {
-      carried = uint32_t(perm[r0 + (t - nin)]);
-      c = uint32_t(((gh_pair_fp(pk[carried], pv[carried]) & mask) - a) & mask);
+  // 1. distinct pairs of the window's records
+  for (uint32_t i0 = r0; i0 < r1; i0 += GH_SBLOCK * GH_SU) {  // uniform trip count
+    int64_t k[GH_SU], v[GH_SU];
+    uint32_t row[GH_SU];
+#pragma unroll
+    for (int u = 0; u < GH_SU; ++u) {
+      const uint32_t i = i0 + uint32_t(u) * GH_SBLOCK + threadIdx.x;
+      k[u] = i < r1 ? rk[i] : 0;
+      v[u] = i < r1 ? rv[i] : 0;
+      row[u] = i < r1 ? rr[i] : GH_EMPTY;
     }
+#pragma unroll
+    for (int u = 0; u < GH_SU; ++u) {
+      const uint32_t i = i0 + uint32_t(u) * GH_SBLOCK + threadIdx.x;
+      if (i >= r1) continue;
+      const uint64_t h = gh_dedupe_hash(k[u], v[u]);
+      const uint32_t tag = uint32_t(h >> 32);
+      const unsigned long long mine = (uint64_t(tag) << 32) | i;
+      // the probe loop only finds the slot; the claimer's store and the
+      // row minimum follow it
+      uint32_t sl = uint32_t(h) & (sd - 1), slot = GH_EMPTY;
+      bool claimed = false;
+      for (uint32_t probe = 0; probe < sd; ++probe) {
+        const unsigned long long old = atomicCAS(&L.u.d.key[sl], GH_NONE, mine);
+        bool hit = old == GH_NONE;
+        if (!hit && uint32_t(old >> 32) == tag) {
+          const uint32_t o = uint32_t(old);
+          hit = rk[o] == k[u] && rv[o] == v[u];
+        }
+        if (hit) {
+          claimed = old == GH_NONE;
+          slot = sl;
+          break;
+        }
+        sl = (sl + 1) & (sd - 1);
+      }
+      if (slot == GH_EMPTY) {
+        L.bad = 1u;  // more distinct pairs than table slots
+        continue;
+      }
+      if (claimed)
+        L.home[slot] = uint16_t(((gh_pair_fp(k[u], v[u]) & mask) - a) & mask);
+      atomicMin(&L.u.d.first[slot], row[u]);
+    }
+  }
+  __syncthreads();
+  if (L.bad) {  // uniform
+    if (threadIdx.x == 0) atomicOr(err, GH_ERR_WIN);
+    return false;
+  }
+  // 2. the distinct pairs to registers
+  unsigned long long id[PER];
+  uint32_t c0[PER];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const uint32_t j = threadIdx.x + uint32_t(s) * GH_SBLOCK;
+    id[s] = GH_NONE;
+    c0[s] = 0;
+    if (j < sd) {
+      const unsigned long long key = L.u.d.key[j];
+      if (key != GH_NONE) {
+        id[s] = (uint64_t(L.u.d.first[j]) << 32) | uint32_t(key);
+        c0[s] = L.home[j];
+      }
+    }
+  }
+  __syncthreads();
+  // 3. each pair's rank among the window's pairs of the same home, in row
+  // order.  In the sequential layout the pairs of one home with smaller
+  // rows sit in distinct slots between that home and the pair's slot, so
+  // the pair's claim may start `rank` slots past its home: every slot it
+  // skips ends with a smaller id.  Without this, all pairs of a home (about
+  // a hundred at half load and 256 partitions) would start at one slot and
+  // each would pass every smaller one.  Pairs are grouped by home (LDS
+  // counters, a scan, a scatter of their rows), then each position of the
+  // grouped list counts the smaller rows of its group (consecutive lanes
+  // read one group, so the reads are broadcasts).
+  {
+    uint32_t* s_hc = L.u.rank;                           // per home: count, then first position
+    uint32_t* s_rows = s_hc + GH_WIN;                    // rows, grouped by home
+    uint32_t* s_rk = s_rows + GH_SD;                     // home, then rank, per position
+    for (uint32_t i = threadIdx.x; i < wsz; i += GH_SBLOCK) s_hc[i] = 0;
+    __syncthreads();
+    uint32_t loc[PER];
+#pragma unroll
+    for (int s = 0; s < PER; ++s) loc[s] = id[s] != GH_NONE ? atomicAdd(&s_hc[c0[s]], 1u) : 0u;
+    __syncthreads();
+    constexpr int HPT = GH_WIN / GH_SBLOCK;  // counters per thread
+    uint32_t hc[HPT], sum = 0;
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+      const uint32_t b = threadIdx.x * HPT + j;
+      hc[j] = b < wsz ? s_hc[b] : 0u;
+      sum += hc[j];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan<GH_SBLOCK / WAVE>(sum, L.tmp, &total);
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+      const uint32_t b = threadIdx.x * HPT + j;
+      if (b < wsz) s_hc[b] = run;
+      run += hc[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PER; ++s)
+      if (id[s] != GH_NONE) {
+        loc[s] += s_hc[c0[s]];
+        s_rows[loc[s]] = uint32_t(id[s] >> 32);
+        s_rk[loc[s]] = c0[s];
+      }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += GH_SBLOCK) {
+      const uint32_t c = s_rk[j], row = s_rows[j];
+      const uint32_t b = s_hc[c], e = c + 1 < wsz ? s_hc[c + 1] : total;
+      uint32_t r = 0;
+      for (uint32_t i = b; i < e; ++i) r += s_rows[i] < row ? 1u : 0u;
+      s_rk[j] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PER; ++s)
+      if (id[s] != GH_NONE) c0[s] += s_rk[loc[s]];
+    __syncthreads();
+  }
+  // the LDS becomes the hold
+  for (uint32_t i = threadIdx.x; i < span; i += GH_SBLOCK) L.u.hold[i] = GH_NONE;
+  __syncthreads();
+  // 3. priority claim: an id that meets a smaller one moves on, one that
+  // meets a larger one takes the slot and carries the evicted id on
+  const uint32_t limit = 4 * span + sd + nin;
+  bool over = false;
+  auto claim = [&](unsigned long long carried, uint32_t c) {
     for (uint32_t it = 0;; ++it) {
       if (c >= span) {
         if (!whole) {
@@ -462,124 +795,140 @@ __device__ void gh_solve_window(uint32_t w, const WinGeom& g, const uint32_t* s_
         }
         c &= wsz - 1;
       }
-      if (it > 4 * span + total) {  // cannot happen while the table has room
+      if (it > limit) {  // cannot happen while the table has room
         over = true;
         break;
       }
-      const uint32_t old = atomicMin(&s_hold[gh_swz(c)], carried);
-      if (old == GH_EMPTY) break;
+      const unsigned long long old = atomicMin(&L.u.hold[gh_swz(c)], carried);
+      if (old == GH_NONE) break;
       if (old > carried) carried = old;
       ++c;
     }
-  }
-  if (over) atomicOr(err, 32u);
+  };
+#pragma unroll
+  for (int s = 0; s < PER; ++s)
+    if (id[s] != GH_NONE) claim(id[s], c0[s]);
+  for (uint32_t t = threadIdx.x; t < nin; t += GH_SBLOCK) claim(spill_in[t], 0);
+  if (over) L.over = 1u;
   __syncthreads();
-  uint32_t* tab = hold + g.toff[q];
-  for (uint32_t i = threadIdx.x; i < wsz; i += blockDim.x) tab[(a + i) & mask] = s_hold[gh_swz(i)];
+  if (L.over) {  // uniform
+    if (threadIdx.x == 0) atomicOr(err, GH_ERR_WIN);
+    return false;
+  }
+  // 4. the window's slots
+  const uint64_t base = uint64_t(g.toff[q]);
+  for (uint32_t i = threadIdx.x; i < wsz; i += GH_SBLOCK) {
+    const unsigned long long x = L.u.hold[gh_swz(i)];
+    if (x != GH_NONE) {
+      const uint32_t o = uint32_t(x);
+      const uint64_t s = base + ((a + i) & mask);
+      tk[s] = rk[o];
+      tv[s] = rv[o];
+      used[s] = 1;
+    }
+  }
   if (ext == 0) {
     if (threadIdx.x == 0) {
-      if (nspill[w] != 0) s_misc[1] = 1;
+      L.changed = nspill[w] != 0 ? 1u : 0u;
       nspill[w] = 0;
     }
     __syncthreads();
-    return;
+    return true;
   }
   // extension ids in slot order (one wave per step of 64 slots)
-  uint32_t* out = spill + size_t(w) * GH_EXT;
+  unsigned long long* out = spill + size_t(w) * GH_EXT;
   const uint32_t prev = nspill[w];
   __syncthreads();
   if (threadIdx.x < WAVE) {
-    uint32_t base = 0;
+    uint32_t cnt = 0;
     bool diff = false;
     for (uint32_t s0 = 0; s0 < ext; s0 += WAVE) {
-      const uint32_t v = s_hold[gh_swz(wsz + s0 + threadIdx.x)];
-      const bool occ = v != GH_EMPTY;
+      const unsigned long long x = L.u.hold[gh_swz(wsz + s0 + threadIdx.x)];
+      const bool occ = x != GH_NONE;
       const uint64_t bal = __ballot(occ);
       if (occ) {
-        const uint32_t pos = base + uint32_t(__popcll(bal & lanemask_lt()));
-        if (pos >= prev || out[pos] != v) diff = true;
-        out[pos] = v;
+        const uint32_t p = cnt + uint32_t(__popcll(bal & lanemask_lt()));
+        if (p >= prev || out[p] != x) diff = true;
+        out[p] = x;
       }
-      base += uint32_t(__popcll(bal));
+      cnt += uint32_t(__popcll(bal));
     }
-    diff = __ballot(diff) != 0 || base != prev;
+    diff = __ballot(diff) != 0 || cnt != prev;
     if (threadIdx.x == 0) {
-      nspill[w] = base;
-      s_misc[1] = diff ? 1u : 0u;
+      nspill[w] = cnt;
+      L.changed = diff ? 1u : 0u;
     }
   }
   __syncthreads();
+  return true;
 }
 
 // Phase A: every window alone.
-__global__ __launch_bounds__(GH_WBLOCK) void k_gh_win(WinGeom g, const int64_t* __restrict__ pk,
-                                                      const int64_t* __restrict__ pv,
-                                                      const int32_t* __restrict__ perm,
-                                                      const uint32_t* __restrict__ wstart,
-                                                      uint32_t* __restrict__ hold,
-                                                      uint32_t* __restrict__ spill,
-                                                      uint32_t* __restrict__ nspill,
-                                                      uint32_t* __restrict__ err) {
-  __shared__ uint32_t s_hold[GH_WIN + GH_EXT];
-  __shared__ uint32_t s_wb[GH_PART_MAX + 1];
-  __shared__ uint32_t s_misc[2];
-  for (uint32_t q = threadIdx.x; q <= g.p; q += blockDim.x) s_wb[q] = g.winbase[q];
+__global__ __launch_bounds__(GH_SBLOCK) void k_gh_wsolve(
+    WinGeom g, const int64_t* __restrict__ rk, const int64_t* __restrict__ rv,
+    const uint32_t* __restrict__ rr, int64_t* __restrict__ tk, int64_t* __restrict__ tv,
+    uint8_t* __restrict__ used, unsigned long long* __restrict__ spill,
+    uint32_t* __restrict__ nspill, uint32_t* __restrict__ err) {
+  __shared__ WinLds L;
+  for (uint32_t q = threadIdx.x; q <= g.p; q += GH_SBLOCK) L.wb[q] = g.winbase[q];
   __syncthreads();
-  gh_solve_window(blockIdx.x, g, s_wb, pk, pv, perm, wstart, nullptr, 0, hold, spill, nspill,
-                  s_hold, s_misc, err);
+  (void)gh_wsolve(blockIdx.x, g, rk, rv, rr, nullptr, 0, tk, tv, used, spill, nspill, L, err);
 }
 
 // Phase B: one workgroup per partition of several windows walks them in
-// cyclic order from a window whose predecessor spilled nothing (else from
-// window 0, three laps), re-solving each window whose predecessor spills.
-// A spill list still changing on the last lap sets the overflow flag.
-__global__ __launch_bounds__(GH_WBLOCK) void k_gh_win_fix(WinGeom g, const int64_t* __restrict__ pk,
-                                                          const int64_t* __restrict__ pv,
-                                                          const int32_t* __restrict__ perm,
-                                                          const uint32_t* __restrict__ wstart,
-                                                          uint32_t* __restrict__ hold,
-                                                          uint32_t* __restrict__ spill,
-                                                          uint32_t* __restrict__ nspill,
-                                                          uint32_t* __restrict__ tmp,
-                                                          uint32_t* __restrict__ err) {
-  __shared__ uint32_t s_hold[GH_WIN + GH_EXT];
-  __shared__ uint32_t s_wb[GH_PART_MAX + 1];
-  __shared__ uint32_t s_misc[2];
-  __shared__ uint32_t s_start;
-  for (uint32_t q = threadIdx.x; q <= g.p; q += blockDim.x) s_wb[q] = g.winbase[q];
+// cyclic order and re-solves each window whose predecessor's spill list
+// changed since the window was last solved: first those whose predecessor
+// spilled at all, then those behind a re-solve that changed its list.  It
+// starts at a window whose predecessor spilled nothing (if any) and stops
+// after a full lap with nothing to re-solve; work left after three laps
+// sets the overflow flag.
+__global__ __launch_bounds__(GH_SBLOCK) void k_gh_wfix(
+    WinGeom g, const int64_t* __restrict__ rk, const int64_t* __restrict__ rv,
+    const uint32_t* __restrict__ rr, int64_t* __restrict__ tk, int64_t* __restrict__ tv,
+    uint8_t* __restrict__ used, unsigned long long* __restrict__ spill,
+    uint32_t* __restrict__ nspill, uint32_t* __restrict__ err) {
+  __shared__ WinLds L;
+  __shared__ uint8_t s_dirty[GH_WQ_MAX];
+  for (uint32_t q = threadIdx.x; q <= g.p; q += GH_SBLOCK) L.wb[q] = g.winbase[q];
   __syncthreads();
   const uint32_t q = blockIdx.x;
-  const uint32_t w0 = s_wb[q], nw = s_wb[q + 1] - w0;
-  if (nw <= 1) return;
-  bool any = false;
-  for (uint32_t i = 0; i < nw; ++i) any |= nspill[w0 + i] != 0;
-  if (!any) return;
+  const uint32_t w0 = L.wb[q], nw = L.wb[q + 1] - w0;
+  if (nw <= 1) return;  // one window holds the whole table and never spills
+  for (uint32_t i = threadIdx.x; i < nw; i += GH_SBLOCK)
+    s_dirty[i] = nspill[w0 + (i + nw - 1) % nw] != 0 ? 1 : 0;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t st = nw;  // a window whose predecessor spilled nothing
-    for (uint32_t i = 0; i < nw && st == nw; ++i)
-      if (nspill[w0 + (i + nw - 1) % nw] == 0) st = i;
-    s_start = st;
+    uint32_t st = 0;
+    while (st < nw && s_dirty[st]) ++st;
+    L.start = st == nw ? 0 : st;
   }
   __syncthreads();
-  const uint32_t st = s_start == nw ? 0 : s_start;
-  const uint32_t laps = s_start == nw ? 3 : 1;
-  // the spill-ins of a window are copied out first: the window rewrites
-  // its own list, and with one window per partition it is its own
-  // predecessor
-  uint32_t* in = tmp + size_t(q) * GH_EXT;
-  for (uint32_t step = 0; step < laps * nw; ++step) {
-    const uint32_t wi = (st + step) % nw, w = w0 + wi, pw = w0 + (wi + nw - 1) % nw;
-    const uint32_t nin = nspill[pw];
-    if (nin == 0 && step < nw && laps == 1) continue;  // its alone-solution stands
-    for (uint32_t i = threadIdx.x; i < nin; i += blockDim.x) in[i] = spill[size_t(pw) * GH_EXT + i];
-    __syncthreads();
-    gh_solve_window(w, g, s_wb, pk, pv, perm, wstart, in, nin, hold, spill, nspill, s_hold,
-                    s_misc, err);
-    if (laps == 3 && step >= 2 * nw && s_misc[1] && threadIdx.x == 0) atomicOr(err, 32u);
+  const uint32_t st = L.start;
+  uint32_t clean = 0;  // windows passed in a row with nothing to do
+  for (uint32_t step = 0; step < 3 * nw && clean < nw; ++step) {
+    const uint32_t wi = (st + step) % nw;
+    if (!s_dirty[wi]) {  // uniform
+      ++clean;
+      continue;
+    }
+    clean = 0;
+    const uint32_t pw = w0 + (wi + nw - 1) % nw;
+    __syncthreads();  // every thread has read s_dirty[wi]
+    if (threadIdx.x == 0) s_dirty[wi] = 0;
+    // with two or more windows the predecessor's list is not the one the
+    // solve rewrites
+    if (!gh_wsolve(w0 + wi, g, rk, rv, rr, spill + size_t(pw) * GH_EXT, nspill[pw], tk, tv,
+                   used, spill, nspill, L, err))
+      return;
+    if (L.changed && threadIdx.x == 0) s_dirty[(wi + 1) % nw] = 1;
     __syncthreads();
   }
+  if (clean < nw && threadIdx.x == 0) {  // work left: a spill chain that does not settle
+    bool left = false;
+    for (uint32_t i = 0; i < nw; ++i) left |= s_dirty[i] != 0;
+    if (left) atomicOr(err, GH_ERR_WIN);
+  }
 }
-
 // 2. place: first rows claim slots in the real geometry by atomic min of
 // the row id; an evicted (larger) id is carried on from the next slot.
 __global__ __launch_bounds__(GH_BLOCK) void k_gh_place(const int64_t* __restrict__ pk,
@@ -897,80 +1246,118 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
   int64_t* dtk = st.out(tk, m, S_OUT0, true);
   int64_t* dtv = st.out(tv, m, S_OUT1, true);
   uint8_t* dused = st.out(used, m, S_OUT2, true);
-  uint32_t* hold = scratch_t<uint32_t>(S_T1, m);
-  uint8_t* first = scratch_t<uint8_t>(S_T2, n + 4);
   fill_u32(err, 0, 1, st.s);
   const PinsGeom g{dgeo, dgeo + p + 1, dgeo + 2 * p + 2, uint32_t(p), n};
-  const unsigned grow = grid_for(n, GH_BLOCK, 16 * num_cus());
-  const unsigned gslot = grid_for(m, GH_BLOCK, 16 * num_cus());
   ProfScope prof("ghash_pins", st.s, double(n) * 16.0 + double(m) * 17.0);
   {
-    fill_u32(hold, GH_EMPTY, m, st.s);
-    fill_u32(first, 0, (n + 3) / 4, st.s);
-    hipLaunchKernelGGL(k_gh_dedupe, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, g, hold, err);
-    NFH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gh_first, dim3(gslot), dim3(GH_BLOCK), 0, st.s, hold, m, first);
-    NFH_LAUNCH_CHECK();
-    // a table with no room for a new pair would make the placement probe
-    // forever (as the reference does): refuse before placing
-    const uint32_t e0 = read_scalar(err, st.s);
-    if (e0 != 0) {
-      st.finish();
-      return err_rc(e0);
-    }
-    fill_u32(hold, GH_EMPTY, m, st.s);
-    // windows of min(slots, GH_WIN) slots per partition
+    // windows of min(slots, GH_WIN) slots per partition; the windowed form
+    // needs every partition to have no more rows than slots (so no table
+    // can run full) and at most GH_WQ_MAX windows
     std::vector<uint32_t> wb(p + 1, 0);
+    bool windowed = true;
     for (size_t q = 0; q < p; ++q) {
       const uint64_t slots = uint64_t(hm[q]) + 1;
-      wb[q + 1] = wb[q] + uint32_t(slots / std::min<uint64_t>(slots, GH_WIN));
+      const uint64_t nwq = slots / std::min<uint64_t>(slots, GH_WIN);
+      if (uint64_t(hs[q + 1] - hs[q]) > slots || nwq > GH_WQ_MAX) windowed = false;
+      wb[q + 1] = wb[q] + uint32_t(std::min<uint64_t>(nwq, GH_WQ_MAX));
     }
     const uint32_t nwin = wb[p];
-    bool windowed = uint64_t(nwin) * GH_EXT * 4 <= (uint64_t(1) << 30);
+    windowed = windowed && uint64_t(nwin) * GH_EXT * 8 <= (uint64_t(1) << 31);
+    std::vector<WTile> tl;
+    std::vector<uint2> qe(p);
+    uint64_t ent = 0;
     if (windowed) {
-      int32_t* key = scratch_t<int32_t>(S_T3, n);
-      int32_t* perm = scratch_t<int32_t>(S_T4, n);
-      uint32_t* t0 = scratch_t<uint32_t>(S_T5, n);
-      uint32_t* t1 = scratch_t<uint32_t>(S_WORK0, n);
-      int32_t* tp = scratch_t<int32_t>(S_WORK1, n);
-      int32_t* skey = scratch_t<int32_t>(S_WORK2, n);
-      uint32_t* small = scratch_t<uint32_t>(S_WORK3, (p + 1) + (nwin + 1) + nwin + p * GH_EXT);
-      uint32_t* dwb = small;
-      uint32_t* wstart = dwb + p + 1;
-      uint32_t* nspill = wstart + nwin + 1;
-      uint32_t* tmpin = nspill + nwin;
-      uint32_t* spill = scratch_t<uint32_t>(S_MISC, size_t(nwin) * GH_EXT);
+      for (size_t q = 0; q < p; ++q) {
+        const uint64_t r0 = uint64_t(hs[q]), r1 = uint64_t(hs[q + 1]);
+        const uint64_t slots = uint64_t(hm[q]) + 1;
+        const uint32_t wsz = uint32_t(std::min<uint64_t>(slots, GH_WIN));
+        uint32_t lw = 0;
+        while ((1u << lw) < wsz) ++lw;
+        const uint32_t nwq = wb[q + 1] - wb[q];
+        const uint32_t nt = uint32_t((r1 - r0 + GH_WTILE - 1) / GH_WTILE);
+        qe[q] = make_uint2(uint32_t(ent), nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+          const uint64_t r = r0 + uint64_t(t) * GH_WTILE;
+          tl.push_back({uint32_t(r), uint32_t(std::min<uint64_t>(r + GH_WTILE, r1)),
+                        uint32_t(ent + t), nt, uint32_t(q), nwq, lw, 0});
+        }
+        ent += uint64_t(nt) * nwq;
+      }
+      windowed = ent < (uint64_t(1) << 28) && !tl.empty();
+    }
+    if (windowed) {
+      const uint32_t np = uint32_t(ent), nc = (np + GH_ECHUNK - 1) / GH_ECHUNK;
+      int64_t* rk = scratch_t<int64_t>(S_T3, n);
+      int64_t* rv = scratch_t<int64_t>(S_T4, n);
+      uint32_t* rr = scratch_t<uint32_t>(S_T5, n);
+      uint32_t* ecnt = scratch_t<uint32_t>(S_WORK0, np);
+      uint32_t* eoff = scratch_t<uint32_t>(S_WORK1, size_t(np) + 1);
+      unsigned long long* spill = scratch_t<unsigned long long>(S_MISC, size_t(nwin) * GH_EXT);
+      unsigned char* aux = static_cast<unsigned char*>(
+          scratch(S_WORK2, tl.size() * sizeof(WTile) + p * sizeof(uint2) + (p + 1) * 4 +
+                               size_t(nwin) * 4 + size_t(nc + 1) * 4 + 256));
+      WTile* dtl = reinterpret_cast<WTile*>(aux);
+      uint2* dqe = reinterpret_cast<uint2*>(dtl + tl.size());
+      uint32_t* dwb = reinterpret_cast<uint32_t*>(dqe + p);
+      uint32_t* nspill = dwb + p + 1;
+      uint32_t* csum = nspill + nwin;
+      NFH_CHECK(hipMemcpyAsync(dtl, tl.data(), tl.size() * sizeof(WTile), hipMemcpyHostToDevice,
+                               st.s));
+      NFH_CHECK(hipMemcpyAsync(dqe, qe.data(), p * sizeof(uint2), hipMemcpyHostToDevice, st.s));
       NFH_CHECK(hipMemcpyAsync(dwb, wb.data(), (p + 1) * 4, hipMemcpyHostToDevice, st.s));
       fill_u32(nspill, 0, nwin, st.s);
-      hipLaunchKernelGGL(k_gh_winkey, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, first, g, dwb,
-                         key);
+      const uint32_t ntl = uint32_t(tl.size());
+      hipLaunchKernelGGL(k_gh_whist, dim3(ntl), dim3(GH_WTHREADS), 0, st.s, dpk, dpv, dtl, g.tm,
+                         ecnt);
       NFH_LAUNCH_CHECK();
-      radix_argsort<int32_t>(key, n, false, perm, t0, t1, tp, skey, st.s);
-      hipLaunchKernelGGL(k_gh_winstart, dim3((nwin + 1 + 255) / 256), dim3(256), 0, st.s, skey,
-                         uint64_t(n), nwin, wstart);
+      hipLaunchKernelGGL(k_gh_esum, dim3(nc), dim3(GH_SCAN), 0, st.s, ecnt, np, csum);
       NFH_LAUNCH_CHECK();
-      const WinGeom wg{g.toff, g.tm, dwb, uint32_t(p), nwin};
-      hipLaunchKernelGGL(k_gh_win, dim3(nwin), dim3(GH_WBLOCK), 0, st.s, wg, dpk, dpv, perm,
-                         wstart, hold, spill, nspill, err);
+      hipLaunchKernelGGL(k_gh_etop, dim3(1), dim3(GH_SCAN), 0, st.s, csum, nc, eoff, np);
       NFH_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gh_win_fix, dim3(p), dim3(GH_WBLOCK), 0, st.s, wg, dpk, dpv, perm,
-                         wstart, hold, spill, nspill, tmpin, err);
+      hipLaunchKernelGGL(k_gh_escan, dim3(nc), dim3(GH_SCAN), 0, st.s, ecnt, np, csum, eoff);
       NFH_LAUNCH_CHECK();
-      const uint32_t e1 = read_scalar(err, st.s);
-      if (e1 & 32u) {  // a cluster longer than the extension: global form
-        fill_u32(err, e1 & ~32u, 1, st.s);
-        fill_u32(hold, GH_EMPTY, m, st.s);
+      hipLaunchKernelGGL(k_gh_wscatter, dim3(ntl), dim3(GH_WTHREADS), 0, st.s, dpk, dpv, dtl,
+                         g.tm, eoff, rk, rv, rr);
+      NFH_LAUNCH_CHECK();
+      const WinGeom wg{g.toff, g.tm, dwb, dqe, eoff, uint32_t(p), nwin};
+      hipLaunchKernelGGL(k_gh_wsolve, dim3(nwin), dim3(GH_SBLOCK), 0, st.s, wg, rk, rv, rr, dtk,
+                         dtv, dused, spill, nspill, err);
+      NFH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gh_wfix, dim3(uint32_t(p)), dim3(GH_SBLOCK), 0, st.s, wg, rk, rv, rr,
+                         dtk, dtv, dused, spill, nspill, err);
+      NFH_LAUNCH_CHECK();
+      if (read_scalar(err, st.s) & GH_ERR_WIN) {  // not solvable in LDS: global form
+        fill_u32(err, 0, 1, st.s);
         windowed = false;
       }
     }
     if (!windowed) {
+      uint32_t* hold = scratch_t<uint32_t>(S_T1, m);
+      uint8_t* first = scratch_t<uint8_t>(S_T2, n + 4);
+      const unsigned grow = grid_for(n, GH_BLOCK, 16 * num_cus());
+      const unsigned gslot = grid_for(m, GH_BLOCK, 16 * num_cus());
+      fill_u32(first, 0, (n + 3) / 4, st.s);
+      fill_u32(hold, GH_EMPTY, m, st.s);
+      hipLaunchKernelGGL(k_gh_dedupe, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, g, hold,
+                         err);
+      NFH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gh_first, dim3(gslot), dim3(GH_BLOCK), 0, st.s, hold, m, first);
+      NFH_LAUNCH_CHECK();
+      // a table with no room for a new pair would make the placement probe
+      // forever (as the reference does): refuse before placing
+      const uint32_t e0 = read_scalar(err, st.s);
+      if (e0 != 0) {
+        st.finish();
+        return err_rc(e0);
+      }
+      fill_u32(hold, GH_EMPTY, m, st.s);
       hipLaunchKernelGGL(k_gh_place, dim3(grow), dim3(GH_BLOCK), 0, st.s, dpk, dpv, first, g,
                          hold, err);
       NFH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gh_materialize, dim3(gslot), dim3(GH_BLOCK), 0, st.s, hold, m, dpk,
+                         dpv, dtk, dtv, dused);
+      NFH_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_gh_materialize, dim3(gslot), dim3(GH_BLOCK), 0, st.s, hold, m, dpk, dpv,
-                       dtk, dtv, dused);
-    NFH_LAUNCH_CHECK();
   }
   const uint32_t e = read_scalar(err, st.s);
   st.finish();
