@@ -27,6 +27,7 @@
 // aborts the process) or probe forever: -2 (partition id or reservation
 // outside its buffer), -3 (a table with no free slot left for a new pair).
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "device.hpp"
@@ -291,6 +292,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_dedupe(const int64_t* __restric
   __syncthreads();
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < g.n; j += stride) {
+    if (j < uint64_t(s_starts[0])) continue;  // before the first partition
     const uint32_t q = gh_part_of(s_starts, g.p, j);
     const uint64_t mask = uint64_t(g.tm[q]);
     uint32_t* tab = hold + g.toff[q];
@@ -360,15 +362,15 @@ __global__ void k_gh_first(const uint32_t* __restrict__ hold, uint64_t m,
 // table holds, sets bit 32 of err and the caller runs the global form,
 // which rewrites every occupied slot.  The windowed form runs only when no
 // partition has more rows than slots, so it never meets a full table.
-constexpr int GH_WIN = 4096;             // window slots
+constexpr int GH_WIN = 2048;             // window slots
 constexpr int GH_EXT = 1024;             // extension slots
 constexpr int GH_WTHREADS = 512;         // threads of the tile kernels
 constexpr int GH_WIPT = 16;
 constexpr uint32_t GH_WTILE = uint32_t(GH_WTHREADS) * GH_WIPT;  // 8192 rows
 constexpr uint32_t GH_WQ_MAX = 4096;     // windows per partition
 constexpr int GH_SBLOCK = 512;           // threads of the window solve
-constexpr uint32_t GH_SD = 4096;         // LDS hash table slots of the window solve
-constexpr int GH_SU = 4;                 // records per thread per pass
+constexpr uint32_t GH_SD = 2048;         // LDS hash table slots of the window solve
+constexpr int GH_SU = 2;                 // records per thread per pass
 constexpr unsigned long long GH_NONE = ~0ull;
 constexpr uint32_t GH_ERR_WIN = 32u;     // fall back to the global form
 
@@ -384,6 +386,16 @@ __device__ __forceinline__ uint64_t gh_dedupe_hash(int64_t k, int64_t v) {
   return h;
 }
 
+struct WPart {
+  uint32_t tb;      // first tile
+  uint32_t e0;      // entry of (window 0, tile 0); (window i, tile t) at e0 + i * nt + t
+  uint32_t nt;      // tiles
+  uint32_t nwq;     // windows
+  uint32_t r0, r1;  // rows
+  uint32_t lw;      // log2 of the window size
+  uint32_t pad;
+};
+
 struct WTile {
   uint32_t r0, r1;  // rows
   uint32_t e0;      // entry of (window 0, this tile); window i at e0 + i * nt
@@ -393,6 +405,27 @@ struct WTile {
   uint32_t lw;      // log2 of the window size
   uint32_t pad;
 };
+
+// tiles[b] from the per-partition table: tile b belongs to the last
+// partition whose first tile is at most b (partitions without rows own no
+// tile and never are that last one).
+__global__ void k_gh_wtiles(const WPart* __restrict__ parts, uint32_t p, uint32_t ntl,
+                            WTile* __restrict__ tiles) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ntl) return;
+  uint32_t lo = 0, hi = p;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (parts[mid].tb <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const WPart w = parts[lo];
+  const uint32_t t = b - w.tb;
+  const uint32_t r = w.r0 + t * GH_WTILE;
+  tiles[b] = WTile{r, min(r + GH_WTILE, w.r1), w.e0 + t, w.nt, lo, w.nwq, w.lw, 0};
+}
 
 // Window (within its partition) of the reference home of pair (k, v).
 __device__ __forceinline__ uint32_t gh_win_of(int64_t k, int64_t v, uint64_t mask, uint32_t q,
@@ -574,8 +607,7 @@ struct WinGeom {
   const int64_t* toff;       // p + 1
   const int64_t* tm;         // p
   const uint32_t* winbase;   // p + 1: first window of each partition
-  const uint2* qe;           // p: (entry of (window 0, tile 0), tiles) of each partition
-  const uint32_t* off;       // first record of each entry (np + 1)
+  const uint4* desc;         // nwin: (first record, end record, partition, window in it)
   uint32_t p;
   uint32_t nwin;
 };
@@ -592,6 +624,18 @@ __device__ __forceinline__ uint32_t gh_win_part(const uint32_t* s_wb, uint32_t p
   return lo;
 }
 
+// desc[w] of every window (records from the scanned entries).
+__global__ void k_gh_wdesc(const uint32_t* __restrict__ winbase, uint32_t p, uint32_t nwin,
+                           const WPart* __restrict__ parts, const uint32_t* __restrict__ off,
+                           uint4* __restrict__ desc) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwin) return;
+  const uint32_t q = gh_win_part(winbase, p, w);  // every partition has a window
+  const uint32_t wi = w - winbase[q];
+  const uint32_t e0 = parts[q].e0, nt = parts[q].nt;
+  desc[w] = make_uint4(off[e0 + wi * nt], off[e0 + (wi + 1) * nt], q, wi);
+}
+
 // LDS address of window slot i.  A partition's reference homes are
 // congruent modulo P (a multiple of the 32 two-bank slots of 64-bit
 // entries), so unswizzled every claim of a window would start in one bank
@@ -603,18 +647,18 @@ __device__ __forceinline__ uint32_t gh_swz(uint32_t i) { return i ^ ((i >> 8) & 
 
 union WinTable {
   struct {
-    unsigned long long key[GH_SD];  // (hash tag << 32) | claimer's record; GH_NONE if free
+    unsigned long long key[GH_SD];  // pair hash; GH_NONE if free
+    uint32_t rec[GH_SD];            // record of the slot's claimer
     uint32_t first[GH_SD];          // smallest row id of the pair
   } d;
   unsigned long long hold[GH_WIN + GH_EXT];  // (row id << 32) | record; GH_NONE if free
   uint32_t rank[GH_WIN + 2 * GH_SD];          // home counters, grouped rows, ranks
 };
-static_assert(sizeof(WinTable) <= 48 * 1024, "two window solves per CU");
+static_assert(sizeof(WinTable) <= 32 * 1024, "four window solves per CU");
 
 struct WinLds {
   WinTable u;
   uint16_t home[GH_SD];  // home slot within the window, set by the claimer
-  uint32_t wb[GH_PART_MAX + 1];
   uint32_t bad, over, changed, start;
   uint32_t tmp[GH_SBLOCK / WAVE];
 };
@@ -632,16 +676,14 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
                           uint32_t* __restrict__ nspill, WinLds& L,
                           uint32_t* __restrict__ err) {
   constexpr int PER = GH_SD / GH_SBLOCK;
-  const uint32_t q = gh_win_part(L.wb, g.p, w);
+  const uint4 dsc = g.desc[w];
+  const uint32_t r0 = dsc.x, r1 = dsc.y, q = dsc.z, wi = dsc.w;
   const uint64_t mask = uint64_t(g.tm[q]);
   const uint32_t wsz = mask + 1 < uint64_t(GH_WIN) ? uint32_t(mask + 1) : uint32_t(GH_WIN);
   const bool whole = uint64_t(wsz) == mask + 1;
   const uint32_t ext = whole ? 0u : uint32_t(GH_EXT);
   const uint32_t span = wsz + ext;
-  const uint32_t wi = w - L.wb[q];
   const uint64_t a = (uint64_t(q) + uint64_t(wi) * wsz) & mask;  // first slot (cyclic)
-  const uint2 qe = g.qe[q];
-  const uint32_t r0 = g.off[qe.x + wi * qe.y], r1 = g.off[qe.x + (wi + 1) * qe.y];
   const uint32_t nrec = r1 - r0;
   uint32_t sd = 64;
   while (sd < GH_SD && sd < 2 * nrec) sd *= 2;
@@ -654,10 +696,13 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
     L.over = 0;
   }
   __syncthreads();
-  // 1. distinct pairs of the window's records
+  // 1. distinct pairs of the window's records: a record claims a free slot
+  // with its pair's hash or joins the slot holding that hash; after each
+  // pass every record that joined compares its pair with the claimer's
+  // (which never changes), so a hash shared by two pairs is always found
   for (uint32_t i0 = r0; i0 < r1; i0 += GH_SBLOCK * GH_SU) {  // uniform trip count
     int64_t k[GH_SU], v[GH_SU];
-    uint32_t row[GH_SU];
+    uint32_t row[GH_SU], slot[GH_SU];
 #pragma unroll
     for (int u = 0; u < GH_SU; ++u) {
       const uint32_t i = i0 + uint32_t(u) * GH_SBLOCK + threadIdx.x;
@@ -668,35 +713,41 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
 #pragma unroll
     for (int u = 0; u < GH_SU; ++u) {
       const uint32_t i = i0 + uint32_t(u) * GH_SBLOCK + threadIdx.x;
+      slot[u] = GH_EMPTY;
       if (i >= r1) continue;
-      const uint64_t h = gh_dedupe_hash(k[u], v[u]);
-      const uint32_t tag = uint32_t(h >> 32);
-      const unsigned long long mine = (uint64_t(tag) << 32) | i;
-      // the probe loop only finds the slot; the claimer's store and the
+      uint64_t h = gh_dedupe_hash(k[u], v[u]);
+      if (h == GH_NONE) h = GH_NONE - 1;
+      // the probe loop only finds the slot; the claimer's stores and the
       // row minimum follow it
-      uint32_t sl = uint32_t(h) & (sd - 1), slot = GH_EMPTY;
+      uint32_t sl = uint32_t(h) & (sd - 1);
       bool claimed = false;
       for (uint32_t probe = 0; probe < sd; ++probe) {
-        const unsigned long long old = atomicCAS(&L.u.d.key[sl], GH_NONE, mine);
-        bool hit = old == GH_NONE;
-        if (!hit && uint32_t(old >> 32) == tag) {
-          const uint32_t o = uint32_t(old);
-          hit = rk[o] == k[u] && rv[o] == v[u];
-        }
-        if (hit) {
+        const unsigned long long old = atomicCAS(&L.u.d.key[sl], GH_NONE, h);
+        if (old == GH_NONE || old == h) {
           claimed = old == GH_NONE;
-          slot = sl;
+          slot[u] = sl;
           break;
         }
         sl = (sl + 1) & (sd - 1);
       }
-      if (slot == GH_EMPTY) {
+      if (slot[u] == GH_EMPTY) {
         L.bad = 1u;  // more distinct pairs than table slots
         continue;
       }
-      if (claimed)
-        L.home[slot] = uint16_t(((gh_pair_fp(k[u], v[u]) & mask) - a) & mask);
-      atomicMin(&L.u.d.first[slot], row[u]);
+      if (claimed) {
+        L.u.d.rec[slot[u]] = i;
+        L.home[slot[u]] = uint16_t(((gh_pair_fp(k[u], v[u]) & mask) - a) & mask);
+      }
+      atomicMin(&L.u.d.first[slot[u]], row[u]);
+    }
+    __syncthreads();  // every claimer has stored its record
+#pragma unroll
+    for (int u = 0; u < GH_SU; ++u) {
+      const uint32_t i = i0 + uint32_t(u) * GH_SBLOCK + threadIdx.x;
+      if (slot[u] != GH_EMPTY) {
+        const uint32_t c = L.u.d.rec[slot[u]];
+        if (c != i && (rk[c] != k[u] || rv[c] != v[u])) L.bad = 1u;  // two pairs, one hash
+      }
     }
   }
   __syncthreads();
@@ -715,7 +766,7 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
     if (j < sd) {
       const unsigned long long key = L.u.d.key[j];
       if (key != GH_NONE) {
-        id[s] = (uint64_t(L.u.d.first[j]) << 32) | uint32_t(key);
+        id[s] = (uint64_t(L.u.d.first[j]) << 32) | L.u.d.rec[j];
         c0[s] = L.home[j];
       }
     }
@@ -769,8 +820,15 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
     for (uint32_t j = threadIdx.x; j < total; j += GH_SBLOCK) {
       const uint32_t c = s_rk[j], row = s_rows[j];
       const uint32_t b = s_hc[c], e = c + 1 < wsz ? s_hc[c + 1] : total;
-      uint32_t r = 0;
-      for (uint32_t i = b; i < e; ++i) r += s_rows[i] < row ? 1u : 0u;
+      uint32_t r = 0, i = b;
+      for (; i + 8 <= e; i += 8) {  // eight independent reads in flight
+        uint32_t x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = s_rows[i + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r += x[k] < row ? 1u : 0u;
+      }
+      for (; i < e; ++i) r += s_rows[i] < row ? 1u : 0u;
       s_rk[j] = r;
     }
     __syncthreads();
@@ -817,8 +875,11 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
   }
   // 4. the window's slots
   const uint64_t base = uint64_t(g.toff[q]);
-  for (uint32_t i = threadIdx.x; i < wsz; i += GH_SBLOCK) {
-    const unsigned long long x = L.u.hold[gh_swz(i)];
+  constexpr int OPT = GH_WIN / GH_SBLOCK;
+#pragma unroll
+  for (int j = 0; j < OPT; ++j) {
+    const uint32_t i = threadIdx.x + uint32_t(j) * GH_SBLOCK;
+    const unsigned long long x = i < wsz ? L.u.hold[gh_swz(i)] : GH_NONE;
     if (x != GH_NONE) {
       const uint32_t o = uint32_t(x);
       const uint64_t s = base + ((a + i) & mask);
@@ -827,7 +888,10 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
       used[s] = 1;
     }
   }
-  if (ext == 0) {
+  // an extension holding nothing spills nothing (the usual case)
+  bool occ = false;
+  for (uint32_t i = threadIdx.x; i < ext; i += GH_SBLOCK) occ |= L.u.hold[gh_swz(wsz + i)] != GH_NONE;
+  if (!__syncthreads_or(occ)) {
     if (threadIdx.x == 0) {
       L.changed = nspill[w] != 0 ? 1u : 0u;
       nspill[w] = 0;
@@ -870,8 +934,6 @@ __global__ __launch_bounds__(GH_SBLOCK) void k_gh_wsolve(
     uint8_t* __restrict__ used, unsigned long long* __restrict__ spill,
     uint32_t* __restrict__ nspill, uint32_t* __restrict__ err) {
   __shared__ WinLds L;
-  for (uint32_t q = threadIdx.x; q <= g.p; q += GH_SBLOCK) L.wb[q] = g.winbase[q];
-  __syncthreads();
   (void)gh_wsolve(blockIdx.x, g, rk, rv, rr, nullptr, 0, tk, tv, used, spill, nspill, L, err);
 }
 
@@ -889,10 +951,8 @@ __global__ __launch_bounds__(GH_SBLOCK) void k_gh_wfix(
     uint32_t* __restrict__ nspill, uint32_t* __restrict__ err) {
   __shared__ WinLds L;
   __shared__ uint8_t s_dirty[GH_WQ_MAX];
-  for (uint32_t q = threadIdx.x; q <= g.p; q += GH_SBLOCK) L.wb[q] = g.winbase[q];
-  __syncthreads();
   const uint32_t q = blockIdx.x;
-  const uint32_t w0 = L.wb[q], nw = L.wb[q + 1] - w0;
+  const uint32_t w0 = g.winbase[q], nw = g.winbase[q + 1] - w0;
   if (nw <= 1) return;  // one window holds the whole table and never spills
   for (uint32_t i = threadIdx.x; i < nw; i += GH_SBLOCK)
     s_dirty[i] = nspill[w0 + (i + nw - 1) % nw] != 0 ? 1 : 0;
@@ -1238,55 +1298,54 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
   if (n == 0) return 0;  // no rows: the tables are left as they are
   const int64_t* dpk = st.in(pk, n, S_IN0);
   const int64_t* dpv = st.in(pv, n, S_IN1);
-  int64_t* dgeo = scratch_t<int64_t>(S_T0, 3 * p + 2 + 2);
-  NFH_CHECK(hipMemcpyAsync(dgeo, hs.data(), (p + 1) * 8, hipMemcpyHostToDevice, st.s));
-  NFH_CHECK(hipMemcpyAsync(dgeo + p + 1, ho.data(), (p + 1) * 8, hipMemcpyHostToDevice, st.s));
-  NFH_CHECK(hipMemcpyAsync(dgeo + 2 * p + 2, hm.data(), p * 8, hipMemcpyHostToDevice, st.s));
-  uint32_t* err = reinterpret_cast<uint32_t*>(dgeo + 3 * p + 2);
+  // windows of min(slots, GH_WIN) slots per partition; the windowed form
+  // needs every partition to have no more rows than slots (so no table can
+  // run full) and at most GH_WQ_MAX windows
+  std::vector<uint32_t> wb(p + 1, 0);
+  std::vector<WPart> parts(p);
+  bool windowed = true;
+  uint64_t ent = 0, ntiles = 0;
+  for (size_t q = 0; q < p; ++q) {
+    const uint64_t r0 = uint64_t(hs[q]), r1 = uint64_t(hs[q + 1]);
+    const uint64_t slots = uint64_t(hm[q]) + 1;
+    const uint32_t wsz = uint32_t(std::min<uint64_t>(slots, GH_WIN));
+    const uint64_t nwq = std::min<uint64_t>(slots / wsz, GH_WQ_MAX);
+    if (r1 - r0 > slots || slots / wsz > GH_WQ_MAX) windowed = false;
+    wb[q + 1] = wb[q] + uint32_t(nwq);
+    uint32_t lw = 0;
+    while ((1u << lw) < wsz) ++lw;
+    const uint32_t nt = uint32_t((r1 - r0 + GH_WTILE - 1) / GH_WTILE);
+    parts[q] = WPart{uint32_t(ntiles), uint32_t(ent), nt, uint32_t(nwq), uint32_t(r0),
+                     uint32_t(r1), lw, 0};
+    ntiles += nt;
+    ent += uint64_t(nt) * nwq;
+  }
+  const uint32_t nwin = wb[p];
+  windowed = windowed && uint64_t(nwin) * GH_EXT * 8 <= (uint64_t(1) << 31) &&
+             ent < (uint64_t(1) << 28) && ntiles != 0;
+  // one copy: starts, toff, tm, the error word, the partition table, wb
+  const size_t geo_b = (3 * p + 3) * 8, part_b = p * sizeof(WPart), wb_b = (p + 1) * 4;
+  std::vector<unsigned char> blob(geo_b + part_b + wb_b, 0);
+  std::memcpy(blob.data(), hs.data(), (p + 1) * 8);
+  std::memcpy(blob.data() + (p + 1) * 8, ho.data(), (p + 1) * 8);
+  std::memcpy(blob.data() + (2 * p + 2) * 8, hm.data(), p * 8);
+  std::memcpy(blob.data() + geo_b, parts.data(), part_b);
+  std::memcpy(blob.data() + geo_b + part_b, wb.data(), wb_b);
+  unsigned char* dblob = static_cast<unsigned char*>(scratch(S_T0, blob.size()));
+  NFH_CHECK(hipMemcpyAsync(dblob, blob.data(), blob.size(), hipMemcpyHostToDevice, st.s));
+  int64_t* dgeo = reinterpret_cast<int64_t*>(dblob);
+  uint32_t* err = reinterpret_cast<uint32_t*>(dgeo + 3 * p + 2);  // zero from the copy
+  const WPart* dparts = reinterpret_cast<const WPart*>(dblob + geo_b);
+  const uint32_t* dwb = reinterpret_cast<const uint32_t*>(dblob + geo_b + part_b);
   int64_t* dtk = st.out(tk, m, S_OUT0, true);
   int64_t* dtv = st.out(tv, m, S_OUT1, true);
   uint8_t* dused = st.out(used, m, S_OUT2, true);
-  fill_u32(err, 0, 1, st.s);
   const PinsGeom g{dgeo, dgeo + p + 1, dgeo + 2 * p + 2, uint32_t(p), n};
   ProfScope prof("ghash_pins", st.s, double(n) * 16.0 + double(m) * 17.0);
   {
-    // windows of min(slots, GH_WIN) slots per partition; the windowed form
-    // needs every partition to have no more rows than slots (so no table
-    // can run full) and at most GH_WQ_MAX windows
-    std::vector<uint32_t> wb(p + 1, 0);
-    bool windowed = true;
-    for (size_t q = 0; q < p; ++q) {
-      const uint64_t slots = uint64_t(hm[q]) + 1;
-      const uint64_t nwq = slots / std::min<uint64_t>(slots, GH_WIN);
-      if (uint64_t(hs[q + 1] - hs[q]) > slots || nwq > GH_WQ_MAX) windowed = false;
-      wb[q + 1] = wb[q] + uint32_t(std::min<uint64_t>(nwq, GH_WQ_MAX));
-    }
-    const uint32_t nwin = wb[p];
-    windowed = windowed && uint64_t(nwin) * GH_EXT * 8 <= (uint64_t(1) << 31);
-    std::vector<WTile> tl;
-    std::vector<uint2> qe(p);
-    uint64_t ent = 0;
-    if (windowed) {
-      for (size_t q = 0; q < p; ++q) {
-        const uint64_t r0 = uint64_t(hs[q]), r1 = uint64_t(hs[q + 1]);
-        const uint64_t slots = uint64_t(hm[q]) + 1;
-        const uint32_t wsz = uint32_t(std::min<uint64_t>(slots, GH_WIN));
-        uint32_t lw = 0;
-        while ((1u << lw) < wsz) ++lw;
-        const uint32_t nwq = wb[q + 1] - wb[q];
-        const uint32_t nt = uint32_t((r1 - r0 + GH_WTILE - 1) / GH_WTILE);
-        qe[q] = make_uint2(uint32_t(ent), nt);
-        for (uint32_t t = 0; t < nt; ++t) {
-          const uint64_t r = r0 + uint64_t(t) * GH_WTILE;
-          tl.push_back({uint32_t(r), uint32_t(std::min<uint64_t>(r + GH_WTILE, r1)),
-                        uint32_t(ent + t), nt, uint32_t(q), nwq, lw, 0});
-        }
-        ent += uint64_t(nt) * nwq;
-      }
-      windowed = ent < (uint64_t(1) << 28) && !tl.empty();
-    }
     if (windowed) {
       const uint32_t np = uint32_t(ent), nc = (np + GH_ECHUNK - 1) / GH_ECHUNK;
+      const uint32_t ntl = uint32_t(ntiles);
       int64_t* rk = scratch_t<int64_t>(S_T3, n);
       int64_t* rv = scratch_t<int64_t>(S_T4, n);
       uint32_t* rr = scratch_t<uint32_t>(S_T5, n);
@@ -1294,19 +1353,16 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
       uint32_t* eoff = scratch_t<uint32_t>(S_WORK1, size_t(np) + 1);
       unsigned long long* spill = scratch_t<unsigned long long>(S_MISC, size_t(nwin) * GH_EXT);
       unsigned char* aux = static_cast<unsigned char*>(
-          scratch(S_WORK2, tl.size() * sizeof(WTile) + p * sizeof(uint2) + (p + 1) * 4 +
-                               size_t(nwin) * 4 + size_t(nc + 1) * 4 + 256));
-      WTile* dtl = reinterpret_cast<WTile*>(aux);
-      uint2* dqe = reinterpret_cast<uint2*>(dtl + tl.size());
-      uint32_t* dwb = reinterpret_cast<uint32_t*>(dqe + p);
-      uint32_t* nspill = dwb + p + 1;
+          scratch(S_WORK2, size_t(ntl) * sizeof(WTile) + size_t(nwin) * 20 +
+                               size_t(nc + 1) * 4 + 256));
+      uint4* ddesc = reinterpret_cast<uint4*>(aux);
+      WTile* dtl = reinterpret_cast<WTile*>(ddesc + nwin);
+      uint32_t* nspill = reinterpret_cast<uint32_t*>(dtl + ntl);
       uint32_t* csum = nspill + nwin;
-      NFH_CHECK(hipMemcpyAsync(dtl, tl.data(), tl.size() * sizeof(WTile), hipMemcpyHostToDevice,
-                               st.s));
-      NFH_CHECK(hipMemcpyAsync(dqe, qe.data(), p * sizeof(uint2), hipMemcpyHostToDevice, st.s));
-      NFH_CHECK(hipMemcpyAsync(dwb, wb.data(), (p + 1) * 4, hipMemcpyHostToDevice, st.s));
       fill_u32(nspill, 0, nwin, st.s);
-      const uint32_t ntl = uint32_t(tl.size());
+      hipLaunchKernelGGL(k_gh_wtiles, dim3((ntl + 255) / 256), dim3(256), 0, st.s, dparts,
+                         uint32_t(p), ntl, dtl);
+      NFH_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_gh_whist, dim3(ntl), dim3(GH_WTHREADS), 0, st.s, dpk, dpv, dtl, g.tm,
                          ecnt);
       NFH_LAUNCH_CHECK();
@@ -1319,7 +1375,10 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
       hipLaunchKernelGGL(k_gh_wscatter, dim3(ntl), dim3(GH_WTHREADS), 0, st.s, dpk, dpv, dtl,
                          g.tm, eoff, rk, rv, rr);
       NFH_LAUNCH_CHECK();
-      const WinGeom wg{g.toff, g.tm, dwb, dqe, eoff, uint32_t(p), nwin};
+      hipLaunchKernelGGL(k_gh_wdesc, dim3((nwin + 255) / 256), dim3(256), 0, st.s, dwb,
+                         uint32_t(p), nwin, dparts, eoff, ddesc);
+      NFH_LAUNCH_CHECK();
+      const WinGeom wg{g.toff, g.tm, dwb, ddesc, uint32_t(p), nwin};
       hipLaunchKernelGGL(k_gh_wsolve, dim3(nwin), dim3(GH_SBLOCK), 0, st.s, wg, rk, rv, rr, dtk,
                          dtv, dused, spill, nspill, err);
       NFH_LAUNCH_CHECK();

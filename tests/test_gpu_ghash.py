@@ -135,6 +135,38 @@ def test_pins_layout_identical(lib, ora, n, P, kind):
         check_linear_probing_layout(pk, pv, starts, *g[1:])
 
 
+@pytest.mark.parametrize("n,P,slotf", [(1 << 20, 4, 1.1), (300_000, 2, 1.3), (60_000, 1, 1.0)])
+def test_pins_windows_that_spill(lib, ora, n, P, slotf):
+    """All-distinct pairs in few partitions at half to 0.92 load: tables of
+    many windows whose clusters cross window ends (spilled ids, the
+    re-solve walk) or outgrow the extension (the global form)."""
+    rng = np.random.default_rng(n + P)
+    pk, pv, starts, mp = _pins_setup(rng, n, P, "wide", slotf)
+    g = _pins(lib, pk, pv, starts, mp)
+    o = _pins(ora, pk, pv, starts, mp)
+    assert g[0] == o[0] == 0
+    for a, b in zip(g[1:], o[1:]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("slotf", [1.5, 0.1])
+def test_pins_rows_before_first_partition(lib, ora, slotf):
+    """Rows before starts[0] belong to no partition and are not inserted,
+    with tables of at least as many slots as rows (windowed form) and with
+    fewer (global form)."""
+    rng = np.random.default_rng(17)
+    pk, pv, starts, mp = _pins_setup(rng, 20_000, 16, "pool", slotf)
+    junk = 37
+    pk = np.concatenate([rng.integers(-5, 5, junk), pk]).astype(np.int64)
+    pv = np.concatenate([rng.integers(-5, 5, junk), pv]).astype(np.int64)
+    starts = starts + junk
+    g = _pins(lib, pk, pv, starts, mp)
+    o = _pins(ora, pk, pv, starts, mp)
+    assert g[0] == o[0] == 0
+    for a, b in zip(g[1:], o[1:]):
+        assert np.array_equal(a, b)
+
+
 def test_pins_dense_tables_and_refusals(lib, ora):
     """Tables at their limit (distinct == slots), and one past it (-3
     where the reference would probe forever)."""
