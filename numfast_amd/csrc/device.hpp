@@ -14,6 +14,16 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (uint64_t(1) << lane_id()) - 1;
 }
 
+// Tile of workgroup b in a grid of t workgroups, one per tile.  MI355X
+// dispatches workgroup b to XCD b % 8, each XCD with its own L2; here XCD
+// x takes the contiguous tiles [x*(t/8) + min(x, t%8), ...) in order, so
+// neighbouring tiles, whose output runs meet inside 128-byte lines, merge
+// those lines in one L2 instead of writing partial lines from two.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t t) {
+  const uint32_t x = b & 7u, base = t >> 3, rem = t & 7u;
+  return x * base + (x < rem ? x : rem) + (b >> 3);
+}
+
 // Lanes of this wave whose 8-bit digit equals mine (restricted to `valid`
 // lanes).  Eight ballots, one per digit bit; every lane must execute it.
 __device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {

@@ -178,28 +178,41 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_tile_base(const uint32_t* __res
   }
 }
 
-// One workgroup per tile: stable in-tile ranks per partition (item j of
-// lane l of wave w is row r0 + w*64*IPT + j*64 + l; per-wave LDS counters
-// give ranks in that order), then the pairs go to their reserved lanes.
+// One workgroup per tile (tiles in XCD order): stable in-tile ranks per
+// partition (item j of lane l of wave w is row r0 + w*64*IPT + j*64 + l;
+// per-wave LDS counters give ranks in that order), then the tile's pairs
+// are staged in LDS in (partition, rank) order, keys first and then
+// values, and stored from there, so consecutive lanes store consecutive
+// rows of one partition's run.
+constexpr int GH_SCAT_BLOCK = 512;
+constexpr int GH_SCAT_IPT = GH_TILE / GH_SCAT_BLOCK;
+static_assert(GH_SCAT_IPT * GH_SCAT_BLOCK == GH_TILE, "scatter tiles are count tiles");
 template <bool LR>
-__global__ __launch_bounds__(GH_BLOCK) void k_gh_scatter(
+__global__ __launch_bounds__(GH_SCAT_BLOCK) void k_gh_scatter(
     const int64_t* __restrict__ k, const int64_t* __restrict__ v, const int64_t* __restrict__ fp,
     TileGeom g, uint32_t p, uint32_t tiles, const int64_t* __restrict__ tb,
     int64_t* __restrict__ pk,
     int64_t* __restrict__ pv, uint32_t* __restrict__ err) {
-  constexpr int NW = GH_BLOCK / WAVE;
+  constexpr int NW = GH_SCAT_BLOCK / WAVE;
+  constexpr int IPT = GH_SCAT_IPT;
   __shared__ uint32_t s_cnt[NW][GH_PART_MAX];
+  __shared__ uint32_t s_off[GH_PART_MAX];  // first staged position of each partition
+  __shared__ int64_t s_base[GH_PART_MAX];  // output row of each partition's run
+  __shared__ uint32_t s_dst[GH_TILE];      // output row of each staged position
+  __shared__ int64_t s_stage[GH_TILE];
+  __shared__ uint32_t s_tmp[NW];
   const int lane = lane_id(), wave = wave_id();
-  for (int i = threadIdx.x; i < NW * GH_PART_MAX; i += GH_BLOCK) (&s_cnt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < NW * GH_PART_MAX; i += GH_SCAT_BLOCK) (&s_cnt[0][0])[i] = 0;
   __syncthreads();
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   uint32_t c;
   uint64_t r0, r1;
-  g.locate(blockIdx.x, &c, &r0, &r1);
-  const uint64_t wb = r0 + uint64_t(wave) * (WAVE * GH_IPT) + uint64_t(lane);
-  uint32_t d[GH_IPT], rk[GH_IPT];
+  g.locate(tile, &c, &r0, &r1);
+  const uint64_t wb = r0 + uint64_t(wave) * (WAVE * IPT) + uint64_t(lane);
+  uint32_t d[IPT], rk[IPT];
   uint32_t live = 0;
 #pragma unroll
-  for (int j = 0; j < GH_IPT; ++j) {
+  for (int j = 0; j < IPT; ++j) {
     const uint64_t i = wb + uint64_t(j * WAVE);
     const int64_t f = i < r1 ? fp[i] : -1;
     const bool ok = f >= 0 && uint64_t(f) < p;  // out-of-range ids: flagged by the histogram
@@ -207,7 +220,7 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_scatter(
     live |= uint32_t(ok) << j;
   }
 #pragma unroll
-  for (int j = 0; j < GH_IPT; ++j) {
+  for (int j = 0; j < IPT; ++j) {
     const bool valid = (live >> j) & 1u;
     if constexpr (LR) {
       rk[j] = valid ? atomicAdd(&s_cnt[wave][d[j]], 1u) : 0u;
@@ -225,31 +238,52 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_scatter(
     }
   }
   __syncthreads();
-  // per partition: exclusive offsets of the waves' runs
-  for (uint32_t q = threadIdx.x; q < p; q += GH_BLOCK) {
-    uint32_t run = 0;
+  // per partition: exclusive offsets of the waves' runs, the tile's count,
+  // its first staged position and its output row
+  uint32_t tot = 0;
+  if (threadIdx.x < p) {
+    const uint32_t q = threadIdx.x;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
       const uint32_t x = s_cnt[w][q];
-      s_cnt[w][q] = run;
-      run += x;
+      s_cnt[w][q] = tot;
+      tot += x;
     }
+    s_base[q] = tb[size_t(q) * tiles + tile];
   }
+  uint32_t all;
+  const uint32_t off = block_excl_scan<NW>(tot, s_tmp, &all);
+  if (threadIdx.x < p) s_off[threadIdx.x] = off;
   __syncthreads();
   bool bad = false;
+  uint32_t pos[IPT];
 #pragma unroll
-  for (int j = 0; j < GH_IPT; ++j) {
+  for (int j = 0; j < IPT; ++j) {
     if (!((live >> j) & 1u)) continue;
     const uint64_t i = wb + uint64_t(j * WAVE);
-    const int64_t lanepos = tb[size_t(d[j]) * tiles + blockIdx.x] + int64_t(s_cnt[wave][d[j]] + rk[j]);
-    if (lanepos < 0 || uint64_t(lanepos) >= g.n) {
-      bad = true;
-      continue;
-    }
-    pk[lanepos] = k[i];
-    pv[lanepos] = v[i];
+    const uint32_t r = s_cnt[wave][d[j]] + rk[j];
+    pos[j] = s_off[d[j]] + r;
+    const int64_t lanepos = s_base[d[j]] + int64_t(r);
+    const bool in = lanepos >= 0 && uint64_t(lanepos) < g.n;
+    bad |= !in;
+    s_dst[pos[j]] = in ? uint32_t(lanepos) : GH_EMPTY;
+    s_stage[pos[j]] = k[i];
   }
   if (bad) atomicOr(err, 2u);
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < all; j += GH_SCAT_BLOCK) {
+    const uint32_t dst = s_dst[j];
+    if (dst != GH_EMPTY) pk[dst] = s_stage[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if ((live >> j) & 1u) s_stage[pos[j]] = v[wb + uint64_t(j * WAVE)];
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < all; j += GH_SCAT_BLOCK) {
+    const uint32_t dst = s_dst[j];
+    if (dst != GH_EMPTY) pv[dst] = s_stage[j];
+  }
 }
 
 // ------------------------------------------------------------ insert
@@ -532,7 +566,7 @@ __global__ __launch_bounds__(GH_WTHREADS) void k_gh_wscatter(
   __shared__ uint32_t s_dst[GH_WTILE];   // record of each staged position
   __shared__ unsigned long long s_stage[GH_WTILE];
   __shared__ uint32_t s_tmp[GH_WTHREADS / WAVE];
-  const WTile t = tiles[blockIdx.x];
+  const WTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const uint64_t mask = uint64_t(tm[t.q]);
   for (uint32_t b = threadIdx.x; b < t.nwq; b += GH_WTHREADS) s_cnt[b] = 0;
   __syncthreads();
@@ -1261,10 +1295,12 @@ NF_EXPORT int32_t nf_ghash_scatter_i64(const int64_t* k, const int64_t* v, const
   {
     ProfScope prof("ghash_scatter", st.s, double(n) * 40.0);
     if (lds_rank_supported())
-      hipLaunchKernelGGL(k_gh_scatter<true>, dim3(tiles), dim3(GH_BLOCK), 0, st.s, dk, dv, dfp, g,
+      hipLaunchKernelGGL(k_gh_scatter<true>, dim3(tiles), dim3(GH_SCAT_BLOCK), 0, st.s, dk, dv,
+                         dfp, g,
                          uint32_t(p), tiles, tb, dpk, dpv, err);
     else
-      hipLaunchKernelGGL(k_gh_scatter<false>, dim3(tiles), dim3(GH_BLOCK), 0, st.s, dk, dv, dfp,
+      hipLaunchKernelGGL(k_gh_scatter<false>, dim3(tiles), dim3(GH_SCAT_BLOCK), 0, st.s, dk, dv,
+                         dfp,
                          g, uint32_t(p), tiles, tb, dpk, dpv, err);
     NFH_LAUNCH_CHECK();
   }
