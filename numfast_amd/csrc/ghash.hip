@@ -641,7 +641,8 @@ struct WinGeom {
   const int64_t* toff;       // p + 1
   const int64_t* tm;         // p
   const uint32_t* winbase;   // p + 1: first window of each partition
-  const uint4* desc;         // nwin: (first record, end record, partition, window in it)
+  const uint4* desc;         // 2 per window: (first record, end record, partition, window
+                             // in it), (table mask, table base) as 32-bit halves
   uint32_t p;
   uint32_t nwin;
 };
@@ -661,13 +662,17 @@ __device__ __forceinline__ uint32_t gh_win_part(const uint32_t* s_wb, uint32_t p
 // desc[w] of every window (records from the scanned entries).
 __global__ void k_gh_wdesc(const uint32_t* __restrict__ winbase, uint32_t p, uint32_t nwin,
                            const WPart* __restrict__ parts, const uint32_t* __restrict__ off,
+                           const int64_t* __restrict__ tm, const int64_t* __restrict__ toff,
                            uint4* __restrict__ desc) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwin) return;
   const uint32_t q = gh_win_part(winbase, p, w);  // every partition has a window
   const uint32_t wi = w - winbase[q];
   const uint32_t e0 = parts[q].e0, nt = parts[q].nt;
-  desc[w] = make_uint4(off[e0 + wi * nt], off[e0 + (wi + 1) * nt], q, wi);
+  const uint64_t mask = uint64_t(tm[q]), base = uint64_t(toff[q]);
+  desc[2 * w] = make_uint4(off[e0 + wi * nt], off[e0 + (wi + 1) * nt], q, wi);
+  desc[2 * w + 1] = make_uint4(uint32_t(mask), uint32_t(mask >> 32), uint32_t(base),
+                               uint32_t(base >> 32));
 }
 
 // LDS address of window slot i.  A partition's reference homes are
@@ -702,7 +707,7 @@ struct WinLds {
 // extension's ids (slot order) to spill[w]; L.changed = whether that list
 // differs from before.  False (uniform) when the window cannot be solved
 // in LDS (bit 32 of err set).
-__device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restrict__ rk,
+__device__ __forceinline__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restrict__ rk,
                           const int64_t* __restrict__ rv, const uint32_t* __restrict__ rr,
                           const unsigned long long* __restrict__ spill_in, uint32_t nin,
                           int64_t* __restrict__ tk, int64_t* __restrict__ tv,
@@ -710,9 +715,10 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
                           uint32_t* __restrict__ nspill, WinLds& L,
                           uint32_t* __restrict__ err) {
   constexpr int PER = GH_SD / GH_SBLOCK;
-  const uint4 dsc = g.desc[w];
+  const uint4 dsc = g.desc[2 * w], dsg = g.desc[2 * w + 1];
   const uint32_t r0 = dsc.x, r1 = dsc.y, q = dsc.z, wi = dsc.w;
-  const uint64_t mask = uint64_t(g.tm[q]);
+  const uint64_t mask = uint64_t(dsg.x) | (uint64_t(dsg.y) << 32);
+  const uint64_t base = uint64_t(dsg.z) | (uint64_t(dsg.w) << 32);
   const uint32_t wsz = mask + 1 < uint64_t(GH_WIN) ? uint32_t(mask + 1) : uint32_t(GH_WIN);
   const bool whole = uint64_t(wsz) == mask + 1;
   const uint32_t ext = whole ? 0u : uint32_t(GH_EXT);
@@ -908,7 +914,6 @@ __device__ bool gh_wsolve(uint32_t w, const WinGeom& g, const int64_t* __restric
     return false;
   }
   // 4. the window's slots
-  const uint64_t base = uint64_t(g.toff[q]);
   constexpr int OPT = GH_WIN / GH_SBLOCK;
 #pragma unroll
   for (int j = 0; j < OPT; ++j) {
@@ -1091,27 +1096,44 @@ __device__ __forceinline__ uint32_t gh_chunk_part(const uint32_t* s_cb, uint32_t
   return lo;
 }
 
-// Used slots per chunk.
+// Used slots per chunk, GH_CPW chunks per workgroup (slot s0 + j*GH_BLOCK
+// + t of a chunk is thread t's j-th, so every load is coalesced).
+constexpr int GH_CPW = 4;
 __global__ __launch_bounds__(GH_BLOCK) void k_gh_chunk_count(const uint8_t* __restrict__ used,
                                                              const int64_t* __restrict__ toff,
                                                              const uint32_t* __restrict__ cb,
-                                                             uint32_t p, uint32_t* __restrict__ cc) {
+                                                             uint32_t p, uint32_t nch,
+                                                             uint32_t* __restrict__ cc) {
   __shared__ uint32_t s_cb[GH_PART_MAX + 1];
-  __shared__ uint32_t s_w[GH_BLOCK / WAVE];
+  __shared__ uint32_t s_w[GH_CPW][GH_BLOCK / WAVE];
   for (uint32_t q = threadIdx.x; q <= p; q += GH_BLOCK) s_cb[q] = cb[q];
   __syncthreads();
-  const uint32_t c = blockIdx.x, q = gh_chunk_part(s_cb, p, c);
-  const int64_t s0 = toff[q] + int64_t(c - s_cb[q]) * GH_CH;
-  const int64_t s1 = s0 + GH_CH < toff[q + 1] ? s0 + GH_CH : toff[q + 1];
-  uint32_t x = 0;
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += GH_BLOCK) x += used[s] != 0;
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  if (lane_id() == 0) s_w[wave_id()] = x;
+  uint32_t x[GH_CPW];
+#pragma unroll
+  for (int i = 0; i < GH_CPW; ++i) {
+    const uint32_t c = blockIdx.x * GH_CPW + i;
+    x[i] = 0;
+    if (c >= nch) continue;  // uniform
+    const uint32_t q = gh_chunk_part(s_cb, p, c);
+    const int64_t s0 = toff[q] + int64_t(c - s_cb[q]) * GH_CH;
+    const int64_t s1 = s0 + GH_CH < toff[q + 1] ? s0 + GH_CH : toff[q + 1];
+#pragma unroll
+    for (int j = 0; j < GH_CH_IPT; ++j) {
+      const int64_t s = s0 + j * GH_BLOCK + threadIdx.x;
+      x[i] += (s < s1 && used[s] != 0) ? 1u : 0u;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < GH_CPW; ++i) {
+    for (int off = 32; off > 0; off >>= 1) x[i] += __shfl_xor(x[i], off);
+    if (lane_id() == 0) s_w[i][wave_id()] = x[i];
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < GH_CPW) {
+    const uint32_t c = blockIdx.x * GH_CPW + threadIdx.x;
     uint32_t t = 0;
-    for (int w = 0; w < GH_BLOCK / WAVE; ++w) t += s_w[w];
-    cc[c] = t;
+    for (int w = 0; w < GH_BLOCK / WAVE; ++w) t += s_w[threadIdx.x][w];
+    if (c < nch) cc[c] = t;
   }
 }
 
@@ -1141,8 +1163,10 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_chunk_scan(const uint32_t* __re
 }
 
 // One workgroup per chunk: its occupied keys in slot order from
-// ok[cpos[chunk]] (thread t owns 16 consecutive slots), bounded by the
-// shard piece's end lim[q] (the reference panics past it).
+// ok[cpos[chunk]], bounded by the shard piece's end lim[q] (the reference
+// panics past it).  Slot s0 + j*GH_BLOCK + w*64 + l is lane l of wave w's
+// j-th, so slot order is (j, w, l): per (j, w) ballot counts in LDS give
+// each group's first position, the ballot its lanes' ranks.
 __global__ __launch_bounds__(GH_BLOCK) void k_gh_occ_fill(const int64_t* __restrict__ tk,
                                                           const uint8_t* __restrict__ used,
                                                           const int64_t* __restrict__ toff,
@@ -1152,33 +1176,48 @@ __global__ __launch_bounds__(GH_BLOCK) void k_gh_occ_fill(const int64_t* __restr
                                                           const int64_t* __restrict__ lim,
                                                           int64_t* __restrict__ ok,
                                                           uint32_t* __restrict__ err) {
+  constexpr int NW = GH_BLOCK / WAVE;
   __shared__ uint32_t s_cb[GH_PART_MAX + 1];
-  __shared__ uint32_t s_tmp[GH_BLOCK / WAVE];
+  __shared__ uint32_t s_c[GH_CH_IPT * NW];
   for (uint32_t q = threadIdx.x; q <= p; q += GH_BLOCK) s_cb[q] = cb[q];
   __syncthreads();
   const uint32_t c = blockIdx.x, q = gh_chunk_part(s_cb, p, c);
   const int64_t s0 = toff[q] + int64_t(c - s_cb[q]) * GH_CH;
   const int64_t s1 = s0 + GH_CH < toff[q + 1] ? s0 + GH_CH : toff[q + 1];
-  const int64_t my = s0 + int64_t(threadIdx.x) * GH_CH_IPT;
-  uint32_t mine = 0;
+  const int w = wave_id();
+  uint8_t u[GH_CH_IPT];
+  int64_t key[GH_CH_IPT];
 #pragma unroll
   for (int j = 0; j < GH_CH_IPT; ++j) {
-    const int64_t s = my + j;
-    mine |= uint32_t(s < s1 && used[s] != 0) << j;
+    const int64_t s = s0 + j * GH_BLOCK + threadIdx.x;
+    u[j] = s < s1 ? used[s] : 0;
+    key[j] = s < s1 ? tk[s] : 0;
   }
-  uint32_t tot;
-  const uint32_t ex = block_excl_scan<GH_BLOCK / WAVE>(uint32_t(__popc(mine)), s_tmp, &tot);
-  int64_t pos = cpos[c] + int64_t(ex);
+  uint64_t bal[GH_CH_IPT];
+#pragma unroll
+  for (int j = 0; j < GH_CH_IPT; ++j) {
+    bal[j] = __ballot(u[j] != 0);
+    if (lane_id() == 0) s_c[j * NW + w] = uint32_t(__popcll(bal[j]));
+  }
+  __syncthreads();
   const int64_t end = lim[q];
+  int64_t run = cpos[c];  // first position of group (j, w), walked in slot order
   bool bad = false;
 #pragma unroll
   for (int j = 0; j < GH_CH_IPT; ++j) {
-    if (!((mine >> j) & 1u)) continue;
-    if (pos < 0 || pos >= end)
-      bad = true;
-    else
-      ok[pos] = tk[my + j];
-    ++pos;
+#pragma unroll
+    for (int v = 0; v < NW; ++v)
+      if (v < w) run += s_c[j * NW + v];
+    if (u[j] != 0) {
+      const int64_t pos = run + int64_t(__popcll(bal[j] & lanemask_lt()));
+      if (pos < 0 || pos >= end)
+        bad = true;
+      else
+        ok[pos] = key[j];
+    }
+#pragma unroll
+    for (int v = 0; v < NW; ++v)
+      if (v >= w) run += s_c[j * NW + v];
   }
   if (bad) atomicOr(err, 16u);
 }
@@ -1389,10 +1428,10 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
       uint32_t* eoff = scratch_t<uint32_t>(S_WORK1, size_t(np) + 1);
       unsigned long long* spill = scratch_t<unsigned long long>(S_MISC, size_t(nwin) * GH_EXT);
       unsigned char* aux = static_cast<unsigned char*>(
-          scratch(S_WORK2, size_t(ntl) * sizeof(WTile) + size_t(nwin) * 20 +
+          scratch(S_WORK2, size_t(ntl) * sizeof(WTile) + size_t(nwin) * 36 +
                                size_t(nc + 1) * 4 + 256));
       uint4* ddesc = reinterpret_cast<uint4*>(aux);
-      WTile* dtl = reinterpret_cast<WTile*>(ddesc + nwin);
+      WTile* dtl = reinterpret_cast<WTile*>(ddesc + 2 * size_t(nwin));
       uint32_t* nspill = reinterpret_cast<uint32_t*>(dtl + ntl);
       uint32_t* csum = nspill + nwin;
       fill_u32(nspill, 0, nwin, st.s);
@@ -1412,7 +1451,7 @@ NF_EXPORT int32_t nf_ghash_pins_i64(const int64_t* pk, const int64_t* pv, const 
                          g.tm, eoff, rk, rv, rr);
       NFH_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_gh_wdesc, dim3((nwin + 255) / 256), dim3(256), 0, st.s, dwb,
-                         uint32_t(p), nwin, dparts, eoff, ddesc);
+                         uint32_t(p), nwin, dparts, eoff, g.tm, g.toff, ddesc);
       NFH_LAUNCH_CHECK();
       const WinGeom wg{g.toff, g.tm, dwb, ddesc, uint32_t(p), nwin};
       hipLaunchKernelGGL(k_gh_wsolve, dim3(nwin), dim3(GH_SBLOCK), 0, st.s, wg, rk, rv, rr, dtk,
@@ -1486,8 +1525,8 @@ NF_EXPORT int32_t nf_ghash_occ_count(const uint8_t* used, const int64_t* toff, s
   {
     ProfScope prof("ghash_occ_count", st.s, double(m));
     if (nch > 0) {
-      hipLaunchKernelGGL(k_gh_chunk_count, dim3(nch), dim3(GH_BLOCK), 0, st.s, du, dtoff, dcb,
-                         uint32_t(p), cc);
+      hipLaunchKernelGGL(k_gh_chunk_count, dim3((nch + GH_CPW - 1) / GH_CPW), dim3(GH_BLOCK), 0,
+                         st.s, du, dtoff, dcb, uint32_t(p), nch, cc);
       NFH_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_gh_chunk_scan, dim3(p), dim3(GH_BLOCK), 0, st.s, cc, dcb, nullptr,
@@ -1540,8 +1579,8 @@ NF_EXPORT int32_t nf_ghash_occ_fill(const int64_t* tk, const uint8_t* used, cons
   NFH_CHECK(hipMemcpyAsync(dcb, hcb.data(), (p + 1) * 4, hipMemcpyHostToDevice, st.s));
   // per-chunk and per-partition counts, then each shard's running bases
   if (nch > 0) {
-    hipLaunchKernelGGL(k_gh_chunk_count, dim3(nch), dim3(GH_BLOCK), 0, st.s, du, dtoff, dcb,
-                       uint32_t(p), cc);
+    hipLaunchKernelGGL(k_gh_chunk_count, dim3((nch + GH_CPW - 1) / GH_CPW), dim3(GH_BLOCK), 0,
+                       st.s, du, dtoff, dcb, uint32_t(p), nch, cc);
     NFH_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_gh_chunk_scan, dim3(p), dim3(GH_BLOCK), 0, st.s, cc, dcb, nullptr, nullptr,
