@@ -93,8 +93,7 @@ TRAFFIC_ROWS = {
                      ["k_onesweep<int, true, false, true, true, LearnedDigit>",
                       "k_onesweep<int, false, false, true, false, LearnedDigit>"], "mean"),
     "ghash_pins": ("ghash_count_distinct_2p26.txt",
-                   ["k_gh_dedupe", "k_gh_first", "k_gh_winkey", "k_gh_win",
-                    "k_gh_materialize"], "sum"),
+                   ["k_gh_whist", "k_gh_wscatter", "k_gh_wsolve", "k_gh_wfix"], "sum"),
 }
 
 

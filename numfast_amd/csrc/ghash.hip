@@ -12,17 +12,16 @@
 //   * occ_fill: shard tt of the partitions writes from ostart[first].
 // The insert (pins) is sequential linear probing per partition in row
 // order; its table layout is an output.  The GPU reproduces that layout
-// without a sequential loop:
-//   1. dedupe: every row inserts its pair into a scratch table of row ids
-//      (compare-and-swap to claim, full pair compare, atomic min of the row
-//      id on equal pairs), so each distinct pair ends with its first row;
-//   2. place: only those first rows insert into the real geometry, each
-//      claiming slots with an atomic min of its row id and carrying any
-//      larger id it evicts onward.  At rest every slot between a pair's
-//      home and its slot holds a smaller row id and its own slot does not,
-//      which is exactly where sequential insertion in row order puts it
-//      (by induction on the row id), so the layout is identical;
-//   3. materialise used / tk / tv from the row ids.
+// without a sequential loop, by a priority claim: each distinct pair's
+// first row claims slots from its home with an atomic minimum of its row
+// id, carrying any larger id it evicts onward.  At rest every slot between
+// a pair's home and its slot holds a smaller row id and its own slot does
+// not, which is exactly where sequential insertion in row order puts it
+// (by induction on the row id), so the layout is identical.  The claim
+// runs in LDS, one window of the table at a time, after the rows are
+// scattered by window (section 2 below); a global form over the table in
+// HBM (dedupe, first-row flags, place, materialise) covers what the
+// windows cannot.
 // Refusals where the reference would index out of bounds (a panic, which
 // aborts the process) or probe forever: -2 (partition id or reservation
 // outside its buffer), -3 (a table with no free slot left for a new pair).
@@ -313,8 +312,8 @@ __device__ __forceinline__ uint32_t ld_holder(const uint32_t* p) {
   return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// 1. dedupe: each distinct pair of a partition ends in one slot of `hold`
-// holding its smallest row id.  Any layout will do here, so the home is
+// The global form.  1. dedupe: each distinct pair of a partition ends in
+// one slot of `hold` holding its smallest row id.  Any layout will do here, so the home is
 // the fingerprint's top bits (well mixed), not the reference's low bits
 // (which within a partition repeat its id and make long probe runs).
 __global__ __launch_bounds__(GH_BLOCK) void k_gh_dedupe(const int64_t* __restrict__ pk,
@@ -376,26 +375,27 @@ __global__ void k_gh_first(const uint32_t* __restrict__ hold, uint64_t m,
 //   k_gh_wscatter writes each row there as a (key, value, row) record,
 //     staged in LDS in window order so that the stores are contiguous runs;
 //   k_gh_wsolve, one workgroup per window, inserts the window's records
-//     into an LDS hash table (64-bit compare-and-swap of (hash tag, record),
-//     the full pair compared through the records on equal tags, atomic
-//     minimum of the row id), so each distinct pair keeps its first row;
-//     runs the priority claim over the window's slots plus GH_EXT extension
-//     slots (atomic minimum of (row id, record), evicted ids carried on);
-//     and writes used / tk / tv of the window's occupied slots from the
+//     into an LDS hash table keyed by a 64-bit pair hash (compare-and-swap
+//     to claim, atomic minimum of the row id), then checks every record
+//     that joined a slot against the claimer's pair, so each distinct pair
+//     keeps its first row; ranks each pair among its home's pairs in row
+//     order; runs the priority claim over the window's slots plus GH_EXT
+//     extension slots (atomic minimum of (row id, record), evicted ids
+//     carried on), each pair starting `rank` slots past its home; and
+//     writes used / tk / tv of the window's occupied slots from the
 //     records.  Ids that end in the extension spill into the next window
 //     (cyclically), where they start at the first slot: every slot they
 //     passed holds a smaller id.  A window of the whole table wraps inside
 //     LDS and never spills;
-//   k_gh_wfix, one workgroup per partition of several windows, walks them in
-//     cyclic order from one whose predecessor spilled nothing (else from
-//     window 0, three laps) and re-solves each window whose predecessor
-//     spills, with those spills.
+//   k_gh_wfix, one workgroup per partition of several windows, re-solves
+//     in cyclic order each window whose predecessor's spill list changed.
 // Spills only add ids to a window, so its occupied slots only grow and a
 // re-solve overwrites every slot the first solve wrote.  A cluster longer
-// than the extension, or a window with more distinct pairs than its LDS
-// table holds, sets bit 32 of err and the caller runs the global form,
-// which rewrites every occupied slot.  The windowed form runs only when no
-// partition has more rows than slots, so it never meets a full table.
+// than the extension, a window with more distinct pairs than its LDS
+// table holds, or two pairs with one hash sets bit 32 of err and the
+// caller runs the global form, which rewrites every occupied slot.  The
+// windowed form runs only when no partition has more rows than slots, so
+// it never meets a full table.
 constexpr int GH_WIN = 2048;             // window slots
 constexpr int GH_EXT = 1024;             // extension slots
 constexpr int GH_WTHREADS = 512;         // threads of the tile kernels
