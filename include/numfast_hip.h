@@ -229,7 +229,9 @@ typedef struct nfh_cdf_model {
   float p[1153];
   uint32_t wmin;
   uint32_t span;
-  uint32_t pad;
+  uint32_t hints; /* bit 0: the training sample was nearly sorted (set by
+                     nfh_cdf_train_i32; changes the sort's speed, never its
+                     result) */
 } nfh_cdf_model;
 
 /* Train a model on a column (2^16-key sample sorted by the radix argsort,

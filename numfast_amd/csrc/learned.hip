@@ -36,6 +36,7 @@
 // LSD argsort's, whatever the model predicts.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "device.hpp"
@@ -87,12 +88,17 @@ __device__ __forceinline__ uint32_t lcode(uint32_t w, const uint32_t* kn, const 
 
 // Digit generator of the onesweep passes: the pass's digit is byte
 // shift / 8 of the key's bucket code (the knots live in the pass's LDS).
-struct LearnedDigit {
+// P: rank and count the run of lanes sharing the first lane's code in one
+// step (nearly sorted keys share one code over long runs of rows; on
+// unordered keys the ballots only cost time, so the model's hint picks).
+template <bool P>
+struct LearnedDigitT {
   LearnedMap m;
   int shift, next_shift;
   static constexpr size_t EXTRA_LDS = ((KNOTS + 1) * 4 + 15) / 16 * 16;
-  // nearly sorted keys share one bucket code over long runs of rows
-  static constexpr bool PEEL = true;
+  static constexpr bool PEEL = P;
+  // the first pass carries the code beside the row (codes < 2^16)
+  static constexpr bool PACK = true;
   __device__ void init(unsigned char* s) const {
     uint32_t* k = reinterpret_cast<uint32_t*>(s);
     for (int i = threadIdx.x; i <= KNOTS; i += blockDim.x) k[i] = m.knots[i];
@@ -720,6 +726,30 @@ static uint32_t code_bits_for(uint32_t n) {
   return uint32_t(std::min(16, std::max(1, lg - 12)));
 }
 
+// Code histogram and the stable code passes: the low byte (reads the raw
+// keys; counts the high byte per chain of the second pass), then the high
+// byte.  (word, row) pairs end in (w1, i1) with two passes, else (w0, i0).
+template <bool P>
+static void code_passes(const int32_t* keys, uint32_t n, const LearnedMap& map, bool two,
+                        uint32_t* hist, uint32_t* bins, uint32_t* ctr, uint64_t* lb, uint32_t* w0,
+                        uint32_t* i0, uint32_t* w1, uint32_t* i1, hipStream_t s) {
+  constexpr size_t CH = size_t(NCHAIN) * 256;
+  chain_hist_first<int32_t>(keys, n, LearnedDigitT<P>{map, 0, 0}, false, hist,
+                            static_cast<uint32_t*>(nullptr), s, "learned_code_hist");
+  hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, hist, bins);
+  NFH_LAUNCH_CHECK();
+  onesweep_pass<int32_t, true, false>(keys, nullptr, nullptr, w0, reinterpret_cast<int32_t*>(i0),
+                                      nullptr, bins, lb, ctr, n, LearnedDigitT<P>{map, 0, 8},
+                                      two ? hist + CH : nullptr, false, s, "learned_pass");
+  if (!two) return;
+  hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, hist + CH, bins + CH);
+  NFH_LAUNCH_CHECK();
+  onesweep_pass<int32_t, false, false>(nullptr, w0, reinterpret_cast<const int32_t*>(i0), w1,
+                                       reinterpret_cast<int32_t*>(i1), nullptr, bins + CH, lb,
+                                       ctr + NCHAIN, n, LearnedDigitT<P>{map, 8, 0}, nullptr,
+                                       false, s, "learned_pass");
+}
+
 void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model, int32_t* perm,
                          hipStream_t s, uint64_t* stats) {
   CdfModel hm;
@@ -746,7 +776,13 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   // model -> knots
   float* dP = scratch_t<float>(S_LMODEL, MLP_P + KNOTS + 8);
   uint32_t* knots = reinterpret_cast<uint32_t*>(dP + MLP_P + 3);
-  NFH_CHECK(hipMemcpyAsync(dP, hm.p, sizeof(float) * MLP_P, hipMemcpyHostToDevice, s));
+  {
+    // through the pinned buffer: a copy from pageable memory would wait
+    // for the stream's earlier work before returning
+    float* hp = static_cast<float*>(pinned(sizeof(float) * MLP_P));
+    std::memcpy(hp, hm.p, sizeof(float) * MLP_P);
+    NFH_CHECK(hipMemcpyAsync(dP, hp, sizeof(float) * MLP_P, hipMemcpyHostToDevice, s));
+  }
   {
     ProfScope prof("learned_knots", s);
     hipLaunchKernelGGL(k_cdf_knots, dim3(1), dim3(1024), 0, s, dP, float(B) * float(1 << KN_UNIT),
@@ -779,33 +815,18 @@ void learned_argsort_i32(const int32_t* keys, uint32_t n, const CdfModel* model,
   uint32_t* unequal = cst + (B + 1);
   fill_u32(meta, 0, head, s);
   const bool two = bits > 8;
-  chain_hist_first<int32_t>(keys, n, LearnedDigit{map, 0, 0}, false, hist,
-                            static_cast<uint32_t*>(nullptr), s, "learned_code_hist");
-  // stable code passes: low byte (reads the raw keys; counts the high byte
-  // per chain of the second pass), then high byte
   uint32_t* w0 = scratch_t<uint32_t>(S_L0, n);
   uint32_t* i0 = scratch_t<uint32_t>(S_L1, n);
   uint32_t* w1 = scratch_t<uint32_t>(S_L2, n);
   uint32_t* i1 = scratch_t<uint32_t>(S_L3, n);
   const size_t tiles = (n + SortGeom<int32_t>::TILE - 1) / SortGeom<int32_t>::TILE;
   uint64_t* lb = lookback_words(tiles * 256, s);
-  hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, hist, bins);
-  NFH_LAUNCH_CHECK();
-  onesweep_pass<int32_t, true, false>(keys, nullptr, nullptr, w0, reinterpret_cast<int32_t*>(i0),
-                                      nullptr, bins, lb, ctr, n, LearnedDigit{map, 0, 8},
-                                      two ? hist + CH : nullptr, false, s, "learned_pass");
-  const uint32_t* W = w0;
-  const uint32_t* I = i0;
-  if (two) {
-    hipLaunchKernelGGL(k_chain_bins<256>, dim3(1), dim3(256), 0, s, hist + CH, bins + CH);
-    NFH_LAUNCH_CHECK();
-    onesweep_pass<int32_t, false, false>(nullptr, w0, reinterpret_cast<const int32_t*>(i0), w1,
-                                         reinterpret_cast<int32_t*>(i1), nullptr, bins + CH, lb,
-                                         ctr + NCHAIN, n, LearnedDigit{map, 8, 0}, nullptr,
-                                         false, s, "learned_pass");
-    W = w1;
-    I = i1;
-  }
+  if (hm.hints & NFH_CDF_HINT_SORTED)
+    code_passes<true>(keys, n, map, two, hist, bins, ctr, lb, w0, i0, w1, i1, s);
+  else
+    code_passes<false>(keys, n, map, two, hist, bins, ctr, lb, w0, i0, w1, i1, s);
+  const uint32_t* W = two ? w1 : w0;
+  const uint32_t* I = two ? i1 : i0;
   {
     ProfScope prof("learned_bounds", s);
     hipLaunchKernelGGL(k_bucket_bounds, dim3((B + 1 + 255) / 256), dim3(256), 0, s, W, n, map, B,
@@ -1025,16 +1046,20 @@ float cdf_train_i32(const int32_t* keys, uint32_t n, uint32_t steps, uint32_t fl
   // the plain kernel's digit histograms over the whole column
   uint32_t* hist = scratch_t<uint32_t>(S_LBUCKET, 4 * 256);
   radix_histograms_i32(keys, n, hist, s);
-  std::vector<int32_t> hs(S);
+  std::vector<int32_t> hs(S), hu(S);
   std::vector<uint32_t> hh(4 * 256);
   NFH_CHECK(hipMemcpyAsync(hs.data(), sorted, size_t(S) * 4, hipMemcpyDeviceToHost, s));
+  NFH_CHECK(hipMemcpyAsync(hu.data(), smp, size_t(S) * 4, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipMemcpyAsync(hh.data(), hist, 4 * 256 * 4, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipStreamSynchronize(s));
   const uint32_t wmin = uint32_t(hs.front()) ^ 0x80000000u;
   const uint32_t wmax = uint32_t(hs.back()) ^ 0x80000000u;
   model->wmin = wmin;
   model->span = wmax - wmin;
-  model->pad = 0;
+  // sortedness of the column in row order, from the strided sample
+  size_t asc = 0;
+  for (size_t i = 1; i < size_t(S); ++i) asc += hu[i - 1] <= hu[i];
+  model->hints = (S > 1 && 4 * asc >= 3 * size_t(S - 1)) ? NFH_CDF_HINT_SORTED : 0u;
   if (model->span == 0) {
     for (int q = 0; q < MLP_P; ++q) model->p[q] = 0.f;
     return 0.f;

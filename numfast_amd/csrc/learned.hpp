@@ -18,8 +18,12 @@ struct CdfModel {
   float p[MLP_P];
   uint32_t wmin;  // smallest sampled key word (order-preserving uint32)
   uint32_t span;  // largest sampled word - wmin (0: every sample equal)
-  uint32_t pad;
+  uint32_t hints;  // NFH_CDF_HINT_* bits: speed only, never the result
 };
+// The training sample was nearly in ascending order (at least 3/4 of its
+// neighbours ascending): the code passes rank runs of rows sharing a code
+// in one step.
+constexpr uint32_t NFH_CDF_HINT_SORTED = 1;
 static_assert(sizeof(CdfModel) == MLP_P * 4 + 12, "model layout");
 
 // Stable ascending argsort of device int32 keys with a trained model

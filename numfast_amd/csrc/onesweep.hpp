@@ -147,6 +147,10 @@ struct RadixDigit {
   int shift, next_shift;
   static constexpr size_t EXTRA_LDS = 0;
   static constexpr bool PEEL = false;
+  // PACK: the first pass keeps each item's code beside its tile-local row
+  // in the LDS row slot, so the write-out does not evaluate it again (for
+  // codes that cost more than a shift; RadixDigit's is the word itself)
+  static constexpr bool PACK = false;
   __device__ void init(unsigned char*) const {}
   template <class W>
   __device__ W code(W w, const unsigned char*) const {
@@ -302,6 +306,11 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
   using W = typename Tk::W;
   using G = SortGeom<K>;
   constexpr int SORT_BLOCK = G::BLOCK, SORT_WAVES = G::WAVES, SORT_TILE = G::TILE;
+  // PACK (first pass only): the LDS row slot holds (code << TB) | local row
+  constexpr bool PK = FIRST && DG::PACK;
+  constexpr int TB = SORT_TILE == 16384 ? 14 : 13;
+  static_assert((1 << TB) == SORT_TILE, "tile-local row bits");
+
   extern __shared__ __align__(16) unsigned char s_raw[];
   W* s_w = reinterpret_cast<W*>(s_raw);
   uint32_t* s_i = reinterpret_cast<uint32_t*>(s_w + SORT_TILE);
@@ -377,7 +386,9 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     for (int j = 0; j < SORT_IPT; ++j) {
       const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
       const bool valid = pos < n;
-      const uint32_t d = uint32_t(dg.code(kw[j], s_dg) >> dg.shift) & 255u;
+      const auto cd = dg.code(kw[j], s_dg);
+      const uint32_t d = uint32_t(cd >> dg.shift) & 255u;
+      if constexpr (PK) ix[j] = uint32_t(cd);  // the row is pos, kept in the tile order
       if constexpr (LR) {
         rk[j] = (rank_lds<DG::PEEL>(s_wh[wave], d, valid) << 8) | d;
       } else {
@@ -422,7 +433,10 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
         const uint32_t d = rk[j] & 255u;
         const uint32_t p = s_tstart[d] + s_wh[wave][d] + (rk[j] >> 8);
         s_w[p] = kw[j];
-        s_i[p] = ix[j];
+        if constexpr (PK)
+          s_i[p] = (ix[j] << TB) | (pos - base);
+        else
+          s_i[p] = ix[j];
       }
     }
 
@@ -458,7 +472,7 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     constexpr int WU = DG::PEEL ? 1 : 2;
     for (uint32_t i0 = 0; i0 < nvalid; i0 += WU * SORT_BLOCK) {  // uniform trip count
       W w[WU];
-      uint32_t xi[WU], gb[WU], cc[WU];
+      uint32_t xi[WU], gb[WU], cc[WU], pc[WU];
       bool ok[WU];
 #pragma unroll
       for (int u = 0; u < WU; ++u) {
@@ -471,7 +485,14 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
 #pragma unroll
       for (int u = 0; u < WU; ++u) {
         if (!ok[u]) w[u] = W(0);
-        const uint32_t d = uint32_t(dg.code(w[u], s_dg) >> dg.shift) & 255u;
+        uint32_t d;
+        if constexpr (PK) {
+          pc[u] = xi[u] >> TB;
+          xi[u] = base + (xi[u] & uint32_t(SORT_TILE - 1));
+          d = (pc[u] >> dg.shift) & 255u;
+        } else {
+          d = uint32_t(dg.code(w[u], s_dg) >> dg.shift) & 255u;
+        }
         gb[u] = s_gbase[d];
         if constexpr (NEXT) cc[u] = s_cc[d];
       }
@@ -497,9 +518,12 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
 #pragma unroll
             for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
           }
-          const auto code = dg.code(w[u], s_dg);
-          hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + (uint32_t(code >> dg.next_shift) & 255u),
-                             ok[u]);
+          uint32_t nd;
+          if constexpr (PK)
+            nd = (pc[u] >> dg.next_shift) & 255u;
+          else
+            nd = uint32_t(dg.code(w[u], s_dg) >> dg.next_shift) & 255u;
+          hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + nd, ok[u]);
         }
       }
     }

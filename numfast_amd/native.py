@@ -873,7 +873,7 @@ MLP_P = 1153
 class CdfModel(ctypes.Structure):
     """struct nfh_cdf_model (include/numfast_hip.h)."""
     _fields_ = [("p", ctypes.c_float * MLP_P), ("wmin", ctypes.c_uint32),
-                ("span", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                ("span", ctypes.c_uint32), ("hints", ctypes.c_uint32)]
 
     def params(self):
         return np.ctypeslib.as_array(self.p).copy()

@@ -122,6 +122,24 @@ def test_learned_sort_equals_stable_argsort(hip, kind, n):
     assert np.array_equal(perm, np.argsort(k, kind="stable"))
 
 
+@pytest.mark.parametrize("kind", ["uniform", "swaps", "distinct1024", "equal"])
+def test_learned_sort_hint_changes_speed_only(hip, kind):
+    """The model's sortedness hint (set by training from its row-order
+    sample) picks the code passes' run ranking; both settings give the
+    stable argsort on the same keys, and training sets it only for nearly
+    sorted columns."""
+    rng = np.random.default_rng(zlib.crc32(kind.encode()))
+    n = (1 << 20) + 77
+    k = _dist(kind, n, rng)
+    model, _ = native.cdf_train(k, steps=10)
+    assert model.hints == (1 if kind in ("swaps", "equal") else 0)
+    ref = np.argsort(k, kind="stable")
+    for h in (0, 1):
+        model.hints = h
+        perm, _ = native.sort_perm_learned(k, model)
+        assert np.array_equal(perm, ref), h
+
+
 @pytest.mark.parametrize("kind", ["narrow", "lognormal", "exponential"])
 def test_learned_sort_mismatched_model_uses_fixup(hip, kind):
     """A model trained on other data puts many keys in few buckets: the
