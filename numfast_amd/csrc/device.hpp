@@ -102,7 +102,9 @@ __device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t x) {
 
 // Exclusive scan across a workgroup of NW waves; `s_tmp` holds NW words.
 // Every thread must call it.  Returns the exclusive prefix, sets *total.
-template <int NW>
+// TRAIL = false leaves out the closing barrier, for callers that pass
+// another barrier before s_tmp is written again.
+template <int NW, bool TRAIL = true>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
                                                     uint32_t* total) {
   const uint32_t inc = wave_incl_scan(v);
@@ -115,7 +117,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
     if (w < wave_id()) off += c;
     tot += c;
   }
-  __syncthreads();
+  if constexpr (TRAIL) __syncthreads();
   *total = tot;
   return off + inc - v;
 }

@@ -24,6 +24,7 @@
 #pragma once
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "device.hpp"
 #include "runtime.hpp"
@@ -308,6 +309,7 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
   constexpr int SORT_BLOCK = G::BLOCK, SORT_WAVES = G::WAVES, SORT_TILE = G::TILE;
   // PACK (first pass only): the LDS row slot holds (code << TB) | local row
   constexpr bool PK = FIRST && DG::PACK;
+  constexpr int RS = PK ? 16 : 8;  // rank shift in the rank word
   constexpr int TB = SORT_TILE == 16384 ? 14 : 13;
   static_assert((1 << TB) == SORT_TILE, "tile-local row bits");
 
@@ -340,10 +342,13 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     for (int i = tid; i < NCHAIN * 256; i += SORT_BLOCK) s_nh[i >> 8][i & 255] = 0;
   dg.init(s_dg);
 
+  // The next tile's index and cleared counters are prepared during the
+  // write-out of the current one (nothing reads s_wh or s_tile there), so
+  // the barrier that ends a tile also publishes them.
+  if (tid == 0) *s_tile = atomicAdd(&tile_ctr[chain], 1u);
+  for (int i = tid; i < SORT_WAVES * 256; i += SORT_BLOCK) s_wh[i >> 8][i & 255] = 0;
+  __syncthreads();
   for (;;) {
-    if (tid == 0) *s_tile = atomicAdd(&tile_ctr[chain], 1u);
-    for (int i = tid; i < SORT_WAVES * 256; i += SORT_BLOCK) s_wh[i >> 8][i & 255] = 0;
-    __syncthreads();
     const uint32_t local = *s_tile;
     if (local >= ctiles) break;  // uniform over the workgroup
     const uint32_t tile = cfirst + local;
@@ -351,182 +356,202 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
     const uint32_t base = tile * uint32_t(SORT_TILE);
     const uint32_t wbase = base + uint32_t(wave) * (WAVE * SORT_IPT);
 
-    W kw[SORT_IPT];
-    uint32_t ix[SORT_IPT];
-#pragma unroll
-    for (int j = 0; j < SORT_IPT; ++j) {
-      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      if (pos < n) {
-        if constexpr (FIRST) {
-          kw[j] = W(keys[pos]);  // converted below, outside the branch
-          ix[j] = pos;
-        } else {
-          kw[j] = win[pos];
-          ix[j] = uint32_t(iin[pos]);
-        }
-      } else {
-        kw[j] = 0;
-        ix[j] = 0;
-      }
-    }
+    // The tile body, compiled twice: FULL tiles (all but a pass's last)
+    // drop every bounds test, so their loads take one address and
+    // immediate offsets and their ranks skip the validity masks.
+    auto body = [&](auto full) {
+      constexpr bool FULL = decltype(full)::value;
 
-    // (A conversion inside the bounds branch above made the compiler wait
-    // for each key load before issuing the next.)
-    if constexpr (FIRST) {
+      W kw[SORT_IPT];
+      uint32_t ix[SORT_IPT];
 #pragma unroll
-      for (int j = 0; j < SORT_IPT; ++j) kw[j] = Tk::word(K(kw[j]), desc);
-    }
-
-    // Stable in-wave ranks: item j of lane l precedes item j of lane l+1,
-    // which precedes item j+1 of lane 0 (that is the position order).  LR:
-    // one lane-ordered LDS atomic per item; else the 8-ballot digit match.
-    // rk[j] keeps (rank << 8) | digit (a wave's rank is below 64 * SORT_IPT).
-    uint32_t rk[SORT_IPT];
-#pragma unroll
-    for (int j = 0; j < SORT_IPT; ++j) {
-      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      const bool valid = pos < n;
-      const auto cd = dg.code(kw[j], s_dg);
-      const uint32_t d = uint32_t(cd >> dg.shift) & 255u;
-      if constexpr (PK) ix[j] = uint32_t(cd);  // the row is pos, kept in the tile order
-      if constexpr (LR) {
-        rk[j] = (rank_lds<DG::PEEL>(s_wh[wave], d, valid) << 8) | d;
-      } else {
-        const uint64_t peers = match_digit8(d, valid);
-        const uint32_t r = __popcll(peers & lanemask_lt());
-        const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
-        uint32_t old = 0;
-        if (valid && r == 0) {
-          old = s_wh[wave][d];
-          s_wh[wave][d] = old + uint32_t(__popcll(peers));
-        }
-        old = __shfl(old, leader);
-        rk[j] = ((old + r) << 8) | d;
-      }
-    }
-    __syncthreads();
-
-    // Per digit (threads 0..255, thread == digit): wave-exclusive offsets
-    // and the tile's count, published at once for later tiles' look-back.
-    const bool digit_thread = tid < 256;
-    uint32_t cnt = 0;
-    uint64_t* my_lb = lookback + size_t(tile) * 256 + (digit_thread ? tid : 0);
-    if (digit_thread) {
-#pragma unroll
-      for (int w = 0; w < SORT_WAVES; ++w) {
-        const uint32_t c = s_wh[w][tid];
-        s_wh[w][tid] = cnt;
-        cnt += c;
-      }
-      lb_store(my_lb, lb_pack(epoch, head ? LB_PREFIX : LB_AGG, cnt));
-    }
-    uint32_t tile_total;
-    const uint32_t tstart = block_excl_scan<SORT_WAVES>(cnt, s_tmp, &tile_total);
-    if (digit_thread) s_tstart[tid] = tstart;
-    __syncthreads();
-
-    // Reorder the tile in LDS by digit (stable).
-#pragma unroll
-    for (int j = 0; j < SORT_IPT; ++j) {
-      const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-      if (pos < n) {
-        const uint32_t d = rk[j] & 255u;
-        const uint32_t p = s_tstart[d] + s_wh[wave][d] + (rk[j] >> 8);
-        s_w[p] = kw[j];
-        if constexpr (PK)
-          s_i[p] = (ix[j] << TB) | (pos - base);
-        else
-          s_i[p] = ix[j];
-      }
-    }
-
-    // Decoupled look-back for this digit's offset among the chain's
-    // earlier tiles (the chain's first tile publishes a prefix at once).
-    if (digit_thread) {
-      uint32_t excl = 0;
-      if (!head) {
-        excl = lb_lookback(lookback + tid, tile, 256, epoch);
-        lb_store(my_lb, lb_pack(epoch, LB_PREFIX, excl + cnt));
-      }
-      s_gbase[tid] = bins[tid] + excl - tstart;
-      if constexpr (NEXT) {
-        // chain of the run's first and last output position (monotone in
-        // the position, so equal ends mean one chain for the whole run)
-        const uint32_t g0 = bins[tid] + excl;
-        const uint32_t g1 = cnt ? g0 + cnt - 1u : g0;
-        uint32_t c0 = 0, c1 = 0;
-#pragma unroll
-        for (int k = 1; k < NCHAIN; ++k) {
-          c0 += g0 >= nb[k] ? 1u : 0u;
-          c1 += g1 >= nb[k] ? 1u : 0u;
-        }
-        s_cc[tid] = (unsigned char)(c0 == c1 ? c0 : 0xFFu);
-      }
-    }
-    __syncthreads();
-
-    const uint32_t nvalid = (n - base) < uint32_t(SORT_TILE) ? (n - base) : uint32_t(SORT_TILE);
-    // WU items per thread per step, their LDS reads issued together so one
-    // wait covers them (radix passes 1.05 -> 1.02 ms at 2^28 with two; the
-    // learned passes, whose code map reads LDS knots, measured slower)
-    constexpr int WU = DG::PEEL ? 1 : 2;
-    for (uint32_t i0 = 0; i0 < nvalid; i0 += WU * SORT_BLOCK) {  // uniform trip count
-      W w[WU];
-      uint32_t xi[WU], gb[WU], cc[WU], pc[WU];
-      bool ok[WU];
-#pragma unroll
-      for (int u = 0; u < WU; ++u) {
-        const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
-        ok[u] = i < nvalid;
-        const uint32_t ic = ok[u] ? i : 0u;
-        w[u] = s_w[ic];
-        xi[u] = s_i[ic];
-      }
-#pragma unroll
-      for (int u = 0; u < WU; ++u) {
-        if (!ok[u]) w[u] = W(0);
-        uint32_t d;
-        if constexpr (PK) {
-          pc[u] = xi[u] >> TB;
-          xi[u] = base + (xi[u] & uint32_t(SORT_TILE - 1));
-          d = (pc[u] >> dg.shift) & 255u;
-        } else {
-          d = uint32_t(dg.code(w[u], s_dg) >> dg.shift) & 255u;
-        }
-        gb[u] = s_gbase[d];
-        if constexpr (NEXT) cc[u] = s_cc[d];
-      }
-#pragma unroll
-      for (int u = 0; u < WU; ++u) {
-        const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
-        const uint32_t g = gb[u] + i;
-        if (ok[u]) {
-          if constexpr (LAST) {
-            iout[g] = int32_t(xi[u]);
-            if (kout != nullptr) kout[g] = Tk::key(w[u], desc);
+      for (int j = 0; j < SORT_IPT; ++j) {
+        const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+        if (FULL || pos < n) {
+          if constexpr (FIRST) {
+            kw[j] = W(keys[pos]);  // converted below, outside the branch
           } else {
-            wout[g] = w[u];
-            iout[g] = int32_t(xi[u]);
+            kw[j] = win[pos];
+            ix[j] = uint32_t(iin[pos]);
           }
-        }
-        if constexpr (NEXT) {
-          // the run's chain from LDS; item by item only in the few runs
-          // that cross a chain boundary
-          uint32_t c = cc[u];
-          if (c == 0xFFu) {
-            c = 0;
-#pragma unroll
-            for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
-          }
-          uint32_t nd;
-          if constexpr (PK)
-            nd = (pc[u] >> dg.next_shift) & 255u;
-          else
-            nd = uint32_t(dg.code(w[u], s_dg) >> dg.next_shift) & 255u;
-          hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + nd, ok[u]);
+        } else {
+          kw[j] = 0;
+          ix[j] = 0;
         }
       }
-    }
+
+      // (A conversion inside the bounds branch above made the compiler wait
+      // for each key load before issuing the next.)
+      if constexpr (FIRST) {
+#pragma unroll
+        for (int j = 0; j < SORT_IPT; ++j) kw[j] = Tk::word(K(kw[j]), desc);
+      }
+
+      // Stable in-wave ranks: item j of lane l precedes item j of lane l+1,
+      // which precedes item j+1 of lane 0 (that is the position order).  LR:
+      // one lane-ordered LDS atomic per item; else the 8-ballot digit match.
+      // rk[j] keeps (rank << RS) | digit (a wave's rank is below 64 * SORT_IPT);
+      // PK: (rank << 16) | code instead (codes below 2^16; the digit is a
+      // byte of it, and the code goes to the row slot in the reorder).
+      uint32_t rk[SORT_IPT];
+#pragma unroll
+      for (int j = 0; j < SORT_IPT; ++j) {
+        const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+        const bool valid = FULL || pos < n;
+        const auto cd = dg.code(kw[j], s_dg);
+        const uint32_t d = uint32_t(cd >> dg.shift) & 255u;
+        if constexpr (LR) {
+          rk[j] = (rank_lds<DG::PEEL>(s_wh[wave], d, valid) << RS) | (PK ? uint32_t(cd) : d);
+        } else {
+          const uint64_t peers = match_digit8(d, valid);
+          const uint32_t r = __popcll(peers & lanemask_lt());
+          const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+          uint32_t old = 0;
+          if (valid && r == 0) {
+            old = s_wh[wave][d];
+            s_wh[wave][d] = old + uint32_t(__popcll(peers));
+          }
+          old = __shfl(old, leader);
+          rk[j] = ((old + r) << RS) | (PK ? uint32_t(cd) : d);
+        }
+      }
+      __syncthreads();
+
+      // Per digit (threads 0..255, thread == digit): wave-exclusive offsets
+      // and the tile's count, published at once for later tiles' look-back.
+      const bool digit_thread = tid < 256;
+      uint32_t cnt = 0;
+      uint64_t* my_lb = lookback + size_t(tile) * 256 + (digit_thread ? tid : 0);
+      if (digit_thread) {
+#pragma unroll
+        for (int w = 0; w < SORT_WAVES; ++w) {
+          const uint32_t c = s_wh[w][tid];
+          s_wh[w][tid] = cnt;
+          cnt += c;
+        }
+        lb_store(my_lb, lb_pack(epoch, head ? LB_PREFIX : LB_AGG, cnt));
+      }
+      uint32_t tile_total;
+      // (no trailing barrier: s_tmp is next written in the next tile)
+      const uint32_t tstart = block_excl_scan<SORT_WAVES, false>(cnt, s_tmp, &tile_total);
+      if (digit_thread) s_tstart[tid] = tstart;
+      __syncthreads();
+
+      // Reorder the tile in LDS by digit (stable).
+#pragma unroll
+      for (int j = 0; j < SORT_IPT; ++j) {
+        const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
+        if (FULL || pos < n) {
+          const uint32_t d = PK ? ((rk[j] >> dg.shift) & 255u) : (rk[j] & 255u);
+          const uint32_t p = s_tstart[d] + s_wh[wave][d] + (rk[j] >> RS);
+          s_w[p] = kw[j];
+          if constexpr (PK)
+            s_i[p] = ((rk[j] & 0xFFFFu) << TB) | (pos - base);
+          else if constexpr (FIRST)
+            s_i[p] = pos;  // the first pass's rows are the positions
+          else
+            s_i[p] = ix[j];
+        }
+      }
+
+      // Decoupled look-back for this digit's offset among the chain's
+      // earlier tiles (the chain's first tile publishes a prefix at once).
+      if (digit_thread) {
+        uint32_t excl = 0;
+        if (!head) {
+          excl = lb_lookback(lookback + tid, tile, 256, epoch);
+          lb_store(my_lb, lb_pack(epoch, LB_PREFIX, excl + cnt));
+        }
+        s_gbase[tid] = bins[tid] + excl - tstart;
+        if constexpr (NEXT) {
+          // chain of the run's first and last output position (monotone in
+          // the position, so equal ends mean one chain for the whole run)
+          const uint32_t g0 = bins[tid] + excl;
+          const uint32_t g1 = cnt ? g0 + cnt - 1u : g0;
+          uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+          for (int k = 1; k < NCHAIN; ++k) {
+            c0 += g0 >= nb[k] ? 1u : 0u;
+            c1 += g1 >= nb[k] ? 1u : 0u;
+          }
+          s_cc[tid] = (unsigned char)(c0 == c1 ? c0 : 0xFFu);
+        }
+      }
+      __syncthreads();
+
+      const uint32_t nvalid =
+          FULL ? uint32_t(SORT_TILE)
+               : ((n - base) < uint32_t(SORT_TILE) ? (n - base) : uint32_t(SORT_TILE));
+      // WU items per thread per step, their LDS reads issued together so one
+      // wait covers them (radix passes 1.05 -> 1.02 ms at 2^28 with two; the
+      // learned passes, whose code map reads LDS knots, measured slower)
+      constexpr int WU = DG::PEEL ? 1 : 2;
+      if (tid == 0) *s_tile = atomicAdd(&tile_ctr[chain], 1u);
+      for (int i = tid; i < SORT_WAVES * 256; i += SORT_BLOCK) s_wh[i >> 8][i & 255] = 0;
+      for (uint32_t i0 = 0; i0 < nvalid; i0 += WU * SORT_BLOCK) {  // uniform trip count
+        W w[WU];
+        uint32_t xi[WU], gb[WU], cc[WU], pc[WU];
+        bool ok[WU];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+          const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
+          ok[u] = FULL || i < nvalid;
+          const uint32_t ic = ok[u] ? i : 0u;
+          w[u] = s_w[ic];
+          xi[u] = s_i[ic];
+        }
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+          if (!ok[u]) w[u] = W(0);
+          uint32_t d;
+          if constexpr (PK) {
+            pc[u] = xi[u] >> TB;
+            xi[u] = base + (xi[u] & uint32_t(SORT_TILE - 1));
+            d = (pc[u] >> dg.shift) & 255u;
+          } else {
+            d = uint32_t(dg.code(w[u], s_dg) >> dg.shift) & 255u;
+          }
+          gb[u] = s_gbase[d];
+          if constexpr (NEXT) cc[u] = s_cc[d];
+        }
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+          const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
+          const uint32_t g = gb[u] + i;
+          if (ok[u]) {
+            if constexpr (LAST) {
+              iout[g] = int32_t(xi[u]);
+              if (kout != nullptr) kout[g] = Tk::key(w[u], desc);
+            } else {
+              wout[g] = w[u];
+              iout[g] = int32_t(xi[u]);
+            }
+          }
+          if constexpr (NEXT) {
+            // the run's chain from LDS; item by item only in the few runs
+            // that cross a chain boundary
+            uint32_t c = cc[u];
+            if (c == 0xFFu) {
+              c = 0;
+#pragma unroll
+              for (int k = 1; k < NCHAIN; ++k) c += g >= nb[k] ? 1u : 0u;
+            }
+            uint32_t nd;
+            if constexpr (PK)
+              nd = (pc[u] >> dg.next_shift) & 255u;
+            else
+              nd = uint32_t(dg.code(w[u], s_dg) >> dg.next_shift) & 255u;
+            hist_add<DG::PEEL>(&s_nh[0][0], c * 256u + nd, ok[u]);
+          }
+        }
+      }
+    };
+    // (digit generators with a costly code keep one body: the second copy
+    // pushed the learned first pass over its register budget)
+    if (!DG::PACK && base + uint32_t(SORT_TILE) <= n)
+      body(std::true_type{});
+    else
+      body(std::false_type{});
     __syncthreads();  // the tile buffers are reused by the next tile
   }
   if constexpr (NEXT) {
