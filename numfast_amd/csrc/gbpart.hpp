@@ -23,12 +23,15 @@ static_assert(sizeof(GpSlot) == 32, "slot is one 32-byte sector");
 // buckets: the histogram pass's LDS; two 7-bit partition passes).
 bool gp_supported(uint64_t span, int bucket_bits = GP_BUCKET_BITS);
 
-// Fill slots[0, span) with {sum, count, min, max} of the valid rows (vm may
-// be null).  `table` / `hot_keys` / `hot_n` describe the sampled hot keys
-// (table may be null).  Asynchronous on s except one small readback.
-void gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, size_t n,
-                  int32_t kmin, uint64_t span, const int2* table, const int32_t* hot_keys,
-                  const uint32_t* hot_n, GpSlot* slots, hipStream_t s,
-                  int bucket_bits = GP_BUCKET_BITS);
+struct GroupAggOut;
+
+// Groups {key, count, sum, min, max, mean} of the valid rows (vm may be
+// null), keys ascending, written to `out`; returns the group count.
+// `table` / `hot_keys` / `hot_n` describe the sampled hot keys (table may be
+// null).  Synchronous.
+int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm, size_t n,
+                     int32_t kmin, uint64_t span, const int2* table, const int32_t* hot_keys,
+                     const uint32_t* hot_n, const GroupAggOut& out, hipStream_t s,
+                     int bucket_bits = GP_BUCKET_BITS);
 
 }  // namespace nfh

@@ -625,15 +625,14 @@ int64_t group_agg_device(const int32_t* keys, const int32_t* vals, const uint8_t
   if (strat == 0) strat = part_ok ? 2 : dense_ok ? 1 : 3;
   t_last_strategy = strat;
   if (strat == 2 || strat == 5) {
-    // Partitioned path: no per-row global atomics.
-    GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
+    // Partitioned path: no per-row global atomics; the bucket pass writes
+    // the groups themselves (no slot array)
     int2* table = nullptr;
     int32_t* hot_keys = nullptr;
     uint32_t* hot_n = nullptr;
     if (!build_hot_table(keys, n, &table, &hot_keys, &hot_n, s)) table = nullptr;
-    gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, slots, s,
-                 strat == 5 ? GP_BUCKET_BITS_WIDE : GP_BUCKET_BITS);
-    return compact_slots(slots, span, nullptr, kmin, o, s);
+    return gp_aggregate(keys, vals, v, n, kmin, span, table, hot_keys, hot_n, o, s,
+                        strat == 5 ? GP_BUCKET_BITS_WIDE : GP_BUCKET_BITS);
   }
   if (strat == 1) {
     GroupSlot* slots = scratch_t<GroupSlot>(S_WORK0, span);
