@@ -30,9 +30,9 @@ namespace nfh {
 typedef unsigned long long u64;
 
 constexpr int GP_BLOCK = 256;
-// the histogram pass: 512 threads, two workgroups per CU (about 54 KB of
-// LDS each with a 2^24-key span)
-constexpr int GP_HBLOCK = 512;
+// the histogram pass: 1024 threads, one workgroup per CU (about 58 KB of
+// LDS with a 2^24-key span; two of 512 threads measured 1.5% slower)
+constexpr int GP_HBLOCK = 1024;
 // Partition passes use 8192-row tiles (512 threads, ~70 KB of LDS) so a
 // digit's run of records fills whole 128-byte lines when written out; 512
 // threads leave the kernel a 256-VGPR budget (1024 would cap it at 128 and
@@ -675,7 +675,7 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
     const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16) == 0 &&
                      (reinterpret_cast<uintptr_t>(vals) % 16) == 0 &&
                      (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
-    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_HBLOCK, 2 * num_cus())),
+    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_HBLOCK, num_cus())),
                        dim3(GP_HBLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
                        chist, tiles, vec, bb, mode);
     NFH_LAUNCH_CHECK();
