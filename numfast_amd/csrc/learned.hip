@@ -55,6 +55,7 @@ constexpr int LS_IPT = 10;
 constexpr int LS_CAP = LS_BLOCK * LS_IPT;  // 5120 keys per bucket in LDS
 constexpr int LS_DBITS = 9;                // in-bucket digit width
 constexpr int LS_DIG = 1 << LS_DBITS;
+static_assert(LS_BLOCK >= LS_DIG, "a thread per in-bucket digit in the scan");
 
 // ------------------------------------------------------------ code map
 // 32-bit integer arithmetic: with u = clamp(w - wmin, 0, span),
@@ -449,16 +450,20 @@ __global__ __launch_bounds__(LS_BLOCK, 6) void k_bucket_sort(const uint32_t* __r
     }
     __syncthreads();
     {
-      // thread == digit: offsets of each wave's run, digit starts
+      // thread == digit (threads past the last digit scan zeros): offsets
+      // of each wave's run, digit starts
       uint32_t c = 0;
+      if (tid < LS_DIG) {
 #pragma unroll
-      for (int w = 0; w < LS_WAVES; ++w) {
-        const uint32_t x = s_cnt[w][tid];
-        s_cnt[w][tid] = c;
-        c += x;
+        for (int w = 0; w < LS_WAVES; ++w) {
+          const uint32_t x = s_cnt[w][tid];
+          s_cnt[w][tid] = c;
+          c += x;
+        }
       }
       uint32_t tot;
-      s_dst[tid] = block_excl_scan<LS_WAVES>(c, s_tmp, &tot);
+      const uint32_t ex = block_excl_scan<LS_WAVES>(c, s_tmp, &tot);
+      if (tid < LS_DIG) s_dst[tid] = ex;
     }
     __syncthreads();
 #pragma unroll
