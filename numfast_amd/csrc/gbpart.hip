@@ -207,7 +207,11 @@ __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ h
 // tile_ctr = NCHAIN zeroed counters; persistent workgroups loop over their
 // chain's tiles.  NEXT: count each record's next digit per chain of the
 // next pass (next_tiles tiles of the same size) into next_hist[NCHAIN][128].
-template <bool FIRST, bool LR, bool NEXT>
+// PO (second pass only, every value in [-2^15, 2^15)): the pass writes one
+// 32-bit record per row, the key's slot in its bucket (the low `sbits`
+// bits of o) below the biased value (x + 2^15); the bucket itself is
+// implied by the record's position after the last pass.
+template <bool FIRST, bool LR, bool NEXT, bool PO = false>
 __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n_rows, int32_t kmin,
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     uint32_t* __restrict__ oout, int32_t* __restrict__ vout, int shift,
     const uint32_t* __restrict__ dstart, uint64_t* __restrict__ lookback,
     uint32_t* __restrict__ tile_ctr, uint32_t epoch, const int2* __restrict__ table,
-    uint32_t next_tiles, uint32_t* __restrict__ next_hist) {
+    uint32_t next_tiles, uint32_t* __restrict__ next_hist, int sbits) {
   extern __shared__ __align__(16) unsigned char s_raw[];
   uint32_t* s_o = reinterpret_cast<uint32_t*>(s_raw);
   int32_t* s_v = reinterpret_cast<int32_t*>(s_o + GP_TILE);
@@ -400,8 +404,12 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
         const uint32_t i = i0 + uint32_t(u * GP_PBLOCK + tid);
         const uint32_t g = gb[u] + i;
         if (ok[u]) {
-          oout[g] = ob[u];
-          vout[g] = int32_t(xv[u]);
+          if constexpr (PO) {
+            oout[g] = (ob[u] & ((1u << sbits) - 1u)) | ((xv[u] + 32768u) << sbits);
+          } else {
+            oout[g] = ob[u];
+            vout[g] = int32_t(xv[u]);
+          }
         }
         if constexpr (NEXT) {
           uint32_t c = cc[u];
@@ -445,7 +453,9 @@ constexpr size_t gp_emit_lds() {
   return size_t(1 << BITS) * ((PACKED ? 16 : 20) + 2);
 }
 
-template <int BITS, int BLOCK, bool PACKED>
+// PR: the records are the second pass's packed 32-bit form (slot below
+// the biased value; v unused).
+template <int BITS, int BLOCK, bool PACKED, bool PR = false>
 __global__ __launch_bounds__(BLOCK) void k_gp_bucket_emit(
     const uint32_t* __restrict__ o, const int32_t* __restrict__ v,
     const uint32_t* __restrict__ bstart, uint32_t nb, uint64_t span, int32_t kmin,
@@ -457,6 +467,7 @@ __global__ __launch_bounds__(BLOCK) void k_gp_bucket_emit(
   constexpr int PER = W / NWV;     // slots per wave
   constexpr int CHK = PER / WAVE;  // ballots per wave
   static_assert(PER % WAVE == 0, "whole 64-slot chunks per wave");
+  static_assert(!PR || PACKED, "packed records carry values in [-2^15, 2^15)");
   extern __shared__ __align__(16) unsigned char s_raw[];
   u64* a_sum = reinterpret_cast<u64*>(s_raw);  // PACKED: count << 40 | biased sum
   uint32_t* a_cnt = reinterpret_cast<uint32_t*>(a_sum + W);  // unused when PACKED
@@ -536,14 +547,14 @@ __global__ __launch_bounds__(BLOCK) void k_gp_bucket_emit(
       const uint32_t i = i0 + uint32_t(r * BLOCK) + tid;
       const uint32_t ic = i < hi ? i : hi - 1u;
       os[r] = o[ic];
-      xs[r] = v[ic];
+      if constexpr (!PR) xs[r] = v[ic];
     }
     asm volatile("" ::: "memory");  // keep every load above the adds
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       if (i0 + uint32_t(r * BLOCK) + tid >= hi) continue;
       const uint32_t sl = os[r] & (W - 1);
-      const int32_t x = xs[r];
+      const int32_t x = PR ? int32_t(os[r] >> BITS) - 32768 : xs[r];
       if constexpr (PACKED) {
         atomicAdd(&a_sum[sl], (u64(1) << 40) | u64(uint32_t(x + 32768)));
       } else {
@@ -698,14 +709,16 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
   uint64_t* lb = lookback_words(size_t(tiles) * 128, s);
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
-    const void* fns[8] = {reinterpret_cast<const void*>(&k_gp_part<true, true, true>),
-                          reinterpret_cast<const void*>(&k_gp_part<true, false, true>),
-                          reinterpret_cast<const void*>(&k_gp_part<false, true, true>),
-                          reinterpret_cast<const void*>(&k_gp_part<false, false, true>),
-                          reinterpret_cast<const void*>(&k_gp_part<true, true, false>),
-                          reinterpret_cast<const void*>(&k_gp_part<true, false, false>),
-                          reinterpret_cast<const void*>(&k_gp_part<false, true, false>),
-                          reinterpret_cast<const void*>(&k_gp_part<false, false, false>)};
+    const void* fns[10] = {reinterpret_cast<const void*>(&k_gp_part<true, true, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<true, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, true, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<true, true, false>),
+                           reinterpret_cast<const void*>(&k_gp_part<true, false, false>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, true, false>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, false, false>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, true, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, false, false, true>)};
     for (const void* f : fns)
       NFH_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(GP_PART_LDS)));
@@ -716,6 +729,11 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
         int(gp_emit_lds<GP_BUCKET_BITS_WIDE, false>())));
     NFH_CHECK(hipFuncSetAttribute(
         reinterpret_cast<const void*>(&k_gp_bucket_emit<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK, true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize,
+        int(gp_emit_lds<GP_BUCKET_BITS_WIDE, true>())));
+    NFH_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(
+            &k_gp_bucket_emit<GP_BUCKET_BITS_WIDE, 2 * GP_BLOCK, true, true>),
         hipFuncAttributeMaxDynamicSharedMemorySize,
         int(gp_emit_lds<GP_BUCKET_BITS_WIDE, true>())));
   });
@@ -735,21 +753,26 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
                         : (lr ? &k_gp_part<true, true, false> : &k_gp_part<true, false, false>);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n, kmin,
                        nullptr, nullptr, 0u, o0, v0, bb, dstart, lb, ctr, epoch, table,
-                       t2, chist + CH);
+                       t2, chist + CH, bb);
     NFH_LAUNCH_CHECK();
   }
   const uint32_t* fo = o0;
   const int32_t* fv = v0;
+  // packed records out of the second pass when every value fits 16 bits
+  // (the bucket pass then reads 4 B per record instead of 8, twice)
+  const bool pr = two && packed;
   if (passes == 2) {
     if (m > 0) {
       hipLaunchKernelGGL(k_chain_bins<128>, dim3(1), dim3(128), 0, s, chist + CH, dstart + CH);
       NFH_LAUNCH_CHECK();
-      ProfScope prof("gp_part", s, double(m) * 16.0);
+      ProfScope prof("gp_part", s, double(m) * (pr ? 12.0 : 16.0));
       const uint32_t epoch = lookback_epoch(s);
-      const auto fn = lr ? &k_gp_part<false, true, false> : &k_gp_part<false, false, false>;
+      const auto fn = pr ? (lr ? &k_gp_part<false, true, false, true>
+                               : &k_gp_part<false, false, false, true>)
+                         : (lr ? &k_gp_part<false, true, false> : &k_gp_part<false, false, false>);
       hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
                          nullptr, 0u, kmin, o0, v0, m, o1, v1, bb + 7, dstart + CH,
-                         lb, ctr + NCHAIN, epoch, nullptr, 0u, nullptr);
+                         lb, ctr + NCHAIN, epoch, nullptr, 0u, nullptr, bb);
       NFH_LAUNCH_CHECK();
     }
     fo = o1;
@@ -762,23 +785,27 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
   {
     // reads the records (twice: the presence count, then the aggregation)
     // and writes 40 B per group; the byte model counts the first read
-    ProfScope prof("gp_bucket_emit", s, double(m) * 8.0);
+    ProfScope prof("gp_bucket_emit", s, double(m) * (pr ? 4.0 : 8.0));
     // 512 threads per bucket for both bucket widths (2048-key buckets:
     // 0.57 ms with 256, 0.47 with 512, 0.56 with 1024 at 2^28 Zipf rows)
-#define NFH_GP_EMIT(BITS, PK)                                                                \
-  hipLaunchKernelGGL((k_gp_bucket_emit<BITS, 2 * GP_BLOCK, PK>), dim3(nb), dim3(2 * GP_BLOCK), \
-                     (gp_emit_lds<BITS, PK>()), s, fo, fv, bstart, nb, span, kmin, hot_acc,    \
-                     hot_keys, hn, out, lbb, ticket, epoch, ng_d)
+#define NFH_GP_EMIT(BITS, PK, PRV)                                                           \
+  hipLaunchKernelGGL((k_gp_bucket_emit<BITS, 2 * GP_BLOCK, PK, PRV>), dim3(nb),                \
+                     dim3(2 * GP_BLOCK), (gp_emit_lds<BITS, PK>()), s, fo, fv, bstart, nb, span, \
+                     kmin, hot_acc, hot_keys, hn, out, lbb, ticket, epoch, ng_d)
     if (bb == GP_BUCKET_BITS_WIDE) {
-      if (packed)
-        NFH_GP_EMIT(GP_BUCKET_BITS_WIDE, true);
+      if (pr)
+        NFH_GP_EMIT(GP_BUCKET_BITS_WIDE, true, true);
+      else if (packed)
+        NFH_GP_EMIT(GP_BUCKET_BITS_WIDE, true, false);
       else
-        NFH_GP_EMIT(GP_BUCKET_BITS_WIDE, false);
+        NFH_GP_EMIT(GP_BUCKET_BITS_WIDE, false, false);
     } else {
-      if (packed)
-        NFH_GP_EMIT(GP_BUCKET_BITS, true);
+      if (pr)
+        NFH_GP_EMIT(GP_BUCKET_BITS, true, true);
+      else if (packed)
+        NFH_GP_EMIT(GP_BUCKET_BITS, true, false);
       else
-        NFH_GP_EMIT(GP_BUCKET_BITS, false);
+        NFH_GP_EMIT(GP_BUCKET_BITS, false, false);
     }
 #undef NFH_GP_EMIT
     NFH_LAUNCH_CHECK();
