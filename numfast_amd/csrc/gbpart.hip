@@ -43,6 +43,12 @@ constexpr int GP_IPT = 16;
 constexpr int GP_TILE = GP_PBLOCK * GP_IPT;
 constexpr size_t GP_PART_LDS = size_t(GP_TILE) * 8 + GP_WAVES * 128 * 4 + 128 * 4 * 2 +
                                GP_WAVES * 4 + 16 + size_t(NCHAIN) * 128 * 4 + 128;
+// Regions of the histogram pass's record mode (one per workgroup of that
+// pass, a multiple of NCHAIN)
+constexpr int GP_REG_MAX = 256;
+static_assert(GP_REG_MAX <= GP_PBLOCK, "one region per thread in the region scan");
+static_assert(2 * (GP_PART_LDS + 4 * (GP_REG_MAX + 1)) <= 160 * 1024,
+              "two partition workgroups per CU with the region table");
 static_assert(HOT_MAX == 512 && HOT_TBL == 1024, "LDS budget of the passes below");
 
 // ------------------------------------------------------------ histogram
@@ -57,14 +63,28 @@ struct HotAcc {
 // One persistent pass over the rows: hot rows aggregate in LDS (flushed
 // once per workgroup), every other valid row is counted in its 2048-key
 // bucket.
+//
+// Record mode (rec_o set and the sampled hot keys carrying at least
+// reg_min of the HOT_SAMPLE sampled rows, so that few rows are records):
+// workgroup g takes the contiguous rows [g * rsize, (g + 1) * rsize) and
+// also writes each record (key - kmin, value) of them, compacted, from
+// rec_o / rec_v + g * rsize on; reg_cnt[g] = its record count.  Region g
+// belongs to look-back chain g / (gridDim.x / NCHAIN) of the first
+// partition pass, which then reads the records instead of the rows.  The
+// records of one wave's batch are stored in (quad, row, lane) order, so
+// each store instruction writes one contiguous run.  mode bit 2 says the
+// mode was taken.
 __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n, int32_t kmin, uint32_t nb,
     const int2* __restrict__ table, uint32_t* __restrict__ hist, HotAcc* __restrict__ hot_acc,
     uint32_t* __restrict__ chist, uint32_t ptiles, bool vec, int bbits,
-    uint32_t* __restrict__ mode) {
+    uint32_t* __restrict__ mode, const uint32_t* __restrict__ hot_n, uint32_t reg_min,
+    uint32_t* __restrict__ rec_o, int32_t* __restrict__ rec_v, uint32_t* __restrict__ reg_cnt,
+    uint32_t rsize) {
   extern __shared__ uint32_t s_h[];  // nb counters
   __shared__ uint32_t s_c0[NCHAIN][128];  // digit-0 counts per chain of the first pass
+  __shared__ uint32_t s_cur;              // records of this workgroup (record mode)
   __shared__ int32_t t_key[HOT_TBL], t_idx[HOT_TBL];
   // hot aggregates: values in [-2^15, 2^15) add (1 << 40) | (x + 2^15) to
   // h_pk (count in bits 40..63, biased sum below: exact while a workgroup
@@ -83,12 +103,14 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     h_mn[b] = INT32_MAX;
     h_mx[b] = INT32_MIN;
   }
+  if (threadIdx.x == 0) s_cur = 0;
   __syncthreads();
   bool oor = false;  // a record value outside [-2^15, 2^15) (mode bit 0)
   // chain of row i's tile in the first partition pass
   auto chain_of_row = [&](uint32_t i) { return chain_of_tile(ptiles, i / uint32_t(GP_TILE)); };
-  auto row = [&](int32_t k, int32_t x, bool ok, uint32_t ch) {
-    if (!ok) return;
+  // true when the row is a record (valid and not hot)
+  auto row = [&](int32_t k, int32_t x, bool ok, uint32_t ch) -> bool {
+    if (!ok) return false;
     const int hi = table ? hot_index(t_key, t_idx, k) : -1;
     if (hi >= 0) {
       if (x >= -32768 && x < 32768) {
@@ -100,17 +122,95 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
       // min / max only when x would change them (see k_gp_bucket_emit)
       if (x < h_mn[hi]) atomicMin(&h_mn[hi], x);
       if (x > h_mx[hi]) atomicMax(&h_mx[hi], x);
-    } else {
-      const uint32_t b = uint32_t(k - kmin) >> bbits;
-      oor |= x < -32768 || x >= 32768;
-      atomicAdd(&s_h[b], 1u);
-      atomicAdd(&s_c0[ch][b & 127u], 1u);
+      return false;
     }
+    const uint32_t b = uint32_t(k - kmin) >> bbits;
+    oor |= x < -32768 || x >= 32768;
+    atomicAdd(&s_h[b], 1u);
+    atomicAdd(&s_c0[ch][b & 127u], 1u);
+    return true;
   };
+  const bool reg = rec_o != nullptr && table != nullptr && hot_n[1] >= reg_min;
+  if (reg) {
+    // record mode: one region per workgroup (the host guarantees
+    // gridDim.x * rsize >= n, rsize % 4 == 0 and aligned quads)
+    const uint32_t ch = blockIdx.x / (gridDim.x / NCHAIN);
+    const uint32_t r0 = blockIdx.x * rsize;
+    const uint32_t r1 = r0 < n ? min(n, r0 + rsize) : r0;
+    const uint32_t q0 = r0 / 4, q1 = r1 / 4;  // whole quads of the region
+    const uint64_t lt = lanemask_lt();
+    const int lane = lane_id();
+    // Emits the records whose bit is set in `cold` (bit 4u + r: row r of
+    // quad u), in (u, r, lane) order, at the wave's share of the region.
+    auto emit = [&](uint32_t cold, const int4* ka, const int4* va, int nq) {
+      uint64_t bal[4 * 4];
+      uint32_t tot = 0;
+#pragma unroll
+      for (int sl = 0; sl < 16; ++sl) {
+        bal[sl] = sl < 4 * nq ? __ballot((cold >> sl) & 1u) : 0ull;
+        tot += uint32_t(__popcll(bal[sl]));
+      }
+      if (tot == 0) return;  // uniform over the wave
+      uint32_t wb = 0;
+      if (lane == 0) wb = atomicAdd(&s_cur, tot);
+      wb = r0 + uint32_t(__shfl(int(wb), 0));
+#pragma unroll
+      for (int sl = 0; sl < 16; ++sl) {
+        if (sl < 4 * nq && ((cold >> sl) & 1u)) {
+          const int4 k4 = ka[sl >> 2], v4 = va[sl >> 2];
+          const int r = sl & 3;
+          const int32_t k = r == 0 ? k4.x : r == 1 ? k4.y : r == 2 ? k4.z : k4.w;
+          const int32_t x = r == 0 ? v4.x : r == 1 ? v4.y : r == 2 ? v4.z : v4.w;
+          const uint32_t at = wb + uint32_t(__popcll(bal[sl] & lt));
+          rec_o[at] = uint32_t(k - kmin);
+          rec_v[at] = x;
+        }
+        wb += uint32_t(__popcll(bal[sl]));
+      }
+    };
+    constexpr int RQ = 4;  // quads per thread per batch, all loaded first
+    for (uint32_t qb = q0; qb < q1; qb += RQ * GP_HBLOCK) {  // uniform trip count
+      int4 ka[RQ], va[RQ];
+      uint32_t ma[RQ];
+      bool in[RQ];
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        const uint32_t qq = qb + uint32_t(u * GP_HBLOCK) + threadIdx.x;
+        in[u] = qq < q1;
+        const uint32_t qc = in[u] ? qq : q1 - 1u;
+        ka[u] = reinterpret_cast<const int4*>(keys)[qc];
+        va[u] = reinterpret_cast<const int4*>(vals)[qc];
+        ma[u] = vm ? reinterpret_cast<const uint32_t*>(vm)[qc] : 0x01010101u;
+      }
+      asm volatile("" ::: "memory");  // keep every load above the adds
+      uint32_t cold = 0;
+#pragma unroll
+      for (int u = 0; u < RQ; ++u) {
+        if (!in[u]) continue;
+        cold |= uint32_t(row(ka[u].x, va[u].x, ma[u] & 0xFFu, ch)) << (4 * u);
+        cold |= uint32_t(row(ka[u].y, va[u].y, ma[u] & 0xFF00u, ch)) << (4 * u + 1);
+        cold |= uint32_t(row(ka[u].z, va[u].z, ma[u] & 0xFF0000u, ch)) << (4 * u + 2);
+        cold |= uint32_t(row(ka[u].w, va[u].w, ma[u] & 0xFF000000u, ch)) << (4 * u + 3);
+      }
+      emit(cold, ka, va, RQ);
+    }
+    // the last 0-3 rows of the region (only the last region ends off a quad)
+    {
+      const uint32_t i = q1 * 4u + threadIdx.x;
+      int4 k1 = make_int4(0, 0, 0, 0), v1 = make_int4(0, 0, 0, 0);
+      uint32_t cold = 0;
+      if (i < r1) {
+        k1.x = keys[i];
+        v1.x = vals[i];
+        cold = uint32_t(row(k1.x, v1.x, !vm || vm[i], ch));
+      }
+      emit(cold, &k1, &v1, 1);
+    }
+  }
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t tail = 0;
-  if (vec) {
+  uint32_t tail = reg ? n : 0u;
+  if (vec && !reg) {
     // quads: 128-bit key / value loads, a 32-bit load of four mask bytes,
     // HQ quads in flight per thread (all loaded before any row is added)
     constexpr int HQ = 4;
@@ -152,6 +252,10 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     row(keys[i], vals[i], !vm || vm[i], chain_of_row(i));
   if (__ballot(oor) != 0 && lane_id() == 0) atomicOr(mode, 1u);
   __syncthreads();
+  if (reg && threadIdx.x == 0) {
+    reg_cnt[blockIdx.x] = s_cur;
+    if (blockIdx.x == 0) atomicOr(mode, 4u);
+  }
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
   for (uint32_t b = threadIdx.x; b < NCHAIN * 128; b += blockDim.x) {
@@ -211,7 +315,12 @@ __global__ __launch_bounds__(1024) void k_gp_bins(const uint32_t* __restrict__ h
 // 32-bit record per row, the key's slot in its bucket (the low `sbits`
 // bits of o) below the biased value (x + 2^15); the bucket itself is
 // implied by the record's position after the last pass.
-template <bool FIRST, bool LR, bool NEXT, bool PO = false>
+// REG (first pass over the histogram pass's records): the input is nreg
+// regions of rsize slots, region g holding reg_cnt[g] records from slot
+// g * rsize on; each region is cut into whole tiles and one partial tile,
+// and chain c takes the tiles of regions [c, c + 1) * nreg / NCHAIN, in
+// order.
+template <bool FIRST, bool LR, bool NEXT, bool PO = false, bool REG = false>
 __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     const int32_t* __restrict__ keys, const int32_t* __restrict__ vals,
     const uint8_t* __restrict__ vm, uint32_t n_rows, int32_t kmin,
@@ -219,7 +328,9 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     uint32_t* __restrict__ oout, int32_t* __restrict__ vout, int shift,
     const uint32_t* __restrict__ dstart, uint64_t* __restrict__ lookback,
     uint32_t* __restrict__ tile_ctr, uint32_t epoch, const int2* __restrict__ table,
-    uint32_t next_tiles, uint32_t* __restrict__ next_hist, int sbits) {
+    uint32_t next_tiles, uint32_t* __restrict__ next_hist, int sbits,
+    const uint32_t* __restrict__ reg_cnt, uint32_t nreg, uint32_t rsize) {
+  static_assert(!(REG && FIRST), "record regions are an input of a record pass");
   extern __shared__ __align__(16) unsigned char s_raw[];
   uint32_t* s_o = reinterpret_cast<uint32_t*>(s_raw);
   int32_t* s_v = reinterpret_cast<int32_t*>(s_o + GP_TILE);
@@ -233,13 +344,30 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
   // when the run crosses a chain boundary
   unsigned char* s_cc = reinterpret_cast<unsigned char*>(s_nh + NCHAIN * 128);
   __shared__ int32_t t_key[FIRST ? HOT_TBL : 1];
+  // REG: first tile of each region (s_rf[nreg] = all tiles)
+  __shared__ uint32_t s_rf[REG ? GP_REG_MAX + 1 : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
   const uint32_t n = FIRST ? n_rows : n_rec;
-  const uint32_t T = (n + GP_TILE - 1) / GP_TILE;
   const uint32_t chain = blockIdx.x % NCHAIN;
-  const uint32_t cfirst = chain_first(T, chain);
-  const uint32_t ctiles = chain_tiles(T, chain);
+  uint32_t cfirst, ctiles;
+  if constexpr (REG) {
+    // the host guarantees nreg <= GP_REG_MAX <= GP_PBLOCK, nreg % NCHAIN == 0
+    const uint32_t rt =
+        uint32_t(tid) < nreg ? (reg_cnt[tid] + uint32_t(GP_TILE) - 1u) / uint32_t(GP_TILE) : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<GP_WAVES>(rt, s_tmp, &tot);
+    if (uint32_t(tid) < nreg) s_rf[tid] = ex;
+    if (tid == 0) s_rf[nreg] = tot;
+    __syncthreads();
+    const uint32_t rpc = nreg / NCHAIN;
+    cfirst = s_rf[chain * rpc];
+    ctiles = s_rf[(chain + 1) * rpc] - cfirst;
+  } else {
+    const uint32_t T = (n + GP_TILE - 1) / GP_TILE;
+    cfirst = chain_first(T, chain);
+    ctiles = chain_tiles(T, chain);
+  }
   const uint32_t* bins = dstart + chain * 128;
   uint32_t nb[NCHAIN];
 #pragma unroll
@@ -256,7 +384,21 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
     if (local >= ctiles) break;  // uniform over the workgroup
     const uint32_t tile = cfirst + local;
     const bool head = local == 0;
-    const uint32_t base = tile * uint32_t(GP_TILE);
+    uint32_t base = tile * uint32_t(GP_TILE);
+    uint32_t lim = n;  // items of this tile are those below lim
+    if constexpr (REG) {
+      // the region holding this tile: the last of the chain's regions whose
+      // first tile is at or before it (regions without tiles share their
+      // first tile with the next one)
+      const uint32_t rpc = nreg / NCHAIN;
+      uint32_t lo = chain * rpc, hi = lo + rpc;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (s_rf[mid] <= tile) lo = mid; else hi = mid;
+      }
+      base = lo * rsize + (tile - s_rf[lo]) * uint32_t(GP_TILE);
+      lim = lo * rsize + reg_cnt[lo];
+    }
     const uint32_t wbase = base + uint32_t(wave) * (WAVE * GP_IPT);
 
     uint32_t ov[GP_IPT];
@@ -300,10 +442,10 @@ __global__ __launch_bounds__(GP_PBLOCK) void k_gp_part(
 #pragma unroll
       for (int j = 0; j < GP_IPT; ++j) {
         const uint32_t pos = wbase + uint32_t(j * WAVE + lane);
-        const uint32_t pc = pos < n ? pos : n - 1u;
+        const uint32_t pc = pos < lim ? pos : lim - 1u;
         ov[j] = oin[pc];
         vv[j] = vin[pc];
-        live |= uint32_t(pos < n) << j;
+        live |= uint32_t(pos < lim) << j;
       }
     }
 
@@ -663,7 +805,7 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
   // [2][CH] | per-chain digit starts [2][CH] | tile counters [2][NCHAIN] |
   // hot_acc
   const size_t meta_words = size_t(nb) * 2 + 2 + 4 * CH + 2 * NCHAIN;
-  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + 4 + HOT_MAX * 6 + 8);
+  uint32_t* meta = scratch_t<uint32_t>(S_META, meta_words + 4 + HOT_MAX * 6 + 8 + GP_REG_MAX);
   uint32_t* hist = meta;
   uint32_t* bstart = hist + nb;
   uint32_t* mode = bstart + nb + 1;
@@ -671,24 +813,49 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
   uint32_t* dstart = chist + 2 * CH;
   uint32_t* ctr = dstart + 2 * CH;
   HotAcc* hot_acc = reinterpret_cast<HotAcc*>(meta + ((meta_words + 3) & ~size_t(3)));
-  // bucket ticket, then the group count (8-byte aligned)
+  // bucket ticket, then the group count (8-byte aligned), then the record
+  // counts of the histogram pass's regions
   uint32_t* ticket = reinterpret_cast<uint32_t*>(hot_acc + HOT_MAX);
   int64_t* ng_d = reinterpret_cast<int64_t*>(ticket + 2);
+  uint32_t* reg_cnt = reinterpret_cast<uint32_t*>(ng_d + 1);
   fill_u32(meta, 0, meta_words, s);
   hipLaunchKernelGGL(k_gp_hotinit, dim3(2), dim3(256), 0, s, hot_acc, uint32_t(HOT_MAX));
   NFH_LAUNCH_CHECK();
+  // record buffers (ping-pong): at most n records
+  uint32_t* o0 = scratch_t<uint32_t>(S_T0, n);
+  int32_t* v0 = scratch_t<int32_t>(S_T1, n);
+  uint32_t* o1 = scratch_t<uint32_t>(S_T2, n);
+  int32_t* v1 = scratch_t<int32_t>(S_T3, n);
   const double mask_b = vm ? 1.0 : 0.0;
   const uint32_t tiles = (n + GP_TILE - 1) / GP_TILE;
+  const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16) == 0 &&
+                   (reinterpret_cast<uintptr_t>(vals) % 16) == 0 &&
+                   (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
+  // Record mode of the histogram pass: one region of rows per workgroup
+  // (one workgroup per CU, a multiple of NCHAIN), offered when the sampled
+  // hot keys exist; the pass takes it when they carry at least 40% of the
+  // sampled rows (the first partition pass then reads 8 B per record
+  // instead of 9 B per row and skips the hot-key lookups: at 2^28 Zipf(1.1)
+  // rows, first partition pass + histogram pass 0.87 + 0.57 ms became
+  // 0.40 + 0.70 ms).
+  unsigned hgrid = grid_for(n / 8 + 1, GP_HBLOCK, num_cus());
+  const uint32_t nreg = uint32_t(std::min(GP_REG_MAX, num_cus() / NCHAIN * NCHAIN));
+  const bool offer_reg = table != nullptr && vec && n >= (1u << 20) && n <= (1u << 31) &&
+                         nreg >= uint32_t(NCHAIN);
+  uint32_t rsize = 0;
+  if (offer_reg) {
+    hgrid = nreg;
+    rsize = ((n + nreg - 1) / nreg + 63u) & ~63u;  // nreg * rsize >= n, whole quads
+  }
+  constexpr uint32_t REG_MIN = uint32_t(HOT_SAMPLE) * 2u / 5u;
   {
-    // algorithmic bytes: key + value + validity per row
+    // algorithmic bytes: key + value + validity per row (+ 8 B per record
+    // in record mode, added below once the count is known)
     ProfScope prof("gp_hist", s, double(n) * (8.0 + mask_b));
     const size_t lds = size_t(nb) * 4;
-    const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16) == 0 &&
-                     (reinterpret_cast<uintptr_t>(vals) % 16) == 0 &&
-                     (vm == nullptr || (reinterpret_cast<uintptr_t>(vm) % 4) == 0);
-    hipLaunchKernelGGL(k_gp_hist, dim3(grid_for(n / 8 + 1, GP_HBLOCK, num_cus())),
-                       dim3(GP_HBLOCK), lds, s, keys, vals, vm, n, kmin, nb, table, hist, hot_acc,
-                       chist, tiles, vec, bb, mode);
+    hipLaunchKernelGGL(k_gp_hist, dim3(hgrid), dim3(GP_HBLOCK), lds, s, keys, vals, vm, n, kmin,
+                       nb, table, hist, hot_acc, chist, tiles, vec, bb, mode, hot_n, REG_MIN,
+                       offer_reg ? o1 : nullptr, v1, reg_cnt, rsize);
     NFH_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_gp_bins, dim3(1), dim3(1024), 0, s, hist, nb, bstart, mode);
@@ -700,16 +867,14 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
   NFH_CHECK(hipMemcpyAsync(h_m, bstart + nb, 8, hipMemcpyDeviceToHost, s));
   NFH_CHECK(hipStreamSynchronize(s));
   const uint32_t m = h_m[0];
-  const bool packed = h_m[1] == 0u;
-  // record buffers (ping-pong): at most n records
-  uint32_t* o0 = scratch_t<uint32_t>(S_T0, n);
-  int32_t* v0 = scratch_t<int32_t>(S_T1, n);
-  uint32_t* o1 = scratch_t<uint32_t>(S_T2, n);
-  int32_t* v1 = scratch_t<int32_t>(S_T3, n);
-  uint64_t* lb = lookback_words(size_t(tiles) * 128, s);
+  const bool packed = (h_m[1] & 3u) == 0u;
+  const bool regm = (h_m[1] & 4u) != 0u;
+  if (regm) prof_add_bytes("gp_hist", double(m) * 8.0);
+  // at most one partial tile per region beyond the whole tiles
+  uint64_t* lb = lookback_words((size_t(tiles) + nreg) * 128, s);
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
-    const void* fns[10] = {reinterpret_cast<const void*>(&k_gp_part<true, true, true>),
+    const void* fns[14] = {reinterpret_cast<const void*>(&k_gp_part<true, true, true>),
                            reinterpret_cast<const void*>(&k_gp_part<true, false, true>),
                            reinterpret_cast<const void*>(&k_gp_part<false, true, true>),
                            reinterpret_cast<const void*>(&k_gp_part<false, false, true>),
@@ -718,7 +883,11 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
                            reinterpret_cast<const void*>(&k_gp_part<false, true, false>),
                            reinterpret_cast<const void*>(&k_gp_part<false, false, false>),
                            reinterpret_cast<const void*>(&k_gp_part<false, true, false, true>),
-                           reinterpret_cast<const void*>(&k_gp_part<false, false, false, true>)};
+                           reinterpret_cast<const void*>(&k_gp_part<false, false, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, true, true, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, false, true, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, true, false, false, true>),
+                           reinterpret_cast<const void*>(&k_gp_part<false, false, false, false, true>)};
     for (const void* f : fns)
       NFH_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(GP_PART_LDS)));
@@ -743,8 +912,21 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
   // persistent: two workgroups per CU (about 75 KB of LDS each), a
   // multiple of NCHAIN
   const unsigned grid = unsigned(std::max<int>(NCHAIN, 2 * num_cus() / NCHAIN * NCHAIN));
-  // pass 1 always runs: it drops NULL and hot rows (digit 0 when nb <= 1)
-  {
+  // pass 1 always runs: over the rows it drops NULL and hot rows (digit 0
+  // when nb <= 1); in record mode it reads the histogram pass's records
+  if (regm) {
+    // reads and writes (o, value) per record
+    ProfScope prof("gp_part_first", s, double(m) * 16.0);
+    const uint32_t epoch = lookback_epoch(s);
+    const auto fn = two ? (lr ? &k_gp_part<false, true, true, false, true>
+                              : &k_gp_part<false, false, true, false, true>)
+                        : (lr ? &k_gp_part<false, true, false, false, true>
+                              : &k_gp_part<false, false, false, false, true>);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
+                       nullptr, 0u, kmin, o1, v1, m, o0, v0, bb, dstart, lb, ctr, epoch,
+                       nullptr, t2, chist + CH, bb, reg_cnt, nreg, rsize);
+    NFH_LAUNCH_CHECK();
+  } else {
     // reads key + validity of every row and the value of each record,
     // writes (o, value) per record
     ProfScope prof("gp_part_first", s, double(n) * (4.0 + mask_b) + double(m) * 12.0);
@@ -753,7 +935,7 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
                         : (lr ? &k_gp_part<true, true, false> : &k_gp_part<true, false, false>);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, keys, vals, vm, n, kmin,
                        nullptr, nullptr, 0u, o0, v0, bb, dstart, lb, ctr, epoch, table,
-                       t2, chist + CH, bb);
+                       t2, chist + CH, bb, nullptr, 0u, 0u);
     NFH_LAUNCH_CHECK();
   }
   const uint32_t* fo = o0;
@@ -772,7 +954,7 @@ int64_t gp_aggregate(const int32_t* keys, const int32_t* vals, const uint8_t* vm
                          : (lr ? &k_gp_part<false, true, false> : &k_gp_part<false, false, false>);
       hipLaunchKernelGGL(fn, dim3(grid), dim3(GP_PBLOCK), GP_PART_LDS, s, nullptr, nullptr,
                          nullptr, 0u, kmin, o0, v0, m, o1, v1, bb + 7, dstart + CH,
-                         lb, ctr + NCHAIN, epoch, nullptr, 0u, nullptr, bb);
+                         lb, ctr + NCHAIN, epoch, nullptr, 0u, nullptr, bb, nullptr, 0u, 0u);
       NFH_LAUNCH_CHECK();
     }
     fo = o1;

@@ -43,7 +43,6 @@ typedef unsigned long long u64;
 
 using GroupSlot = GpSlot;
 
-constexpr int HOT_SAMPLE = 1 << 16;
 constexpr uint32_t HOT_MIN_COUNT = 4;  // sampled occurrences to qualify
 constexpr int HOT_LEN_BUCKETS = 1024;
 constexpr int ACC_BLOCK = 256;
@@ -262,7 +261,8 @@ __global__ __launch_bounds__(1024) void k_hot_count(const int32_t* __restrict__ 
 }
 
 // One block: keep the candidates counted most often (at most HOT_MAX) and
-// build their cuckoo table.
+// build their cuckoo table.  hot_n[0] = hot keys, hot_n[1] = sampled rows
+// that carry one of them (of HOT_SAMPLE).
 __global__ __launch_bounds__(1024) void k_hot_pick(const int2* __restrict__ cand,
                                                    const uint32_t* __restrict__ ncand,
                                                    int32_t* __restrict__ hot_keys,
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int2* __restrict__ cand
   __shared__ unsigned long long cell[HOT_TBL];
   __shared__ int32_t t_idx[HOT_TBL];
   __shared__ int32_t t_key[HOT_TBL];
-  __shared__ uint32_t s_thr, s_cnt;
+  __shared__ uint32_t s_thr, s_cnt, s_cov;
   __shared__ uint32_t s_tmp[1024 / WAVE];
   static_assert(HOT_LEN_BUCKETS == 1024, "one count bucket per thread below");
   const uint32_t nc = *ncand;
@@ -281,6 +281,7 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int2* __restrict__ cand
   if (threadIdx.x == 0) {
     s_thr = 0xFFFFFFFFu;
     s_cnt = 0;
+    s_cov = 0;
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
@@ -307,13 +308,17 @@ __global__ __launch_bounds__(1024) void k_hot_pick(const int2* __restrict__ cand
       const uint32_t idx = atomicAdd(&s_cnt, 1u);
       hot_keys[idx] = c.x;
       s_hot[idx] = c.x;
+      atomicAdd(&s_cov, L);
     }
   }
   __syncthreads();
   const uint32_t nh = s_cnt;
   hot_build(s_hot, nh, cell, t_key, t_idx);
   for (int b = threadIdx.x; b < HOT_TBL; b += blockDim.x) table[b] = make_int2(t_key[b], t_idx[b]);
-  if (threadIdx.x == 0) *hot_n = nh;
+  if (threadIdx.x == 0) {
+    hot_n[0] = nh;
+    hot_n[1] = s_cov;
+  }
 }
 
 __device__ __forceinline__ void slot_add(GroupSlot* p, int32_t x) {

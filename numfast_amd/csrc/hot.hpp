@@ -18,6 +18,7 @@ namespace nfh {
 constexpr int HOT_MAX = 512;   // hot keys privatised per workgroup
 constexpr int HOT_TBL = 1024;  // cuckoo slots
 constexpr int HOT_TBL_BITS = 10;
+constexpr int HOT_SAMPLE = 1 << 16;  // sampled keys the hot set is chosen from
 
 __host__ __device__ __forceinline__ uint32_t hot_mix(int32_t k) {
   uint32_t h = uint32_t(k);

@@ -258,6 +258,14 @@ ProfScope::~ProfScope() {
   }
 }
 
+void prof_add_bytes(const char* name, double bytes) {
+  for (auto it = t_pending.rbegin(); it != t_pending.rend(); ++it)
+    if (std::strcmp(it->name, name) == 0) {
+      it->bytes += bytes;
+      return;
+    }
+}
+
 void prof_collect() {
   if (t_pending.empty()) return;
   std::vector<Pending> p;

@@ -174,6 +174,9 @@ class ProfScope {
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 void prof_collect();
+// Adds `bytes` to the algorithmic bytes of this thread's latest pending
+// ProfScope named `name` (a launch whose byte count was known only after it).
+void prof_add_bytes(const char* name, double bytes);
 // While one lives on this thread, ProfScope creates no events (entry points
 // that must not allocate run under it).
 class ProfSuspend {

@@ -105,6 +105,75 @@ def test_partitioned_bucket_over_2p24_records(hip):
         S.nfh_set_groupby_strategy(0)
 
 
+def zipf_keys(rng, n, ids, s):
+    """Bounded Zipf(s) over ranks 1..ids (inverse CDF), through a seeded
+    permutation of [0, ids)."""
+    cdf = np.cumsum(np.arange(1, ids + 1, dtype=np.float64) ** -s)
+    r = np.searchsorted(cdf / cdf[-1], rng.random(n), side="right")
+    return rng.permutation(ids).astype(np.int32)[np.minimum(r, ids - 1)]
+
+
+def _hist_bytes(L, fn):
+    """Runs fn() with kernel timing on; returns (fn's result, the algorithmic
+    bytes the partitioned path's histogram pass recorded)."""
+    import ctypes
+    L.nfh_profile_enable(1)
+    L.nfh_profile_reset()
+    try:
+        res = fn()
+        ms, cnt, by = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
+        assert L.nfh_profile_get(b"gp_hist", ctypes.byref(ms), ctypes.byref(cnt)) == 0
+        assert L.nfh_profile_bytes(b"gp_hist", ctypes.byref(by)) == 0
+        return res, by.value
+    finally:
+        L.nfh_profile_enable(0)
+        L.nfh_profile_reset()
+
+
+@pytest.mark.parametrize("case", ["zipf_tail_row", "zipf_nulls", "all_hot", "wide_values",
+                                  "uniform"])
+def test_partitioned_record_mode(hip, case):
+    """The histogram pass writes the records itself when the sampled hot
+    keys carry at least 40% of the rows (one region of rows per workgroup,
+    the first partition pass reading the records): a row count whose last
+    regions are empty and whose last region ends off a quad, 40% NULL rows,
+    every row hot (no records), values outside 16 bits (unpacked records
+    after the record mode), and uniform keys (no record mode), for both
+    partition counts."""
+    S = native.lib("selector")
+    L = native.lib("hip")
+    rng = np.random.default_rng(21)
+    n = {"zipf_tail_row": (1 << 20) + 1, "zipf_nulls": (1 << 21) + 6}.get(case, 1 << 20)
+    vals = rng.integers(-(2 ** 15), 2 ** 15, n).astype(np.int32)
+    valid = rng.random(n) > 0.05
+    if case in ("zipf_tail_row", "wide_values"):
+        keys = (zipf_keys(rng, n, 1 << 16, 1.1) * 7).astype(np.int32)
+    elif case == "zipf_nulls":
+        keys = zipf_keys(rng, n, 1 << 20, 1.1) - (1 << 19)
+        valid = rng.random(n) > 0.4
+    elif case == "all_hot":  # 200 keys spread over a 2^20 span, all sampled hot
+        ids = rng.choice(1 << 20, 200, replace=False).astype(np.int32)
+        keys = ids[rng.integers(0, 200, n)]
+    else:
+        keys = rng.integers(0, 1 << 20, n).astype(np.int32)
+    if case == "wide_values":
+        vals[::31] = rng.integers(-(2 ** 31), 2 ** 31 - 1, vals[::31].size, dtype=np.int64)
+    ref = ox.group_agg(keys, vals, valid)
+    try:
+        for st in (2, 5):
+            S.nfh_set_groupby_strategy(st)
+            got, by = _hist_bytes(L, lambda: native.group_agg(keys, vals, valid))
+            _check(got, ref)
+            assert S.nfh_groupby_last_strategy() == st
+            # 9 B per row, plus 8 B per record written in record mode
+            if case == "uniform" or case == "all_hot":
+                assert by == 9.0 * n
+            else:
+                assert by > 9.0 * n
+    finally:
+        S.nfh_set_groupby_strategy(0)
+
+
 def test_hash_strategy_overflow_falls_back(hip):
     """Distinct keys beyond the hash table's room: the overflow is detected
     and the sort-based path gives the result."""
