@@ -108,7 +108,10 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   bool oor = false;  // a record value outside [-2^15, 2^15) (mode bit 0)
   // chain of row i's tile in the first partition pass
   auto chain_of_row = [&](uint32_t i) { return chain_of_tile(ptiles, i / uint32_t(GP_TILE)); };
-  // true when the row is a record (valid and not hot)
+  const bool reg = rec_o != nullptr && table != nullptr && hot_n[1] >= reg_min;
+  // true when the row is a record (valid and not hot); in record mode the
+  // per-chain digit counts come from the bucket counts at the end (one
+  // chain per workgroup)
   auto row = [&](int32_t k, int32_t x, bool ok, uint32_t ch) -> bool {
     if (!ok) return false;
     const int hi = table ? hot_index(t_key, t_idx, k) : -1;
@@ -127,10 +130,9 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
     const uint32_t b = uint32_t(k - kmin) >> bbits;
     oor |= x < -32768 || x >= 32768;
     atomicAdd(&s_h[b], 1u);
-    atomicAdd(&s_c0[ch][b & 127u], 1u);
+    if (!reg) atomicAdd(&s_c0[ch][b & 127u], 1u);
     return true;
   };
-  const bool reg = rec_o != nullptr && table != nullptr && hot_n[1] >= reg_min;
   if (reg) {
     // record mode: one region per workgroup (the host guarantees
     // gridDim.x * rsize >= n, rsize % 4 == 0 and aligned quads)
@@ -184,13 +186,50 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
       }
       asm volatile("" ::: "memory");  // keep every load above the adds
       uint32_t cold = 0;
+      // two quads at a time: their eight hot lookups, then the current
+      // min / max of the hot rows' keys, are read in one batch each
 #pragma unroll
-      for (int u = 0; u < RQ; ++u) {
-        if (!in[u]) continue;
-        cold |= uint32_t(row(ka[u].x, va[u].x, ma[u] & 0xFFu, ch)) << (4 * u);
-        cold |= uint32_t(row(ka[u].y, va[u].y, ma[u] & 0xFF00u, ch)) << (4 * u + 1);
-        cold |= uint32_t(row(ka[u].z, va[u].z, ma[u] & 0xFF0000u, ch)) << (4 * u + 2);
-        cold |= uint32_t(row(ka[u].w, va[u].w, ma[u] & 0xFF000000u, ch)) << (4 * u + 3);
+      for (int u2 = 0; u2 < RQ; u2 += 2) {
+        int32_t k8[8], x8[8];
+        int h8[8];
+        bool ok8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int u = u2 + (e >> 2), r = e & 3;
+          k8[e] = r == 0 ? ka[u].x : r == 1 ? ka[u].y : r == 2 ? ka[u].z : ka[u].w;
+          x8[e] = r == 0 ? va[u].x : r == 1 ? va[u].y : r == 2 ? va[u].z : va[u].w;
+          ok8[e] = in[u] && ((ma[u] >> (8 * r)) & 0xFFu) != 0u;
+        }
+        hot_index_batch<8>(t_key, t_idx, k8, h8);
+        int32_t mn8[8], mx8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int h = h8[e] >= 0 ? h8[e] : 0;
+          mn8[e] = h_mn[h];
+          mx8[e] = h_mx[h];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          if (!ok8[e]) continue;
+          const int32_t k = k8[e], x = x8[e];
+          const int hi = h8[e];
+          if (hi >= 0) {
+            if (x >= -32768 && x < 32768) {
+              atomicAdd(&h_pk[hi], (u64(1) << 40) | u64(uint32_t(x + 32768)));
+            } else {
+              atomicAdd(&h_sum[hi], (u64)(long long)x);
+              atomicAdd(&h_cnt[hi], 1u);
+            }
+            // min / max only when x would change them (see k_gp_bucket_emit)
+            if (x < mn8[e]) atomicMin(&h_mn[hi], x);
+            if (x > mx8[e]) atomicMax(&h_mx[hi], x);
+          } else {
+            const uint32_t b = uint32_t(k - kmin) >> bbits;
+            oor |= x < -32768 || x >= 32768;
+            atomicAdd(&s_h[b], 1u);
+            cold |= 1u << (4 * u2 + e);
+          }
+        }
       }
       emit(cold, ka, va, RQ);
     }
@@ -258,9 +297,19 @@ __global__ __launch_bounds__(GP_HBLOCK) void k_gp_hist(
   }
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     if (s_h[b]) atomicAdd(&hist[b], s_h[b]);
-  for (uint32_t b = threadIdx.x; b < NCHAIN * 128; b += blockDim.x) {
-    const uint32_t c = (&s_c0[0][0])[b];
-    if (c) atomicAdd(&chist[b], c);
+  if (reg) {
+    // the first partition pass's digit is the bucket id's low 7 bits
+    const uint32_t ch = blockIdx.x / (gridDim.x / NCHAIN);
+    for (uint32_t d = threadIdx.x; d < 128; d += blockDim.x) {
+      uint32_t c = 0;
+      for (uint32_t b = d; b < nb; b += 128) c += s_h[b];
+      if (c) atomicAdd(&chist[ch * 128 + d], c);
+    }
+  } else {
+    for (uint32_t b = threadIdx.x; b < NCHAIN * 128; b += blockDim.x) {
+      const uint32_t c = (&s_c0[0][0])[b];
+      if (c) atomicAdd(&chist[b], c);
+    }
   }
   if (table) {
     for (int b = threadIdx.x; b < HOT_MAX; b += blockDim.x) {

@@ -48,6 +48,41 @@ __device__ __forceinline__ int hot_index(const int32_t* t_key, const int32_t* t_
   return s >= 0 ? t_idx[s] : -1;
 }
 
+// Hot indices of N keys at once (-1: not hot).  All 3N key reads are
+// issued before any compare and the N index reads after them: written row
+// by row, the compiler moves the second and third reads behind the first
+// compare, and each row then waits for its own LDS reads in turn.
+template <int N>
+__device__ __forceinline__ void hot_index_batch(const int32_t* t_key, const int32_t* t_idx,
+                                                const int32_t (&k)[N], int (&hi)[N]) {
+  uint32_t sl[N][3];
+  int32_t tk[N][3];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t h = hot_mix(k[i]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sl[i][c] = hot_slot(h, c);
+      tk[i][c] = t_key[sl[i][c]];
+    }
+  }
+  // every read above is complete here (each value is an operand)
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(tk[i][c]));
+  int s[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    s[i] = tk[i][0] == k[i] ? int(sl[i][0])
+         : tk[i][1] == k[i] ? int(sl[i][1])
+         : tk[i][2] == k[i] ? int(sl[i][2]) : -1;
+    hi[i] = t_idx[s[i] >= 0 ? s[i] : 0];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) hi[i] = s[i] >= 0 ? hi[i] : -1;
+}
+
 // Copy of the global table into LDS.  A null table (no hot keys) loads
 // key 0 / index -1 everywhere, which a lookup of key 0 would match: callers
 // make no lookups without a table.  t_idx may be null (membership only).
