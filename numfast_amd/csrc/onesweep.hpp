@@ -314,8 +314,12 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
   static_assert((1 << TB) == SORT_TILE, "tile-local row bits");
 
   extern __shared__ __align__(16) unsigned char s_raw[];
+  // 32-bit words: word and row id side by side (one 64-bit LDS access per
+  // item in the reorder and the write-out); 64-bit words: two arrays
+  constexpr bool IL = sizeof(W) == 4;
   W* s_w = reinterpret_cast<W*>(s_raw);
   uint32_t* s_i = reinterpret_cast<uint32_t*>(s_w + SORT_TILE);
+  uint2* s_wi = reinterpret_cast<uint2*>(s_raw);
   uint32_t(*s_wh)[256] = reinterpret_cast<uint32_t(*)[256]>(s_i + SORT_TILE);
   uint32_t* s_tstart = &s_wh[SORT_WAVES][0];
   uint32_t* s_gbase = s_tstart + 256;
@@ -444,13 +448,19 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
         if (FULL || pos < n) {
           const uint32_t d = PK ? ((rk[j] >> dg.shift) & 255u) : (rk[j] & 255u);
           const uint32_t p = s_tstart[d] + s_wh[wave][d] + (rk[j] >> RS);
-          s_w[p] = kw[j];
+          uint32_t xr;
           if constexpr (PK)
-            s_i[p] = ((rk[j] & 0xFFFFu) << TB) | (pos - base);
+            xr = ((rk[j] & 0xFFFFu) << TB) | (pos - base);
           else if constexpr (FIRST)
-            s_i[p] = pos;  // the first pass's rows are the positions
+            xr = pos;  // the first pass's rows are the positions
           else
-            s_i[p] = ix[j];
+            xr = ix[j];
+          if constexpr (IL) {
+            s_wi[p] = make_uint2(uint32_t(kw[j]), xr);
+          } else {
+            s_w[p] = kw[j];
+            s_i[p] = xr;
+          }
         }
       }
 
@@ -497,8 +507,14 @@ __global__ __launch_bounds__(SortGeom<K>::BLOCK) void k_onesweep(
           const uint32_t i = i0 + uint32_t(u * SORT_BLOCK + tid);
           ok[u] = FULL || i < nvalid;
           const uint32_t ic = ok[u] ? i : 0u;
-          w[u] = s_w[ic];
-          xi[u] = s_i[ic];
+          if constexpr (IL) {
+            const uint2 e = s_wi[ic];
+            w[u] = W(e.x);
+            xi[u] = e.y;
+          } else {
+            w[u] = s_w[ic];
+            xi[u] = s_i[ic];
+          }
         }
 #pragma unroll
         for (int u = 0; u < WU; ++u) {
