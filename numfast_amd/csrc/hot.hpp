@@ -20,12 +20,20 @@ constexpr int HOT_TBL = 1024;  // cuckoo slots
 constexpr int HOT_TBL_BITS = 10;
 constexpr int HOT_SAMPLE = 1 << 16;  // sampled keys the hot set is chosen from
 
+// Key hash: xor-shifts around two 24-bit multiplies (full-rate
+// v_mul_u32_u24 on the GPU; a 32-bit multiply issues at a quarter of the
+// rate, and every row of the hot-key kernels hashes its key).  The first
+// shift folds the key's top byte into the multiplied low 24 bits, the last
+// brings the product's high bits into the low slot fields.
+__host__ __device__ __forceinline__ uint32_t hot_mul24(uint32_t a, uint32_t b) {
+  return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+}
 __host__ __device__ __forceinline__ uint32_t hot_mix(int32_t k) {
   uint32_t h = uint32_t(k);
   h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  h *= 0xC2B2AE35u;
+  h = hot_mul24(h, 0x9E3779u);
+  h ^= h >> 15;
+  h = hot_mul24(h, 0x85EBCAu);
   h ^= h >> 16;
   return h;
 }
