@@ -170,6 +170,10 @@ def test_partitioned_record_mode(hip, case):
                 assert by == 9.0 * n
             else:
                 assert by > 9.0 * n
+            # without a validity mask (every row valid: 8 B per row)
+            got, by = _hist_bytes(L, lambda: native.group_agg(keys, vals))
+            _check(got, ox.group_agg(keys, vals))
+            assert by >= 8.0 * n
     finally:
         S.nfh_set_groupby_strategy(0)
 
