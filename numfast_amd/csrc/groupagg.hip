@@ -980,18 +980,28 @@ __global__ __launch_bounds__(UM_B_THREADS) void k_umark_bits(
       }
   }
   const uint32_t lane = uint32_t(lane_id());
-  for (uint32_t t = g + G * uint32_t(wave_id()); t < ntiles; t += G * NW) {
-    const uint2 e = entry[size_t(r) * ntiles + t];
-    for (uint32_t j = lane; j < e.y; j += 4 * WAVE) {
-      uint32_t o[4];
+  // The wave's entries are t0, t0 + stride, ...: 64 of them are loaded at
+  // once (one per lane) and handed out by shuffles, so only the records'
+  // loads wait on memory in the loop below.
+  const uint32_t t0 = g + G * uint32_t(wave_id()), stride = G * NW;
+  for (uint32_t kb = 0; t0 + kb * stride < ntiles; kb += WAVE) {  // uniform over the wave
+    const uint32_t tl = t0 + (kb + lane) * stride;
+    const uint2 el = tl < ntiles ? entry[size_t(r) * ntiles + tl] : make_uint2(0u, 0u);
+    const uint32_t kn = min(uint32_t(WAVE), (ntiles - t0 - kb * stride + stride - 1) / stride);
+    for (uint32_t k = 0; k < kn; ++k) {
+      const uint32_t ex = uint32_t(__shfl(int(el.x), int(k)));
+      const uint32_t ey = uint32_t(__shfl(int(el.y), int(k)));
+      for (uint32_t j = lane; j < ey; j += 4 * WAVE) {
+        uint32_t o[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t jj = j + uint32_t(WAVE) * u;
-        o[u] = jj < e.y ? __builtin_nontemporal_load(rec + e.x + jj) : 0xFFFFFFFFu;
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t jj = j + uint32_t(WAVE) * u;
+          o[u] = jj < ey ? __builtin_nontemporal_load(rec + ex + jj) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (o[u] != 0xFFFFFFFFu) atomicOr(&s_bits[(o[u] & LOW) >> 5], 1u << (o[u] & 31u));
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (o[u] != 0xFFFFFFFFu) atomicOr(&s_bits[(o[u] & LOW) >> 5], 1u << (o[u] & 31u));
     }
   }
   __syncthreads();
